@@ -1,0 +1,78 @@
+/*
+ * CPU restatement of the ASTR per-step hot path (test infrastructure only).
+ *
+ * This library is the oracle that the HIP path is checked against.  It is
+ * never linked into, loaded by or called from the product library
+ * (astr_amd/csrc); only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg use it, and only as a checker.
+ *
+ * Every routine follows a named piece of the reference and cites it
+ * (file:line relative to the astr-code/astr tree).  Pinning of each routine
+ * (what checks it beyond itself) is listed in the header comment of
+ * astr_oracle.c.
+ */
+#ifndef ASTR_ORACLE_H
+#define ASTR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OR_HM 5 /* halo depth, src/commvar.F90:184 */
+
+typedef struct or_cfg {
+  int im, jm, km;          /* computed lattice 0..im etc.; km==0 means 2-D */
+  double alfa_filter;      /* 0.49 */
+  double beta_halo;        /* 1.11 */
+  double beta_bound;       /* 1.09 */
+  double gamma, mach, reynolds, prandtl, ref_tem;
+  double suth;             /* Sutherland constant in ref_tem units: 110.4/ref_tem */
+  double dx, dy, dz;
+  double dt;
+} or_cfg;
+
+/* ---- line operators (one line of f(-hm:n+hm), stored f[t+hm]) ---- */
+void or_thomas_prep(int n, const double *a, const double *c, double *ac);
+void or_thomas_solve(int n, const double *ac, double *d, double *x);
+int  or_deriv_nrows(int n);
+void or_deriv_lhs(int n, double *a, double *c);
+void or_deriv_rhs(int n, const double *f, double *d);
+void or_df_compact(int n, const double *f, double *df);
+int  or_filter_nrows(int n);
+void or_filter_coefs(double alfa, double beta_halo, double beta_bound,
+                     double *coef10i, double *coefh);
+void or_filter_lhs(int n, double alfa, double beta_halo, double *a, double *c);
+void or_filter_rhs(int n, const double *coef10i, const double *coefh,
+                   const double *f, double *d);
+void or_compact_filter(int n, double alfa, double beta_halo, double beta_bound,
+                       const double *f, double *ff);
+
+/* ---- field state: q[5][N], prims rho,u,v,w,p,T each [N]; N=(im+1)(jm+1)(km+1),
+ *      index i + (im+1)*(j + (jm+1)*k) ---- */
+typedef struct or_state or_state;
+or_state *or_create(const or_cfg *cfg);
+void or_destroy(or_state *s);
+long or_npoints(const or_state *s);
+double *or_q(or_state *s);       /* 5*N */
+double *or_prims(or_state *s);   /* 6*N: rho,u,v,w,p,T */
+double *or_qrhs(or_state *s);    /* 5*N */
+double *or_dvel(or_state *s);    /* 9*N, component m, direction d at (3*d+m)... see .c */
+double *or_dtmp(or_state *s);    /* 3*N */
+
+void or_tgv_init(or_state *s, double noise_amp, uint64_t seed);
+void or_set_q(or_state *s, const double *q);      /* q -> prims by q2fvar */
+void or_q2fvar_all(or_state *s);
+void or_rhscal(or_state *s);
+void or_filterq(or_state *s);
+void or_stacal(or_state *s, double *ke, double *ens);
+void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens);
+
+/* counter-based noise shared with the device path (not a reference routine) */
+double or_gauss(uint64_t seed, uint32_t i, uint32_t j, uint32_t k, uint32_t c);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
