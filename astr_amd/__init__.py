@@ -1,0 +1,196 @@
+"""astr_amd -- MI355X-native per-step hot path of the ASTR compact-scheme solver.
+
+The compute lives in astr_amd/libastr_hip.so (HIP for gfx950, C ABI declared
+in include/astr_hip.h).  This module is a thin ctypes host over that ABI,
+used by tests and bench.py; the Fortran host binding is
+astr_amd/fortran/astr_hip_mod.F90.  There is no CPU fallback: if the
+library is missing or no GPU is visible, every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("ASTR_HIP_LIB", os.path.join(PKG, "libastr_hip.so"))
+SEM_MINIAPP = 0
+
+_dp = C.POINTER(C.c_double)
+
+
+class AstrError(RuntimeError):
+    pass
+
+
+class Config(C.Structure):
+    """astr_hip_config (include/astr_hip.h)."""
+
+    _fields_ = [
+        ("im", C.c_int), ("jm", C.c_int), ("km", C.c_int), ("semantics", C.c_int),
+        ("alfa_filter", C.c_double), ("beta_halo", C.c_double), ("beta_bound", C.c_double),
+        ("gamma", C.c_double), ("mach", C.c_double), ("reynolds", C.c_double),
+        ("prandtl", C.c_double), ("ref_tem", C.c_double), ("sutherland", C.c_double),
+        ("lx", C.c_double), ("ly", C.c_double), ("lz", C.c_double), ("deltat", C.c_double),
+    ]
+
+
+class Layout(C.Structure):
+    _fields_ = [("pitch", C.c_int), ("plane", C.c_longlong), ("field_stride", C.c_longlong),
+                ("q", C.c_void_p), ("qrhs", C.c_void_p)]
+
+
+EXPORTS = [
+    "astr_hip_last_error", "astr_hip_create", "astr_hip_destroy", "astr_hip_set_stream",
+    "astr_hip_set_deltat", "astr_hip_field_info", "astr_hip_init_tgv", "astr_hip_upload",
+    "astr_hip_download", "astr_hip_filterq", "astr_hip_gradcal", "astr_hip_rhscal",
+    "astr_hip_rk3_steps", "astr_hip_sync", "astr_hip_line_batch",
+    "astr_hip_profile_begin", "astr_hip_profile_end",
+]
+PROF_SLOTS = 21
+PASS_NAMES = ["gc_first", "gc_mid", "gc_last", "visc_mid", "visc_last_rhs",
+              "visc_last_rk3_filter", "filter"]
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AstrError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        L.astr_hip_last_error.restype = C.c_char_p
+        L.astr_hip_create.argtypes = [C.POINTER(Config), C.POINTER(C.c_void_p)]
+        L.astr_hip_destroy.argtypes = [C.c_void_p]
+        L.astr_hip_set_stream.argtypes = [C.c_void_p, C.c_void_p]
+        L.astr_hip_set_deltat.argtypes = [C.c_void_p, C.c_double]
+        L.astr_hip_field_info.argtypes = [C.c_void_p, C.POINTER(Layout)]
+        L.astr_hip_init_tgv.argtypes = [C.c_void_p, C.c_double, C.c_ulonglong]
+        L.astr_hip_upload.argtypes = [C.c_void_p, _dp, C.c_int]
+        L.astr_hip_download.argtypes = [C.c_void_p, _dp, _dp, C.c_int]
+        L.astr_hip_filterq.argtypes = [C.c_void_p]
+        L.astr_hip_gradcal.argtypes = [C.c_void_p, _dp, _dp]
+        L.astr_hip_rhscal.argtypes = [C.c_void_p, _dp]
+        L.astr_hip_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.astr_hip_sync.argtypes = [C.c_void_p]
+        L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
+                                          C.c_double, C.c_double]
+        L.astr_hip_profile_begin.argtypes = [C.c_void_p, C.c_int]
+        L.astr_hip_profile_end.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _chk(rc):
+    if rc != 0:
+        raise AstrError(lib().astr_hip_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def tgv_config(im, jm=None, km=None, deltat=1e-3, reynolds=1600.0, mach=0.1,
+               sutherland=110.4, ref_tem=273.15, lx=2 * math.pi, ly=None, lz=None):
+    """Configuration of the reference's TGV miniapp (tgvsolver.F90:152-261)."""
+    jm = im if jm is None else jm
+    km = im if km is None else km
+    return Config(im, jm, km, SEM_MINIAPP, 0.49, 1.11, 1.09, 1.4, mach, reynolds, 0.72,
+                  ref_tem, sutherland, lx, lx if ly is None else ly,
+                  lx if lz is None else lz, deltat)
+
+
+class Flow:
+    """One device-resident case.  Method names follow the reference's
+    subroutines (tgvsolver.F90): filterq, gradcal, rhscal, rk3/steploop."""
+
+    def __init__(self, cfg: Config):
+        self.cfg = cfg
+        h = C.c_void_p()
+        _chk(lib().astr_hip_create(C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.shape = (cfg.km + 1, cfg.jm + 1, cfg.im + 1)
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().astr_hip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state ---------------------------------------------------------
+    def init_tgv(self, noise=0.0, seed=0):
+        _chk(lib().astr_hip_init_tgv(self.h, noise, seed))
+
+    def upload(self, q):
+        """q: (5, km+1, jm+1, im+1) C-order == q(0:im,0:jm,0:km,1:5) Fortran."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != (5,) + self.shape:
+            raise AstrError(f"q must have shape {(5,) + self.shape}")
+        _chk(lib().astr_hip_upload(self.h, _p(q), 0))
+
+    def download(self, prims=False):
+        q = np.empty((5,) + self.shape)
+        pr = np.empty((6,) + self.shape) if prims else None
+        _chk(lib().astr_hip_download(self.h, _p(q), _p(pr), 0))
+        return (q, pr) if prims else q
+
+    def set_deltat(self, dt):
+        _chk(lib().astr_hip_set_deltat(self.h, dt))
+
+    # -- reference subroutines ----------------------------------------
+    def filterq(self):
+        _chk(lib().astr_hip_filterq(self.h))
+
+    def gradcal(self):
+        dvel = np.empty((9,) + self.shape)
+        dtmp = np.empty((3,) + self.shape)
+        _chk(lib().astr_hip_gradcal(self.h, _p(dvel), _p(dtmp)))
+        return dvel, dtmp
+
+    def rhscal(self):
+        qrhs = np.empty((5,) + self.shape)
+        _chk(lib().astr_hip_rhscal(self.h, _p(qrhs)))
+        return qrhs
+
+    def rk3(self, nsteps, stats=True):
+        ke = np.empty(nsteps) if stats else None
+        en = np.empty(nsteps) if stats else None
+        _chk(lib().astr_hip_rk3_steps(self.h, nsteps, _p(ke), _p(en)))
+        return (ke, en) if stats else None
+
+    steploop = rk3
+
+    def profile_begin(self, max_launches):
+        _chk(lib().astr_hip_profile_begin(self.h, max_launches))
+
+    def profile_end(self):
+        """{(pass_name, dir): (total_ms, launches)} of the profiled launches."""
+        ms = np.zeros(PROF_SLOTS)
+        cnt = (C.c_int * PROF_SLOTS)()
+        _chk(lib().astr_hip_profile_end(self.h, _p(ms), cnt))
+        out = {}
+        for k in range(PROF_SLOTS):
+            if cnt[k]:
+                out[(PASS_NAMES[k // 3], "xyz"[k % 3])] = (float(ms[k]), int(cnt[k]))
+        return out
+
+    def sync(self):
+        _chk(lib().astr_hip_sync(self.h))
+
+
+def line_batch(op, f, alfa_filter=0.49, beta_halo=1.11):
+    """Batched per-line df_compact (op 0) / compact_filter (op 1); f: (nlines, n+11)."""
+    f = np.ascontiguousarray(np.atleast_2d(f), dtype=np.float64)
+    n = f.shape[1] - 11
+    out = np.empty((f.shape[0], n + 1))
+    _chk(lib().astr_hip_line_batch(op, n, f.shape[0], _p(f), _p(out), alfa_filter, beta_halo))
+    return out

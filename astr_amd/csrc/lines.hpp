@@ -1,0 +1,311 @@
+// lines.hpp -- device engine for many independent compact line solves.
+//
+// A workgroup owns a tile of TL whole lines of one direction.  The tile's
+// points are spread over the NT threads for the pointwise work (NPT slots per
+// thread, coalesced along the fastest memory index), and the solve maps each
+// line onto P = NT/TL consecutive lanes of one wavefront, lane p owning the
+// chunk of R consecutive unknowns [p*R, p*R+R).
+//
+// The tridiagonal systems are the reference's (derivative.F90:63-158,
+// filter.F90:31-100) with the reference's LU factors ac1,ac2,ac3
+// (commfunc.F90:752-774).  The reference's Thomas recurrences
+//     d'(u) = d(u)*ac2(u) - d'(u-1)*ac3(u)        (forward)
+//     x(u)  = d'(u) - ac1(u)*x(u+1)                (backward)
+// (commfunc.F90:790-813) are first-order linear recurrences; each lane runs
+// them over its chunk with a zero carry, the carries across the P chunks are
+// composed by a Kogge-Stone scan over lanes (DPP row shifts inside each row
+// of 16 lanes, v_readlane across rows), and each lane then adds
+// carry * (running product of the recurrence coefficients).  This is the
+// same linear map as the sequential sweep, re-associated.
+//
+// LDS line layout: the input line holds rows -8..P*R+2 at offset 8, i.e. the
+// reference's halo f(-5:n+5) (written explicitly) plus zero padding, so every
+// chunk reads its stencil window at compile-time offsets.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace astr {
+
+// stencil coefficients of 643c (constdef.F90:17-19)
+constexpr double k7d9 = 7.0 / 9.0;
+constexpr double k1d36 = 1.0 / 36.0;
+constexpr double k1d60 = 1.0 / 60.0;
+
+constexpr int kLineOff = 8;  // LDS offset of row 0 in an input line
+
+// filter RHS coefficients, filled by the host from filter_coefficient_cal
+// (filter.F90:299-432): c10[0..5] interior rows by value, the halo rows'
+// coefh(k,j) as a device table ch[k*11+j], k=0..2.
+struct FilterCoef {
+  double c10[6];
+  const double* ch;
+};
+
+// ---------------------------------------------------------------------------
+// cross-lane helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int lo32(double v) { return (int)(__double_as_longlong(v) & 0xffffffffll); }
+__device__ __forceinline__ int hi32(double v) { return (int)(__double_as_longlong(v) >> 32); }
+__device__ __forceinline__ double mk64(int lo, int hi) {
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// DPP move inside rows of 16 lanes; lanes whose source is outside the row
+// receive `old`.
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v, double old) {
+  const int l = __builtin_amdgcn_update_dpp(lo32(old), lo32(v), CTRL, 0xf, 0xf, false);
+  const int h = __builtin_amdgcn_update_dpp(hi32(old), hi32(v), CTRL, 0xf, 0xf, false);
+  return mk64(l, h);
+}
+
+// DPP move inside rows of 16 lanes; lanes whose source is outside the row
+// receive 0.
+template <int CTRL>
+__device__ __forceinline__ double dpp0(double v) {
+  const int l = __builtin_amdgcn_mov_dpp(lo32(v), CTRL, 0xf, 0xf, true);
+  const int h = __builtin_amdgcn_mov_dpp(hi32(v), CTRL, 0xf, 0xf, true);
+  return mk64(l, h);
+}
+
+__device__ __forceinline__ double readlane(double v, int lane) {
+  return mk64(__builtin_amdgcn_readlane(lo32(v), lane), __builtin_amdgcn_readlane(hi32(v), lane));
+}
+
+// Inclusive scan of affine maps x -> A*x + B over segments of P lanes, in the
+// direction of increasing lane (FWD) or decreasing lane (!FWD).  On return
+// (A,B) of lane p composes the maps of lanes segstart..p (FWD) or p..segend.
+// The exclusive carry (B of the neighbour towards the segment start, 0 at
+// the segment boundary) is returned.
+template <int P, bool FWD>
+__device__ __forceinline__ double affine_scan(double& A, double& B) {
+  static_assert(P == 16 || P == 32 || P == 64, "segment must be 16, 32 or 64 lanes");
+  const int lane = threadIdx.x & 63;
+  const int row = lane >> 4, col = lane & 15;
+#define ASTR_SCAN_STEP(S)                                                  \
+  {                                                                        \
+    const double A2 = dpp<(FWD ? 0x110 : 0x100) + S>(A, 1.0);              \
+    const double B2 = dpp0<(FWD ? 0x110 : 0x100) + S>(B);                  \
+    B = fma(A, B2, B);                                                     \
+    A = A * A2;                                                            \
+  }
+  ASTR_SCAN_STEP(1)
+  ASTR_SCAN_STEP(2)
+  ASTR_SCAN_STEP(4)
+  ASTR_SCAN_STEP(8)
+#undef ASTR_SCAN_STEP
+  if constexpr (P >= 32) {
+    // pair rows (0,1) and (2,3): the receiving row composes the other's total
+    const double Aa = readlane(A, FWD ? 15 : 16), Ba = readlane(B, FWD ? 15 : 16);
+    const double Ab = readlane(A, FWD ? 47 : 48), Bb = readlane(B, FWD ? 47 : 48);
+    const bool recv_a = FWD ? row == 1 : row == 0;
+    const bool recv_b = FWD ? row == 3 : row == 2;
+    const double Ac = recv_a ? Aa : (recv_b ? Ab : 1.0);
+    const double Bc = recv_a ? Ba : (recv_b ? Bb : 0.0);
+    B = fma(A, Bc, B);
+    A = A * Ac;
+  }
+  if constexpr (P == 64) {
+    const double Aa = readlane(A, FWD ? 31 : 32), Ba = readlane(B, FWD ? 31 : 32);
+    const bool recv = FWD ? row >= 2 : row <= 1;
+    B = fma(A, recv ? Ba : 0.0, B);
+    A = A * (recv ? Aa : 1.0);
+  }
+  // exclusive carry: B of the neighbour (inside the row by DPP, across rows
+  // by readlane of the neighbouring row's edge lane)
+  double c = dpp0<FWD ? 0x111 : 0x101>(B);
+  const double e0 = readlane(B, FWD ? 15 : 16);
+  const double e1 = readlane(B, FWD ? 31 : 32);
+  const double e2 = readlane(B, FWD ? 47 : 48);
+  const bool edge = FWD ? col == 0 : col == 15;
+  if (edge) {
+    // forward: row r reads lane 16r-1 (rows 1..3 -> e0,e1,e2); backward:
+    // row r reads lane 16(r+1) (rows 0..2 -> e0,e1,e2); only inside a segment
+    const int nrow = FWD ? row - 1 : row + 1;
+    const bool inseg = nrow >= 0 && nrow <= 3 && (nrow / (P / 16)) == (row / (P / 16));
+    const int idx = FWD ? nrow : row;
+    const double ev = idx == 0 ? e0 : (idx == 1 ? e1 : e2);
+    c = inseg ? ev : 0.0;
+  }
+  return c;
+}
+
+// LU factors of one chunk, held in registers, with encodings that make the
+// recurrences branch-free: the first row, which the reference's solve leaves
+// as is, has c2=1, c3=0; the last row has c1=0; rows past the end have all
+// factors 0.  For rows with a = c = alpha the reference's factors
+// (commfunc.F90:752-774) satisfy ac1 = ac3 = alpha*ac2; the derivative's
+// first and last rows have a = c = 0, so there c1 = c3 (= 0) as well and one
+// array serves both (SAME13).
+template <int R, bool SAME13>
+struct ChunkCoef {
+  double c2[R];
+  double c1[R];
+  double c3[SAME13 ? 1 : R];
+  __device__ __forceinline__ double f3(int k) const { return SAME13 ? c1[k] : c3[k]; }
+};
+
+template <int R, bool SAME13>
+__device__ __forceinline__ void load_chunk_coef(ChunkCoef<R, SAME13>& C,
+                                                const double* __restrict__ ac, int M, int u0) {
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int u = u0 + k;
+    const bool ok = u < M;
+    const int uu = ok ? u : 0;
+    double a1 = ac[uu], a2 = ac[M + uu], a3 = ac[2 * M + uu];
+    if (u == 0) { a2 = 1.0; a3 = 0.0; }
+    if (u == M - 1) a1 = 0.0;
+    C.c2[k] = ok ? a2 : 0.0;
+    C.c1[k] = ok ? a1 : 0.0;
+    if constexpr (!SAME13) C.c3[k] = ok ? a3 : 0.0;
+  }
+}
+
+// Solve one line held in LDS.
+//   T  : input line, row r at T[kLineOff + r]; rows -5..n+5 hold the line
+//        and its halo, the padding rows hold 0
+//   X  : output, x(r) at X[r], r = 0..n
+// FILT=false: 643c derivative, unknowns -1..n+1 (M=n+3), explicit 6th-order
+//             rows at -1 and n+1 (derivative.F90:250-304).
+// FILT=true : 10th-order filter, unknowns -3..n+3 (M=n+7), halo rows
+//             -3..-1, n+1..n+3 from coefh (filter.F90:206-283).
+template <int P, int R, bool FILT>
+__device__ __forceinline__ void solve_line(const double* __restrict__ T, double* __restrict__ X,
+                                           int n, int p, const ChunkCoef<R, !FILT>& C,
+                                           const FilterCoef& fc) {
+  constexpr int OFF = FILT ? 3 : 1;
+  constexpr int H = FILT ? 5 : 3;
+  constexpr int W = R + 2 * H;
+  const int u0 = p * R;
+  const double* __restrict__ Tw = T + kLineOff + (u0 - OFF - H);
+
+  double w[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) w[k] = Tw[k];
+
+  double y[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const double* c = w + k + H;  // c[s] = f(r+s), r = u0+k-OFF
+    if (FILT) {
+      const double var0 = c[0] + c[0];
+      const double var1 = c[1] + c[-1];
+      const double var2 = c[2] + c[-2];
+      const double var3 = c[3] + c[-3];
+      const double var4 = c[4] + c[-4];
+      const double var5 = c[5] + c[-5];
+      y[k] = fc.c10[0] * var0 + fc.c10[1] * var1 + fc.c10[2] * var2 + fc.c10[3] * var3 +
+             fc.c10[4] * var4 + fc.c10[5] * var5;
+    } else {
+      const int r = u0 + k - OFF;
+      const double var1 = c[1] - c[-1];
+      const double var2 = c[2] - c[-2];
+      const double var3 = c[3] - c[-3];
+      const double dc = k7d9 * var1 + k1d36 * var2;
+      const double de = 0.75 * var1 - 0.15 * var2 + k1d60 * var3;
+      y[k] = (r == -1 || r == n + 1) ? de : dc;
+    }
+  }
+
+  if constexpr (FILT) {
+    // the six halo rows: lane q<6 of each line evaluates row q (-3,-2,-1,
+    // n+1,n+2,n+3) over f(-5..5) or f(n+5..n-5); the owners pick them up
+    // by readlane
+    double ev = 0.0;
+    if (p < 6) {
+      const int hr = p < 3 ? p : 5 - p;  // coefh row
+      const double* __restrict__ Tl = T + kLineOff;
+#pragma unroll
+      for (int j = 0; j <= 10; j++) {
+        const double f = p < 3 ? Tl[j - 5] : Tl[n + 5 - j];
+        ev = ev + fc.ch[hr * 11 + j] * f;
+      }
+    }
+    constexpr int NSEG = 64 / P;
+    const int myseg = (threadIdx.x & 63) / P;
+    double evs[NSEG][6];
+#pragma unroll
+    for (int s = 0; s < NSEG; s++)
+#pragma unroll
+      for (int q = 0; q < 6; q++) evs[s][q] = readlane(ev, s * P + q);
+    double mine[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      double v = evs[0][q];
+#pragma unroll
+      for (int s = 1; s < NSEG; s++) v = myseg == s ? evs[s][q] : v;
+      mine[q] = v;
+    }
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      const int r = u0 + k - OFF;
+      if (k < 3) {
+        y[k] = r == -3 ? mine[0] : (r == -2 ? mine[1] : (r == -1 ? mine[2] : y[k]));
+      }
+      const int q = r - n - 1;
+      y[k] = q == 0 ? mine[3] : (q == 1 ? mine[4] : (q == 2 ? mine[5] : y[k]));
+    }
+  }
+
+  // forward recurrence over the chunk with zero carry-in
+  double B = 0.0, A = 1.0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const double c3 = C.f3(k);
+    B = y[k] * C.c2[k] - B * c3;
+    A *= -c3;
+    y[k] = B;
+  }
+  const double cin = affine_scan<P, true>(A, B);
+  {
+    double m = 1.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      m *= -C.f3(k);
+      y[k] = fma(m, cin, y[k]);
+    }
+  }
+
+  // backward recurrence with zero carry from the right
+  B = 0.0;
+  A = 1.0;
+#pragma unroll
+  for (int k = R - 1; k >= 0; k--) {
+    const double c1 = C.c1[k];
+    B = y[k] - c1 * B;
+    A *= -c1;
+    y[k] = B;
+  }
+  const double xin = affine_scan<P, false>(A, B);
+  {
+    double g = 1.0;
+#pragma unroll
+    for (int k = R - 1; k >= 0; k--) {
+      g *= -C.c1[k];
+      y[k] = fma(g, xin, y[k]);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int r = u0 + k - OFF;
+    if (r >= 0 && r <= n) X[r] = y[k];
+  }
+}
+
+// Launch shape of a line kernel.
+template <int NT_, int TL_, int R_, int NPT_>
+struct LineCfg {
+  static constexpr int NT = NT_;   // threads per workgroup
+  static constexpr int TL = TL_;   // lines per tile
+  static constexpr int P = NT_ / TL_;  // lanes (chunks) per line
+  static constexpr int R = R_;     // unknowns per chunk
+  static constexpr int NPT = NPT_; // point slots per thread
+  static constexpr int LT = (P * R_ + 11) | 1;  // input line stride in LDS (rows -8..P*R+2)
+  static_assert(P == 16 || P == 32 || P == 64, "a line's chunks must fill 16, 32 or 64 lanes");
+  static_assert((TL_ & (TL_ - 1)) == 0, "TL must be a power of two");
+  static_assert(R_ >= 3, "chunk 0 must hold the three filter halo rows");
+};
+
+}  // namespace astr

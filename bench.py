@@ -1,0 +1,173 @@
+#!/usr/bin/env python3
+"""Headline benchmark: TGV 256^3 FP64 RK3 steps on one MI355X.
+
+Workload (one GPU): the reference's Taylor-Green vortex miniapp
+(miniapps/tgv_solver_3d/tgvsolver.F90) at 256^3, Re=1600, Ma=0.1, scheme
+643c + 10th-order compact filter (alpha 0.49), RK3, dt=5e-4, velocity
+perturbed by seeded 1e-3 Gaussian noise.  A step = 3 RK stages over the
+257^3 computed lattice with all fields resident in HBM.
+
+Prints one JSON line: value = Mpt-updates/s (lattice points x RK stages per
+second), roofline of the dominant kernel from HIP events recorded around
+every pass launch inside the timed region, and the reference Fortran timed
+on this host's cores as cpu_baseline.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "TGV 256³ FP64 Mpt-updates/s + % HBM peak; batched/served cases/s, p50/p99"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic FP64 words per lattice point moved by the reference operators
+# each pass fuses (one streaming sweep per operator, reads + writes):
+#   gc_first  : gradients x (4+4) + convective x (7+5)                 = 20
+#   gc_mid    : gradients y (4+4) + convective y (12+5)                = 25
+#   gc_last   : gradients z (4+4) + convective z (12+5)
+#               + pointwise stress/heat flux (16+9)                    = 50
+#   visc_mid  : viscous derivative (4+4 read, 4 write)                 = 12
+#   visc_last_rk3_filter: viscous z (12) + RK3 update (15+5)
+#               + filter z (5+5)                                       = 42
+#   filter    : one filter sweep (5+5)                                 = 10
+# Summed over a 3-D stage: 20+25+50+12+12+42+10+10 = 181, plus the
+# reference's updatefvar (5+6 = 11) that the passes make unnecessary:
+# 192 words = 1536 B per point per stage.
+PASS_WORDS = {"gc_first": 20, "gc_mid": 25, "gc_last": 50, "visc_mid": 12,
+              "visc_last_rhs": 12, "visc_last_rk3_filter": 42, "filter": 10}
+STAGE_BYTES = 1536
+
+
+def cpu_baseline(seconds_target=12.0):
+    """The reference's own Fortran (oracle/_ref, single thread) on a bounded
+    64^3 sample; falls back to the C restatement if that build is absent."""
+    from oracle import oracle as O
+
+    n = 64
+    pts = (n + 1) ** 3
+    if O.ref_available():
+        try:
+            steps = 2
+            r = O.run_ref("rk3", dims=[n, n, n], dt=5e-4, nsteps=steps)
+            per = float(r["seconds"]) / steps
+            steps = max(2, int(seconds_target / max(per, 1e-3)))
+            r = O.run_ref("rk3", dims=[n, n, n], dt=5e-4, nsteps=steps)
+            sec = float(r["seconds"])
+            return {"value": pts * 3 * steps / sec / 1e6, "unit": "Mpt-updates/s", "cores": 1,
+                    "kind": "reference",
+                    "sample": f"reference tgvsolver.F90 rk3 (flang -O3, 1 thread), TGV {n}^3, "
+                              f"{steps} steps, {sec:.1f} s"}
+        except Exception as e:  # pragma: no cover - reported, not fatal
+            sys.stderr.write(f"reference baseline failed: {e}\n")
+    s = O.OracleState(O.tgv_cfg(n, dt=5e-4))
+    s.tgv_init(1e-3, 1)
+    t0 = time.perf_counter()
+    s.rk3(1)
+    per = time.perf_counter() - t0
+    steps = max(1, int(seconds_target / max(per, 1e-3)))
+    t0 = time.perf_counter()
+    s.rk3(steps)
+    sec = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": pts * 3 * steps / sec / 1e6, "unit": "Mpt-updates/s", "cores": cores,
+            "kind": "port", "sample": f"C restatement, TGV {n}^3, {steps} steps, {sec:.1f} s"}
+
+
+def load_traffic(n):
+    path = os.path.join(ROOT, "profiles", f"traffic_tgv{n}.json")
+    if os.path.exists(path):
+        with open(path) as fh:
+            return json.load(fh)
+    return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: the whole {args.n}^3 domain sits in one MI355X's HBM; "
+              f"--gpus {args.gpus} is not supported", file=sys.stderr)
+        sys.exit(2)
+
+    import astr_amd as A
+
+    n = args.n
+    dt = 5e-4 * 256.0 / n
+    f = A.Flow(A.tgv_config(n, deltat=dt))
+    f.init_tgv(1e-3, 2024)
+    if args.warmup:
+        f.rk3(args.warmup, stats=True)
+    f.sync()
+
+    launches_per_step = 3 * 8
+    f.profile_begin(launches_per_step * args.steps + 8)
+    f.sync()
+    t0 = time.perf_counter()
+    ke, en = f.rk3(args.steps, stats=True)
+    f.sync()
+    wall = time.perf_counter() - t0
+    prof = f.profile_end()
+
+    if not all(math.isfinite(x) for x in list(ke) + list(en)):
+        raise SystemExit("non-finite kinetic energy/enstrophy: run is broken")
+
+    pts = (n + 1) ** 3
+    ms_step = wall / args.steps * 1e3
+    value = pts * 3 * args.steps / wall / 1e6
+
+    # dominant kernel by total time
+    (pname, pdir), (ms_tot, cnt) = max(prof.items(), key=lambda kv: kv[1][0])
+    avg_s = ms_tot / cnt / 1e3
+    alg_bytes = PASS_WORDS[pname] * 8 * pts
+    achieved = alg_bytes / avg_s / 1e9
+    tr = load_traffic(n)
+    traffic = None
+    if tr and f"{pname}_{pdir}" in tr.get("kernels", {}):
+        traffic = tr["kernels"][f"{pname}_{pdir}"]["bytes_per_launch"]
+    step_alg = STAGE_BYTES * pts * 3 * args.steps / wall / 1e9
+
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpt-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (TGV closed form + seeded 1e-3 Gaussian velocity noise)",
+        "config": {"workload": f"TGV {n}^3 RK3 step, compact 643c derivative + 10th-order "
+                               f"filter, Re=1600, Ma=0.1, dt={dt:g}",
+                   "grid": [n, n, n], "lattice_points": pts, "parallelism": "1 GPU, whole domain"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": f"{pname} ({pdir})",
+                     "avg_launch_ms": round(avg_s * 1e3, 4),
+                     "alg_bytes_per_launch": alg_bytes},
+        "step_roofline": {"alg_bytes_per_point_stage": STAGE_BYTES,
+                          "achieved_GBs": round(step_alg, 1),
+                          "frac": round(step_alg / HBM_PEAK_GBS, 4)},
+        "physical_point_rate": round(n ** 3 * 3 * args.steps / wall / 1e6, 3),
+        "kernels_ms_per_launch": {f"{k[0]}_{k[1]}": round(v[0] / v[1], 4) for k, v in prof.items()},
+        "ke_last": float(ke[-1]),
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,112 @@
+/*
+ * astr_hip.h -- C ABI of the MI355X (gfx950) implementation of ASTR's
+ * per-step hot path.
+ *
+ * The reference has no FFI: its hot path is a set of Fortran module
+ * procedures working on module-global arrays (src/commarray.F90:273-285,
+ * miniapps/tgv_solver_3d/tgvsolver.F90:1-40).  Each entry point below
+ * replaces one of those procedures; a Fortran host binds them with
+ * ISO_C_BINDING (astr_amd/fortran/astr_hip_mod.F90) under the reference's
+ * own subroutine names.
+ *
+ * Conventions
+ *   - Return value 0 = success, nonzero = error; astr_hip_last_error()
+ *     returns the message (per host thread).  The reference stops the
+ *     program on error (`stop '<msg>'`); the Fortran binding does the same.
+ *   - Host arrays use the reference's layout: column-major, i fastest, the
+ *     variable index slowest, each spatial index running -hm..n+hm, e.g.
+ *     q(-hm:im+hm,-hm:jm+hm,-hm:km+hm,1:5).  hm = 0 passes the computed
+ *     lattice 0..im only (halo cells are never read: on one device the
+ *     periodic halo is the index wrap the reference's halo copy produces).
+ *   - The library owns device memory; every field stays resident in HBM
+ *     between calls.  Calls on one context are stream-ordered and not
+ *     thread-safe (one host thread per context, as in the reference).
+ */
+#ifndef ASTR_HIP_H
+#define ASTR_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* semantics mode: ASTR_SEM_MINIAPP = the stage order, halo refresh and
+ * constants of miniapps/tgv_solver_3d/tgvsolver.F90 (rk3 :994-1082). */
+#define ASTR_SEM_MINIAPP 0
+
+/* Solver set-up.  Replaces program_init / solver_init / refcal constants
+ * (tgvsolver.F90:152-261; src/solver.F90:104-128). */
+typedef struct astr_hip_config {
+  int im, jm, km;          /* computed lattice 0..im, 0..jm, 0..km; km = 0: 2-D */
+  int semantics;           /* ASTR_SEM_MINIAPP */
+  double alfa_filter;      /* compact filter alpha (0.49); filter.F90:79-80 */
+  double beta_halo;        /* halo-row beta (1.11); filter.F90:57-62 */
+  double beta_bound;       /* boundary-row beta (1.09; ntype 1,2,4 only) */
+  double gamma, mach, reynolds, prandtl;
+  double ref_tem;          /* reference temperature (K) */
+  double sutherland;       /* Sutherland constant (K): 110.4 in tgvsolver.F90:133 */
+  double lx, ly, lz;       /* domain lengths; dx = lx/im (tgvsolver.F90:199-201) */
+  double deltat;           /* time step */
+} astr_hip_config;
+
+typedef struct astr_hip_ctx astr_hip_ctx;
+
+/* device layout of the resident fields (for callers that share device memory) */
+typedef struct astr_hip_layout {
+  int pitch;               /* doubles between consecutive j rows */
+  long long plane;         /* doubles between consecutive k planes */
+  long long field_stride;  /* doubles between variables of q / qrhs */
+  double *q;               /* device pointer, 5 fields */
+  double *qrhs;            /* device pointer, 5 fields */
+} astr_hip_layout;
+
+const char *astr_hip_last_error(void);
+
+/* alloccommarray + solver_init: tgvsolver.F90:13-38, :236-261 */
+int astr_hip_create(const astr_hip_config *cfg, astr_hip_ctx **out);
+int astr_hip_destroy(astr_hip_ctx *ctx);
+int astr_hip_set_stream(astr_hip_ctx *ctx, void *hip_stream);
+int astr_hip_set_deltat(astr_hip_ctx *ctx, double deltat);
+int astr_hip_field_info(astr_hip_ctx *ctx, astr_hip_layout *layout);
+
+/* flowfield_init (tgvsolver.F90:172-234), on the device; noise_amp > 0 adds
+ * seeded Gaussian velocity noise (Philox4x32-10 + Box-Muller, not part of
+ * the reference) on i,j,k < n, copied to the duplicate planes. */
+int astr_hip_init_tgv(astr_hip_ctx *ctx, double noise_amp, unsigned long long seed);
+
+/* set / read the conservative state q(-hm:..,1:5); download also returns the
+ * primitives rho,u,v,w,p,T (q2fvar, tgvsolver.F90:92-121) when prims != 0. */
+int astr_hip_upload(astr_hip_ctx *ctx, const double *q, int hm);
+int astr_hip_download(astr_hip_ctx *ctx, double *q, double *prims, int hm);
+
+/* filterq: tgvsolver.F90:791-883 (x, y, z sweeps of the compact filter) */
+int astr_hip_filterq(astr_hip_ctx *ctx);
+
+/* gradcal: tgvsolver.F90:450-537; dvel(0:im,0:jm,0:km,1:3,1:3), dtmp(...,1:3) */
+int astr_hip_gradcal(astr_hip_ctx *ctx, double *dvel, double *dtmp);
+
+/* bchomo + rhscal: tgvsolver.F90:1039-1041, :962-992; qrhs(0:im,..,1:5) */
+int astr_hip_rhscal(astr_hip_ctx *ctx, double *qrhs);
+
+/* steploop/rk3: tgvsolver.F90:944-960, :994-1082.  ke/ens (nullable) receive
+ * stacal's kinetic energy and enstrophy (:893-942) at the start of each step. */
+int astr_hip_rk3_steps(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens);
+int astr_hip_sync(astr_hip_ctx *ctx);
+
+/* Per-pass timing with HIP events on the context's stream (no reference
+ * counterpart; the reference's timerept accumulators, src/utility.F90:27-297,
+ * are no-ops).  Slot = pass*3 + direction, pass 0..6 = convective+gradient
+ * first/middle/last, viscous middle/last(rhs)/last(RK3+filter), filter. */
+#define ASTR_PROF_SLOTS 21
+int astr_hip_profile_begin(astr_hip_ctx *ctx, int max_launches);
+int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
+
+/* fds%central / compact_filter on a batch of lines: derivative.F90:171-198
+ * (op 0, df_compact, scheme 643c) and filter.F90:112-144 (op 1), ntype=3.
+ * f holds nlines lines of f(-5:n+5) (n+11 values each), out nlines x (n+1). */
+int astr_hip_line_batch(int op, int n, long long nlines, const double *f, double *out,
+                        double alfa_filter, double beta_halo);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -10,6 +10,7 @@ which is a C-order numpy array of shape (5, km+1, jm+1, im+1).
 import ctypes as C
 import os
 import sys
+import time
 
 import numpy as np
 
@@ -57,7 +58,9 @@ def main():
             L.ref_filterq()
         elif task == "rk3":
             nsteps = int(z["nsteps"])
+            t0 = time.perf_counter()
             L.ref_rk3_steps(C.c_int(nsteps))
+            res["seconds"] = np.array(time.perf_counter() - t0)
             rows = []
             with open("state.dat") as fh:
                 for line in fh:
