@@ -67,6 +67,7 @@ struct FlowArgs {
   double const2, const5, const6;
   double reynolds, prandtl, suth, suth1;
   double rk_a, rk_b, rk_c, dt;
+  int trash;            // a point index inside every field's padding
   double alfa, bhalo;   // filter alpha and halo-row beta (LHS of the filter rows)
   int stage;            // 0,1,2
   int dostats;          // stage-1 KE/enstrophy partial sums
@@ -101,12 +102,22 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 template <class C>
 struct Slots {
-  int g[C::NPT];  // offset inside a field, -1 if none
-  int s[C::NPT];  // index of the point in a line buffer (l*LT + r), -1 if none
+  int g[C::NPT];  // offset inside a field; the trash point for lines past the
+                  // end of the lattice (partial last tile)
+  int s[C::NPT];  // index of the point in a line buffer (l*LT + r)
+  int h[C::NPT];  // line-buffer index of the slot's periodic halo copy
   int interior;   // bit t: i,j,k >= 1 (stats domain)
-  int halo_lo;    // bit t: row r in [n-5,n-1], also the halo row r-n
-  int halo_hi;    // bit t: row r in [1,5], also the halo row r+n
 };
+
+// LDS of a pass: the output lines X (TL x LT), then the input lines T
+// (TL x LT), then a write-only dummy cell.  A slot holding no point (past the
+// tile) or no halo copy points s/h at index TL*LT: its T writes land in the
+// dummy cell (T + kLineOff + TL*LT) and its X reads return T's zero padding
+// (X + TL*LT = row -8 of T's first line), so no slot needs a guard.
+template <class C>
+constexpr int lds_doubles() {
+  return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
+}
 
 // Map the tile's points onto slots.  dir 0: lines are (j,k), rows i; dir 1:
 // lines (i,k) rows j; dir 2: lines (i,j) rows k.
@@ -115,14 +126,14 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
   const int n = a.n[dir];
   const int np = C::TL * (n + 1);
   const int ni = (a.n[0] + C::TL) / C::TL;
+  const int dummy = C::TL * C::LT;
   S.interior = 0;
-  S.halo_lo = 0;
-  S.halo_hi = 0;
 #pragma unroll
   for (int t = 0; t < C::NPT; t++) {
     const int e = threadIdx.x + t * C::NT;
-    S.g[t] = -1;
-    S.s[t] = -1;
+    S.g[t] = a.trash;
+    S.s[t] = dummy;
+    S.h[t] = dummy;
     if (e < np) {
       int l, r, i, j, k;
       bool ok;
@@ -143,8 +154,10 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
         if (dir == 1) { j = r; k = o; } else { j = o; k = r; }
       }
       S.s[t] = l * C::LT + r;
-      if (r >= n - 5 && r <= n - 1) S.halo_lo |= 1 << t;
-      if (r >= 1 && r <= 5) S.halo_hi |= 1 << t;
+      // rows n-5..n-1 are copied to the halo rows -5..-1, rows 1..5 to
+      // n+1..n+5 (disjoint for n >= 11)
+      if (r >= n - 5 && r <= n - 1) S.h[t] = S.s[t] - n;
+      if (r >= 1 && r <= 5) S.h[t] = S.s[t] + n;
       if (ok) {
         S.g[t] = (int)((long long)k * a.plane + (long long)j * a.pitch + i);
         const bool in = i >= 1 && j >= 1 && (a.ndir == 2 || k >= 1);
@@ -157,11 +170,9 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
 // store slot t's value into its input line with the periodic halo copies
 // q(-h)=q(n-h), q(n+h)=q(h) (tgvsolver.F90:371-440)
 template <class C>
-__device__ __forceinline__ void put(double* T, const Slots<C>& S, int t, int n, double v) {
-  double* b = T + kLineOff + S.s[t];
-  b[0] = v;
-  if ((S.halo_lo >> t) & 1) b[-n] = v;
-  if ((S.halo_hi >> t) & 1) b[n] = v;
+__device__ __forceinline__ void put(double* T, const Slots<C>& S, int t, double v) {
+  T[kLineOff + S.s[t]] = v;
+  T[kLineOff + S.h[t]] = v;
 }
 
 // zero the padding rows -8..-6 and n+6..P*R+2 of every input line (never
@@ -255,7 +266,7 @@ __device__ __forceinline__ void visc_flux(const FlowArgs& a, const double g[3][3
 // the pass kernel
 // ---------------------------------------------------------------------------
 #ifndef ASTR_MINB_GC
-#define ASTR_MINB_GC 1
+#define ASTR_MINB_GC 2
 #endif
 #ifndef ASTR_MINB_V
 #define ASTR_MINB_V 1
@@ -274,8 +285,8 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
   extern __shared__ double smem[];
   constexpr int NPT = C::NPT;
   const int n = a.n[dir];
-  double* T = smem;                 // input lines, TL x LT
-  double* X = smem + C::TL * C::LT; // output lines, TL x LT (rows at offset 0)
+  double* X = smem;                 // output lines, TL x LT (rows at offset 0)
+  double* T = smem + C::TL * C::LT; // input lines, TL x LT
   const int tile = xcd_tile(blockIdx.x, gridDim.x);
   Slots<C> S;
   make_slots<C>(S, a, dir, tile);
@@ -294,7 +305,7 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
 #pragma unroll
     for (int t = 0; t < NPT; t++)
 #pragma unroll
-      for (int m = 0; m < 5; m++) qv[m][t] = S.g[t] >= 0 ? a.q[m * fs + S.g[t]] : 1.0;
+      for (int m = 0; m < 5; m++) qv[m][t] = a.q[m * fs + S.g[t]];
 
     // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
     double gz[4][NPT];
@@ -302,9 +313,9 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
     for (int c = 0; c < 4; c++) {
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if (S.s[t] >= 0) {
+        {
           double f = 0.0;
-          if (S.g[t] >= 0) {
+          {
             if (c < 3) {
               f = qv[c + 1][t] / qv[0][t];
             } else {
@@ -312,7 +323,7 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
               f = pr / qv[0][t] * a.const2;
             }
           }
-          put<C>(T, S, t, n, f);
+          put<C>(T, S, t, f);
         }
       }
       __syncthreads();
@@ -321,7 +332,7 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
       const int fld = c < 3 ? c + 3 * dir : 9 + dir;
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if (S.g[t] >= 0) {
+        {
           const double v = X[S.s[t]] * idn;
           if (!LAST || a.store_last_grad) a.dv[fld * fs + S.g[t]] = v;
           gz[c][t] = v;
@@ -335,7 +346,6 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
       double sk = 0.0, se = 0.0;
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if (S.g[t] < 0) continue;
         const int o = S.g[t];
         double g[3][3], gt[3];
 #pragma unroll
@@ -396,21 +406,20 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
     for (int m = 0; m < 5; m++) {
 #pragma unroll
       for (int t = 0; t < NPT; t++)
-        if (S.s[t] >= 0)
-          put<C>(T, S, t, n, S.g[t] >= 0 ? conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]) : 0.0);
+        put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
       double* dst = a.qrhs + m * fs;
       double qr[NPT];
       if (PROG != GC_FIRST) {
         // issued before the solve so the load latency overlaps it
 #pragma unroll
-        for (int t = 0; t < NPT; t++) qr[t] = S.g[t] >= 0 ? dst[S.g[t]] : 0.0;
+        for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
       }
       __syncthreads();
       solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if (S.g[t] >= 0) {
+        {
           const double v = X[S.s[t]] * idn;
           if (PROG == GC_FIRST) dst[S.g[t]] = v;
           else if (PROG == GC_MID) dst[S.g[t]] = qr[t] + v;
@@ -429,30 +438,30 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
     double acc[4][NPT];
     double nx[NPT];
 #pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = S.g[t] >= 0 ? a.sg[c0 * fs + S.g[t]] : 0.0;
+    for (int t = 0; t < NPT; t++) nx[t] = a.sg[c0 * fs + S.g[t]];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
 #pragma unroll
       for (int t = 0; t < NPT; t++)
-        if (S.s[t] >= 0) put<C>(T, S, t, n, nx[t]);
+        put<C>(T, S, t, nx[t]);
       // loads for this column's accumulation and the next column's input are
       // issued before the solve so their latency overlaps it
       double qr[NPT];
       double* dst = a.qrhs + (1 + c) * fs;
 #pragma unroll
-      for (int t = 0; t < NPT; t++) qr[t] = S.g[t] >= 0 ? dst[S.g[t]] : 0.0;
+      for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
       if (c < 3) {
         const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
         const double* src = a.sg + col * fs;
 #pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = S.g[t] >= 0 ? src[S.g[t]] : 0.0;
+        for (int t = 0; t < NPT; t++) nx[t] = src[S.g[t]];
       }
       __syncthreads();
       solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if (S.g[t] >= 0) {
+        {
           const double v = qr[t] + X[S.s[t]] * idn;
           if (PROG == V_LAST_RK) acc[c][t] = v;
           else dst[S.g[t]] = v;
@@ -466,7 +475,6 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
       for (int t = 0; t < NPT; t++) {
 #pragma unroll
         for (int m = 0; m < 5; m++) qn[m][t] = 0.0;
-        if (S.g[t] < 0) continue;
         const int o = S.g[t];
 #pragma unroll
         for (int m = 0; m < 5; m++) {
@@ -488,14 +496,14 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
       for (int m = 0; m < 5; m++) {
 #pragma unroll
         for (int t = 0; t < NPT; t++)
-          if (S.s[t] >= 0) put<C>(T, S, t, n, qn[m][t]);
+          put<C>(T, S, t, qn[m][t]);
         __syncthreads();
         solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
         __syncthreads();
         double* dst = a.q + m * fs;
 #pragma unroll
         for (int t = 0; t < NPT; t++)
-          if (S.g[t] >= 0) dst[S.g[t]] = X[S.s[t]];
+          dst[S.g[t]] = X[S.s[t]];
       }
     }
   } else {  // F_ONLY: filter of q along dir, in place (filterq, tgvsolver.F90:791-883)
@@ -503,15 +511,15 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
     load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
     double nx[NPT];
 #pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = S.g[t] >= 0 ? a.q[S.g[t]] : 0.0;
+    for (int t = 0; t < NPT; t++) nx[t] = a.q[S.g[t]];
 #pragma unroll
     for (int m = 0; m < 5; m++) {
 #pragma unroll
       for (int t = 0; t < NPT; t++)
-        if (S.s[t] >= 0) put<C>(T, S, t, n, nx[t]);
+        put<C>(T, S, t, nx[t]);
       if (m < 4) {
 #pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = S.g[t] >= 0 ? a.q[(m + 1) * fs + S.g[t]] : 0.0;
+        for (int t = 0; t < NPT; t++) nx[t] = a.q[(m + 1) * fs + S.g[t]];
       }
       __syncthreads();
       solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
@@ -519,7 +527,7 @@ __global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, i
       double* dst = a.q + m * fs;
 #pragma unroll
       for (int t = 0; t < NPT; t++)
-        if (S.g[t] >= 0) dst[S.g[t]] = X[S.s[t]];
+        dst[S.g[t]] = X[S.s[t]];
     }
   }
 }
@@ -714,7 +722,7 @@ int pick_shape(int n) {
 template <class C>
 hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
   const int nt = tiles_of(a.n, dir, C::TL);
-  const size_t lds = std::max<size_t>(2u * C::TL * C::LT, 2u * C::NT) * sizeof(double);
+  const size_t lds = (size_t)lds_doubles<C>() * sizeof(double);
   dim3 g(nt), b(C::NT);
   switch (prog) {
     case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST>), g, b, lds, st, a, dir); break;
@@ -876,8 +884,8 @@ const char* astr_hip_last_error(void) { return g_err.c_str(); }
 int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   if (!cfg || !out) return fail("null argument");
   *out = nullptr;
-  if (cfg->im < 8 || cfg->jm < 8 || cfg->km < 0 || (cfg->km > 0 && cfg->km < 8))
-    return fail("grid sizes must be >= 8 (km == 0 for two-dimensional cases)");
+  if (cfg->im < 11 || cfg->jm < 11 || cfg->km < 0 || (cfg->km > 0 && cfg->km < 11))
+    return fail("grid sizes must be >= 11 (km == 0 for two-dimensional cases)");
   if (cfg->semantics != ASTR_SEM_MINIAPP) return fail("unsupported semantics mode");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device");
@@ -890,7 +898,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   a.ndir = cfg->km > 0 ? 3 : 2;
   a.pitch = ((cfg->im + 1) + 15) / 16 * 16;
   a.plane = (long long)a.pitch * (cfg->jm + 1);
-  a.fs = (a.plane * (cfg->km + 1) + 63) / 64 * 64;
+  a.trash = (int)(a.plane * (cfg->km + 1));
+  a.fs = (a.plane * (cfg->km + 1) + 1 + 63) / 64 * 64;
   const double g = cfg->gamma, ma = cfg->mach;
   a.const2 = g * ma * ma;
   a.const5 = (g - 1.0) * ma * ma;
