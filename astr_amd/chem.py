@@ -1,0 +1,136 @@
+"""Host side of the chemical source terms: a ctypes front end over the
+astr_hip_chem_* entry points (include/astr_hip.h) and the synthetic H2-air
+cell generator used by the batch benchmark and the tests.
+
+Method names follow the reference routines they replace
+(src/thermchem.F90): chemrate (net production rates times W_k), cpeval
+(cp_mass) and enthpy (h_k with the reference's R = 8314.2).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import AstrError, lib
+
+_dp = C.POINTER(C.c_double)
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp) if a is not None else None
+
+
+def _bind():
+    L = lib()
+    if getattr(L, "_chem_bound", False):
+        return L
+    L.astr_hip_chem_last_error.restype = C.c_char_p
+    L.astr_hip_chem_create.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p)]
+    L.astr_hip_chem_destroy.argtypes = [C.c_void_p]
+    L.astr_hip_chem_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), _dp,
+                                     C.c_char_p, C.c_int]
+    L.astr_hip_chem_rates.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, C.c_int, _dp, _dp,
+                                      _dp, _dp, _dp]
+    L.astr_hip_chem_rates_device.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p,
+                                             C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]
+    L._chem_bound = True
+    return L
+
+
+def _chk(rc):
+    if rc != 0:
+        raise AstrError(_bind().astr_hip_chem_last_error().decode())
+
+
+class Mechanism:
+    """A CTML mechanism resident on the device (chemread, thermchem.F90:59-98)."""
+
+    def __init__(self, path, phase_id="gas"):
+        L = _bind()
+        h = C.c_void_p()
+        _chk(L.astr_hip_chem_create(path.encode(), phase_id.encode(), C.byref(h)))
+        self.h = h
+        ns, nr = C.c_int(), C.c_int()
+        _chk(L.astr_hip_chem_info(h, C.byref(ns), C.byref(nr), None, None, 0))
+        self.ns, self.nr = ns.value, nr.value
+        self.W = np.zeros(self.ns)
+        buf = C.create_string_buffer(16 * self.ns)
+        _chk(L.astr_hip_chem_info(h, None, None, _p(self.W), buf, 16))
+        self.names = [buf.raw[16 * k:16 * (k + 1)].split(b"\0")[0].decode() for k in range(self.ns)]
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _bind().astr_hip_chem_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def rates(self, T, rho_or_p, Y, input_is_p=True, species_h=False):
+        """T, rho|p: (n,), Y: (ns, n) -> wdot (ns, n) kg/m^3/s, h_mix, cp_mix[, h_k]."""
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        rp = np.ascontiguousarray(rho_or_p, dtype=np.float64)
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        n = T.size
+        if Y.shape != (self.ns, n) or rp.shape != (n,):
+            raise AstrError("shapes: T (n,), rho_or_p (n,), Y (ns, n)")
+        wd = np.empty((self.ns, n))
+        hm = np.empty(n)
+        cp = np.empty(n)
+        hk = np.empty((self.ns, n)) if species_h else None
+        _chk(_bind().astr_hip_chem_rates(self.h, n, _p(T), _p(rp), int(input_is_p), _p(Y),
+                                         _p(wd), _p(hm), _p(cp), _p(hk)))
+        return (wd, hm, cp, hk) if species_h else (wd, hm, cp)
+
+    chemrate = rates
+
+
+def mixture(names, W, phi, c):
+    """Mass fractions (ns, n): unburnt H2/air at equivalence ratio phi (O2:N2
+    = 0.21:0.79 by mole) blended by progress c with complete-combustion
+    products plus a fixed radical pool (mole fractions OH=H=O=1e-3,
+    HO2=H2O2=1e-5).  A build-defined input generator, not a reference
+    routine."""
+    ix = {s: k for k, s in enumerate(names)}
+    W = np.asarray(W, dtype=float)
+    phi = np.asarray(phi, dtype=float)
+    c = np.asarray(c, dtype=float)
+    n = phi.size
+    ns = len(names)
+    Xu = np.zeros((ns, n))
+    Xu[ix["H2"]] = 2.0 * phi
+    Xu[ix["O2"]] = 1.0
+    Xu[ix["N2"]] = 0.79 / 0.21
+    Xb = np.zeros((ns, n))
+    Xb[ix["H2O"]] = 2.0 * np.minimum(phi, 1.0)
+    Xb[ix["O2"]] = np.maximum(1.0 - phi, 0.0)
+    Xb[ix["H2"]] = 2.0 * np.maximum(phi - 1.0, 0.0)
+    Xb[ix["N2"]] = 0.79 / 0.21
+    Xb = Xb / Xb.sum(axis=0)
+    for s, v in (("OH", 1e-3), ("H", 1e-3), ("O", 1e-3), ("HO2", 1e-5), ("H2O2", 1e-5)):
+        Xb[ix[s]] += v
+    Xb = Xb / Xb.sum(axis=0)
+    Xu = Xu / Xu.sum(axis=0)
+
+    def x2y(X):
+        m = X * W[:, None]
+        return m / m.sum(axis=0)
+
+    Y = (1.0 - c) * x2y(Xu) + c * x2y(Xb)
+    return Y / Y.sum(axis=0)
+
+
+def sample_cells(names, W, n, seed):
+    """The chemistry batch workload: T ~ U[900,2500] K, p log-uniform in
+    [0.5,5] atm, phi ~ U[0.3,3], progress c ~ U[0,1]; returns T, p, Y."""
+    rng = np.random.default_rng(seed)
+    T = rng.uniform(900.0, 2500.0, n)
+    p = np.exp(rng.uniform(np.log(0.5), np.log(5.0), n)) * 101325.0
+    phi = rng.uniform(0.3, 3.0, n)
+    c = rng.uniform(0.0, 1.0, n)
+    return T, p, mixture(names, W, phi, c)

@@ -1,0 +1,493 @@
+// chem.hip -- per-cell chemical source terms on gfx950 and their C ABI.
+//
+// Replaces the reference's per-cell Cantera calls on the hot path:
+//   chemrate(den,tmp,spc,wi)   src/thermchem.F90:1553-1575  (setState_TRY +
+//                              getNetProductionRates, wi = wdot*W)
+//   cpeval(tmp,spc,cp=)        src/thermchem.F90:1342-1428  (cp_mass)
+//   enthpy(tmp,hi)             src/thermchem.F90:1498-1541  (h_k = H/RT *
+//                              (8314.2/W_k) * T)
+// for a CTML mechanism (Cantera's XML format, e.g.
+// chemMech/Jachimowski88_H2-9-19.xml): ideal gas, NASA 7-coefficient
+// thermodynamics, elementary and three-body mass-action reactions with
+// Arrhenius rate constants, reverse rates from the equilibrium constant.
+// One thread evaluates one cell; all arithmetic is FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/astr_hip.h"
+
+namespace {
+
+thread_local std::string g_cerr;
+int cfail(const std::string& m) {
+  g_cerr = m;
+  return -1;
+}
+
+#define CHIP_OK(x)                                                               \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) return cfail(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int NSMAX = 16;
+constexpr int NRMAX = 48;
+constexpr int NTERM = 3;  // species per side of a reaction
+
+// gas constant (J/kmol/K) of Cantera 2.5 (CODATA 2018 Avogadro x Boltzmann),
+// used inside Cantera's thermodynamics and kinetics
+constexpr double kRgas = 8314.46261815324;
+// the reference's own constant in enthpy/rgcmix (thermchem.F90:26,490,1513)
+constexpr double kRref = 8314.2;
+
+struct Mech {
+  int ns, nr;
+  double W[NSMAX];         // molecular weights, kg/kmol
+  double tmid[NSMAX];
+  double lo[NSMAX][7], hi[NSMAX][7];
+  double lnA[NRMAX], b[NRMAX], EaR[NRMAX];
+  int rev[NRMAX], tb[NRMAX];
+  double eff[NRMAX][NSMAX];  // third-body efficiencies
+  int nre[NRMAX], rsp[NRMAX][NTERM];
+  double rnu[NRMAX][NTERM];
+  int npr[NRMAX], psp[NRMAX][NTERM];
+  double pnu[NRMAX][NTERM];
+  double dnu[NRMAX];       // sum of product minus reactant coefficients
+  double p0;               // standard-state pressure of the thermo data, Pa
+};
+
+// ---------------------------------------------------------------------------
+// device evaluation
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double conc_pow(double c, double nu) {
+  return nu == 1.0 ? c : (nu == 2.0 ? c * c : pow(c, nu));
+}
+
+__global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
+                                              const double* __restrict__ Tin,
+                                              const double* __restrict__ RPin, int input_is_p,
+                                              const double* __restrict__ Y, double* __restrict__ wdot,
+                                              double* __restrict__ hmix, double* __restrict__ cpmix,
+                                              double* __restrict__ hk) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ns = M->ns, nr = M->nr;
+  const double T = Tin[i];
+  double y[NSMAX], c[NSMAX], g[NSMAX], w[NSMAX];
+  double winv = 0.0;
+  for (int k = 0; k < ns; k++) {
+    y[k] = Y[k * n + i];
+    winv += y[k] / M->W[k];
+  }
+  const double rho = input_is_p ? RPin[i] / (kRgas * T * winv) : RPin[i];
+  const double logT = log(T), invT = 1.0 / T;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  double cp = 0.0, h = 0.0;
+  for (int k = 0; k < ns; k++) {
+    c[k] = rho * y[k] / M->W[k];
+    const double* a = T <= M->tmid[k] ? M->lo[k] : M->hi[k];
+    // NASA 7-coefficient polynomials
+    const double cpr = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
+    const double hrt = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 +
+                       a[4] * T4 / 5.0 + a[5] * invT;
+    const double sr = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 +
+                      a[4] * T4 / 4.0 + a[6];
+    g[k] = hrt - sr;
+    cp += y[k] * cpr / M->W[k];
+    const double hks = hrt * (kRref / M->W[k]) * T;
+    h += y[k] * hks;
+    if (hk) hk[k * n + i] = hks;
+    w[k] = 0.0;
+  }
+  const double lc0 = log(M->p0 / (kRgas * T));  // ln of the standard concentration
+  for (int r = 0; r < nr; r++) {
+    const double kf = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT);
+    double fwd = kf;
+    double dg = 0.0;
+    for (int j = 0; j < M->nre[r]; j++) {
+      fwd *= conc_pow(c[M->rsp[r][j]], M->rnu[r][j]);
+      dg -= M->rnu[r][j] * g[M->rsp[r][j]];
+    }
+    double rv = 0.0;
+    if (M->rev[r]) {
+      double pp = 1.0;
+      for (int j = 0; j < M->npr[r]; j++) {
+        pp *= conc_pow(c[M->psp[r][j]], M->pnu[r][j]);
+        dg += M->pnu[r][j] * g[M->psp[r][j]];
+      }
+      // kr = kf / Kc, Kc = exp(-dG/RT) * (p0/RT)^dnu
+      const double kr = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT + dg - M->dnu[r] * lc0);
+      rv = kr * pp;
+    }
+    double q = fwd - rv;
+    if (M->tb[r]) {
+      double cm = 0.0;
+      for (int k = 0; k < ns; k++) cm += M->eff[r][k] * c[k];
+      q *= cm;
+    }
+    for (int j = 0; j < M->nre[r]; j++) w[M->rsp[r][j]] -= M->rnu[r][j] * q;
+    for (int j = 0; j < M->npr[r]; j++) w[M->psp[r][j]] += M->pnu[r][j] * q;
+  }
+  for (int k = 0; k < ns; k++) wdot[k * n + i] = w[k] * M->W[k];
+  if (hmix) hmix[i] = h;
+  if (cpmix) cpmix[i] = cp * kRgas;
+}
+
+// ---------------------------------------------------------------------------
+// CTML reading (the subset the reference's mechanisms use)
+// ---------------------------------------------------------------------------
+std::string trim(const std::string& s) {
+  size_t a = s.find_first_not_of(" \t\r\n"), b = s.find_last_not_of(" \t\r\n");
+  return a == std::string::npos ? std::string() : s.substr(a, b - a + 1);
+}
+
+// text between <tag ...> and </tag> starting at pos; attributes in attr
+bool element(const std::string& x, const std::string& tag, size_t& pos, std::string& attr,
+             std::string& body) {
+  for (;;) {
+    size_t a = x.find("<" + tag, pos);
+    if (a == std::string::npos) return false;
+    const char nc = x[a + 1 + tag.size()];
+    if (nc != ' ' && nc != '>' && nc != '\t' && nc != '\n' && nc != '/') {
+      pos = a + 1;
+      continue;
+    }
+    size_t e = x.find('>', a);
+    if (e == std::string::npos) return false;
+    attr = x.substr(a + 1 + tag.size(), e - a - 1 - tag.size());
+    if (x[e - 1] == '/') {
+      body.clear();
+      pos = e + 1;
+      return true;
+    }
+    size_t c = x.find("</" + tag + ">", e);
+    if (c == std::string::npos) return false;
+    body = x.substr(e + 1, c - e - 1);
+    pos = c + tag.size() + 3;
+    return true;
+  }
+}
+
+std::string attr_of(const std::string& attrs, const std::string& key) {
+  size_t p = attrs.find(key + "=\"");
+  if (p == std::string::npos) return "";
+  p += key.size() + 2;
+  size_t e = attrs.find('"', p);
+  return attrs.substr(p, e - p);
+}
+
+// "A:1.0 B:2" -> pairs
+std::vector<std::pair<std::string, double>> pairs_of(const std::string& s) {
+  std::vector<std::pair<std::string, double>> out;
+  std::istringstream is(s);
+  std::string tok;
+  while (is >> tok) {
+    size_t c = tok.find(':');
+    if (c == std::string::npos) continue;
+    out.push_back({tok.substr(0, c), std::stod(tok.substr(c + 1))});
+  }
+  return out;
+}
+
+std::vector<double> floats_of(const std::string& s) {
+  std::vector<double> v;
+  std::string t = s;
+  for (char& ch : t)
+    if (ch == ',') ch = ' ';
+  std::istringstream is(t);
+  double d;
+  while (is >> d) v.push_back(d);
+  return v;
+}
+
+// atomic weights (kg/kmol) of the elements the reference's mechanisms use
+double atomic_weight(const std::string& e) {
+  static const std::map<std::string, double> w = {
+      {"H", 1.008}, {"O", 15.999}, {"N", 14.007}, {"C", 12.011}, {"AR", 39.95}, {"Ar", 39.95},
+      {"HE", 4.002602}, {"He", 4.002602}};
+  auto it = w.find(e);
+  return it == w.end() ? -1.0 : it->second;
+}
+
+// factor turning an activation energy in the given units into E/R (K)
+bool ea_factor(const std::string& units, double& f) {
+  if (units == "cal/mol") f = 4184.0 / kRgas;
+  else if (units == "kcal/mol") f = 4184000.0 / kRgas;
+  else if (units == "J/mol") f = 1000.0 / kRgas;
+  else if (units == "kJ/mol") f = 1.0e6 / kRgas;
+  else if (units == "J/kmol") f = 1.0 / kRgas;
+  else if (units == "K") f = 1.0;
+  else return false;
+  return true;
+}
+
+int parse_ctml(const std::string& path, const std::string& phase_id, Mech& m,
+               std::vector<std::string>& names) {
+  std::ifstream f(path);
+  if (!f) return cfail("cannot open mechanism file " + path);
+  std::stringstream ss;
+  ss << f.rdbuf();
+  const std::string x = ss.str();
+  std::memset(&m, 0, sizeof(m));
+  // phase: species order
+  size_t pos = 0;
+  std::string attr, body;
+  bool found = false;
+  while (element(x, "phase", pos, attr, body)) {
+    if (attr_of(attr, "id") == phase_id) { found = true; break; }
+  }
+  if (!found) return cfail("phase '" + phase_id + "' not found in " + path);
+  {
+    size_t p2 = 0;
+    std::string a2, b2;
+    if (!element(body, "speciesArray", p2, a2, b2)) return cfail("phase without speciesArray");
+    std::istringstream is(b2);
+    std::string nm;
+    while (is >> nm) names.push_back(nm);
+    p2 = 0;
+    if (element(body, "kinetics", p2, a2, b2) && attr_of(a2, "model") != "GasKinetics")
+      return cfail("unsupported kinetics model " + attr_of(a2, "model"));
+  }
+  m.ns = (int)names.size();
+  if (m.ns < 1 || m.ns > NSMAX) return cfail("species count out of range");
+  std::map<std::string, int> idx;
+  for (int k = 0; k < m.ns; k++) idx[names[k]] = k;
+  // species data
+  int nfound = 0;
+  pos = 0;
+  m.p0 = -1.0;
+  while (element(x, "species", pos, attr, body)) {
+    auto it = idx.find(attr_of(attr, "name"));
+    if (it == idx.end()) continue;
+    const int k = it->second;
+    size_t p2 = 0;
+    std::string a2, b2;
+    if (!element(body, "atomArray", p2, a2, b2)) return cfail("species without atomArray");
+    double W = 0.0;
+    for (auto& pr : pairs_of(b2)) {
+      const double aw = atomic_weight(pr.first);
+      if (aw < 0) return cfail("unknown element " + pr.first);
+      W += pr.second * aw;
+    }
+    m.W[k] = W;
+    p2 = 0;
+    int nr = 0;
+    while (element(body, "NASA", p2, a2, b2)) {
+      size_t p3 = 0;
+      std::string a3, b3;
+      if (!element(b2, "floatArray", p3, a3, b3)) return cfail("NASA block without coeffs");
+      auto v = floats_of(b3);
+      if (v.size() != 7) return cfail("NASA block needs 7 coefficients");
+      const double tmin = std::stod(attr_of(a2, "Tmin")), tmax = std::stod(attr_of(a2, "Tmax"));
+      const std::string sp0 = attr_of(a2, "P0");
+      if (!sp0.empty()) m.p0 = std::stod(sp0);
+      double* dst = nr == 0 ? m.lo[k] : m.hi[k];
+      for (int j = 0; j < 7; j++) dst[j] = v[j];
+      if (nr == 0) m.tmid[k] = tmax;
+      else if (tmin < m.tmid[k]) {  // listed high range first: swap
+        double tmp[7];
+        std::memcpy(tmp, m.lo[k], sizeof(tmp));
+        std::memcpy(m.lo[k], m.hi[k], sizeof(tmp));
+        std::memcpy(m.hi[k], tmp, sizeof(tmp));
+        m.tmid[k] = tmax;
+      }
+      nr++;
+    }
+    if (nr != 2) return cfail("species " + names[k] + " needs two NASA ranges");
+    nfound++;
+  }
+  if (nfound != m.ns) return cfail("species data missing for some phase species");
+  if (m.p0 <= 0) m.p0 = 101325.0;
+  // reactions
+  pos = 0;
+  int r = 0;
+  while (element(x, "reaction", pos, attr, body)) {
+    if (r >= NRMAX) return cfail("too many reactions");
+    const std::string type = attr_of(attr, "type");
+    if (!type.empty() && type != "threeBody") return cfail("unsupported reaction type " + type);
+    m.tb[r] = type == "threeBody";
+    m.rev[r] = attr_of(attr, "reversible") != "no";
+    size_t p2 = 0;
+    std::string a2, b2;
+    if (!element(body, "Arrhenius", p2, a2, b2)) return cfail("reaction without Arrhenius");
+    {
+      size_t p3 = 0;
+      std::string a3, b3;
+      if (!element(b2, "A", p3, a3, b3)) return cfail("Arrhenius without A");
+      const double A = std::stod(trim(b3));
+      p3 = 0;
+      if (!element(b2, "b", p3, a3, b3)) return cfail("Arrhenius without b");
+      m.b[r] = std::stod(trim(b3));
+      p3 = 0;
+      if (!element(b2, "E", p3, a3, b3)) return cfail("Arrhenius without E");
+      std::string u = attr_of(a3, "units");
+      if (u.empty()) u = "cal/mol";
+      double fac = 0.0;
+      if (!ea_factor(u, fac)) return cfail("unsupported energy units " + u);
+      m.EaR[r] = std::stod(trim(b3)) * fac;
+      if (A <= 0) return cfail("non-positive pre-exponential factor");
+      m.lnA[r] = std::log(A);
+    }
+    for (int k = 0; k < m.ns; k++) m.eff[r][k] = 1.0;
+    p2 = 0;
+    if (element(body, "efficiencies", p2, a2, b2)) {
+      const std::string d = attr_of(a2, "default");
+      const double def = d.empty() ? 1.0 : std::stod(d);
+      for (int k = 0; k < m.ns; k++) m.eff[r][k] = def;
+      for (auto& pr : pairs_of(b2)) {
+        auto it = idx.find(pr.first);
+        if (it != idx.end()) m.eff[r][it->second] = pr.second;
+      }
+    }
+    double sr = 0.0, sp = 0.0;
+    p2 = 0;
+    if (!element(body, "reactants", p2, a2, b2)) return cfail("reaction without reactants");
+    for (auto& pr : pairs_of(b2)) {
+      auto it = idx.find(pr.first);
+      if (it == idx.end() || m.nre[r] >= NTERM) return cfail("bad reactant " + pr.first);
+      m.rsp[r][m.nre[r]] = it->second;
+      m.rnu[r][m.nre[r]++] = pr.second;
+      sr += pr.second;
+    }
+    p2 = 0;
+    if (!element(body, "products", p2, a2, b2)) return cfail("reaction without products");
+    for (auto& pr : pairs_of(b2)) {
+      auto it = idx.find(pr.first);
+      if (it == idx.end() || m.npr[r] >= NTERM) return cfail("bad product " + pr.first);
+      m.psp[r][m.npr[r]] = it->second;
+      m.pnu[r][m.npr[r]++] = pr.second;
+      sp += pr.second;
+    }
+    m.dnu[r] = sp - sr;
+    r++;
+  }
+  m.nr = r;
+  return 0;
+}
+
+}  // namespace
+
+struct astr_chem {
+  Mech host;
+  Mech* dev = nullptr;
+  std::vector<std::string> names;
+  int device = 0;
+};
+
+extern "C" {
+
+const char* astr_hip_chem_last_error(void) { return g_cerr.c_str(); }
+
+int astr_hip_chem_create(const char* ctml_path, const char* phase_id, astr_chem** out) {
+  if (!ctml_path || !out) return cfail("null argument");
+  *out = nullptr;
+  astr_chem* c = new astr_chem();
+  if (parse_ctml(ctml_path, phase_id ? phase_id : "gas", c->host, c->names)) {
+    delete c;
+    return -1;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    delete c;
+    return cfail("no HIP device");
+  }
+  (void)hipGetDevice(&c->device);
+  if (hipMalloc(&c->dev, sizeof(Mech)) != hipSuccess) {
+    delete c;
+    return cfail("hipMalloc of mechanism failed");
+  }
+  if (hipMemcpy(c->dev, &c->host, sizeof(Mech), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(c->dev);
+    delete c;
+    return cfail("copy of mechanism failed");
+  }
+  *out = c;
+  return 0;
+}
+
+int astr_hip_chem_destroy(astr_chem* c) {
+  if (!c) return 0;
+  (void)hipSetDevice(c->device);
+  (void)hipFree(c->dev);
+  delete c;
+  return 0;
+}
+
+int astr_hip_chem_info(astr_chem* c, int* nsp, int* nrx, double* mw, char* names, int name_len) {
+  if (!c) return cfail("null mechanism");
+  if (nsp) *nsp = c->host.ns;
+  if (nrx) *nrx = c->host.nr;
+  for (int k = 0; k < c->host.ns; k++) {
+    if (mw) mw[k] = c->host.W[k];
+    if (names && name_len > 0) {
+      std::strncpy(names + (size_t)k * name_len, c->names[k].c_str(), name_len - 1);
+      names[(size_t)k * name_len + name_len - 1] = 0;
+    }
+  }
+  return 0;
+}
+
+int astr_hip_chem_rates_device(astr_chem* c, long long n, const double* T, const double* rho_or_p,
+                               int input_is_p, const double* Y, double* wdot, double* h_mix,
+                               double* cp_mix, double* h_k, void* stream) {
+  if (!c) return cfail("null mechanism");
+  if (n < 0) return cfail("negative cell count");
+  if (n == 0) return 0;
+  if (!T || !rho_or_p || !Y || !wdot) return cfail("null buffer");
+  CHIP_OK(hipSetDevice(c->device));
+  const int bs = 256;
+  hipLaunchKernelGGL(k_chem, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0,
+                     (hipStream_t)stream, c->dev, n, T, rho_or_p, input_is_p, Y, wdot, h_mix,
+                     cp_mix, h_k);
+  CHIP_OK(hipGetLastError());
+  return 0;
+}
+
+int astr_hip_chem_rates(astr_chem* c, long long n, const double* T, const double* rho_or_p,
+                        int input_is_p, const double* Y, double* wdot, double* h_mix,
+                        double* cp_mix, double* h_k) {
+  if (!c) return cfail("null mechanism");
+  if (n < 0) return cfail("negative cell count");
+  if (n == 0) return 0;
+  if (!T || !rho_or_p || !Y || !wdot) return cfail("null buffer");
+  CHIP_OK(hipSetDevice(c->device));
+  const int ns = c->host.ns;
+  const size_t b1 = (size_t)n * sizeof(double);
+  double* d = nullptr;
+  const size_t nbuf = 2 + 2 * (size_t)ns + 2 + (h_k ? ns : 0);
+  CHIP_OK(hipMalloc(&d, nbuf * b1));
+  double* dT = d;
+  double* dR = dT + n;
+  double* dY = dR + n;
+  double* dW = dY + (size_t)ns * n;
+  double* dH = dW + (size_t)ns * n;
+  double* dC = dH + n;
+  double* dK = h_k ? dC + n : nullptr;
+  int rc = 0;
+  if (hipMemcpy(dT, T, b1, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dR, rho_or_p, b1, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dY, Y, b1 * ns, hipMemcpyHostToDevice) != hipSuccess) {
+    rc = cfail("copy to device failed");
+  }
+  if (!rc) rc = astr_hip_chem_rates_device(c, n, dT, dR, input_is_p, dY, dW, dH, dC, dK, nullptr);
+  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = cfail("chemistry kernel failed");
+  if (!rc) {
+    if (hipMemcpy(wdot, dW, b1 * ns, hipMemcpyDeviceToHost) != hipSuccess ||
+        (h_mix && hipMemcpy(h_mix, dH, b1, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (cp_mix && hipMemcpy(cp_mix, dC, b1, hipMemcpyDeviceToHost) != hipSuccess) ||
+        (h_k && hipMemcpy(h_k, dK, b1 * ns, hipMemcpyDeviceToHost) != hipSuccess))
+      rc = cfail("copy from device failed");
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+}  // extern "C"

@@ -14,6 +14,8 @@ on this host's cores as cpu_baseline.
 """
 import argparse
 import json
+
+import numpy as np
 import math
 import os
 import sys
@@ -78,6 +80,42 @@ def cpu_baseline(seconds_target=12.0):
             "kind": "port", "sample": f"C restatement, TGV {n}^3, {steps} steps, {sec:.1f} s"}
 
 
+def chem_batch(ncells=1 << 20, reps=20):
+    """Chemistry source-term batch: 2^20 H2-air cells (9 species, 19
+    reactions), inputs resident in HBM, FP64.  Algorithmic bytes per cell:
+    T, p, Y(9) in and wdot(9), h, cp out = 22 words = 176 B."""
+    import astr_amd.chem as AC
+    from astr_amd import hip
+
+    mech = AC.Mechanism(os.path.join(ROOT, "tests", "golden", "Jachimowski88_H2-9-19.xml"))
+    T, p, Y = AC.sample_cells(mech.names, mech.W, ncells, 2026)
+    dT, dp, dY = (hip.DeviceArray.from_host(a) for a in (T, p, Y))
+    dW = hip.DeviceArray((mech.ns, ncells))
+    dH = hip.DeviceArray((ncells,))
+    dC = hip.DeviceArray((ncells,))
+    L = AC._bind()
+    call = lambda: AC._chk(L.astr_hip_chem_rates_device(  # noqa: E731
+        mech.h, ncells, dT.ptr, dp.ptr, 1, dY.ptr, dW.ptr, dH.ptr, dC.ptr, None, None))
+    for _ in range(3):
+        call()
+    hip.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record()
+    for _ in range(reps):
+        call()
+    e1.record()
+    ms = e0.elapsed_ms(e1) / reps
+    wd = dW.to_host()
+    mass = float(np.abs(wd.sum(axis=0)).max() / np.abs(wd).max())
+    evals = ncells / (ms * 1e-3)
+    gbs = 176.0 * ncells / (ms * 1e-3) / 1e9
+    return {"cells": ncells, "ms_per_batch": round(ms, 4), "evals_per_s": round(evals, 1),
+            "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(gbs / HBM_PEAK_GBS, 4), "alg_bytes_per_cell": 176},
+            "mass_conservation_relerr": mass, "dtype": "f64",
+            "mechanism": "Jachimowski88_H2-9-19 (9 species, 19 reactions)"}
+
+
 def load_traffic(n):
     path = os.path.join(ROOT, "profiles", f"traffic_tgv{n}.json")
     if os.path.exists(path):
@@ -93,6 +131,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-chem", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the whole {args.n}^3 domain sits in one MI355X's HBM; "
@@ -164,6 +203,8 @@ def main():
         "kernels_ms_per_launch": {f"{k[0]}_{k[1]}": round(v[0] / v[1], 4) for k, v in prof.items()},
         "ke_last": float(ke[-1]),
     }
+    if not args.no_chem:
+        out["chem_batch"] = chem_batch()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

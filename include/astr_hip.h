@@ -106,6 +106,33 @@ int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
 int astr_hip_line_batch(int op, int n, long long nlines, const double *f, double *out,
                         double alfa_filter, double beta_halo);
 
+/* ------------------------------------------------------------------------
+ * Chemical source terms (per cell), replacing the reference's Cantera calls
+ * chemrate (src/thermchem.F90:1553-1575: wi = net production rate * W_k,
+ * kg/m^3/s), cpeval (:1342-1428: cp_mass, J/kg/K) and enthpy (:1498-1541:
+ * h_k = (H/RT)_k * (8314.2/W_k) * T, J/kg) for a CTML mechanism file
+ * (chemread, :59-98).  Arrays are structure-of-arrays: species k of cell i
+ * at [k*n + i].
+ * ------------------------------------------------------------------------ */
+typedef struct astr_chem astr_chem;
+
+const char *astr_hip_chem_last_error(void);
+int astr_hip_chem_create(const char *ctml_path, const char *phase_id, astr_chem **out);
+int astr_hip_chem_destroy(astr_chem *chem);
+/* species count, reaction count, molecular weights (kg/kmol), names (name_len
+ * bytes each, NUL-terminated); any output may be null */
+int astr_hip_chem_info(astr_chem *chem, int *nsp, int *nrx, double *mw, char *names, int name_len);
+/* host buffers: T (K), rho (kg/m^3) or p (Pa) per input_is_p, Y (nsp x n);
+ * out wdot (nsp x n), h_mix, cp_mix (n each, nullable), h_k (nsp x n, nullable) */
+int astr_hip_chem_rates(astr_chem *chem, long long n, const double *T, const double *rho_or_p,
+                        int input_is_p, const double *Y, double *wdot, double *h_mix,
+                        double *cp_mix, double *h_k);
+/* the same on device buffers already resident in HBM, stream-ordered on stream */
+int astr_hip_chem_rates_device(astr_chem *chem, long long n, const double *T,
+                               const double *rho_or_p, int input_is_p, const double *Y,
+                               double *wdot, double *h_mix, double *cp_mix, double *h_k,
+                               void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,142 @@
+"""Chemical source terms: the numpy restatement (oracle/chem.py) pinned on the
+CPU, and the HIP kernel checked against it on the GPU.
+
+Cantera 2.5.1, which the reference calls (src/thermchem.F90:1553-1575), is
+not available and no reference fixture pins its outputs, so parity with
+Cantera itself is unpinned; the restatement is pinned by conservation laws,
+thermodynamic consistency, a published rate constant, and detailed balance
+at an equilibrium state computed by an independent element-potential solve.
+"""
+import os
+
+import numpy as np
+import pytest
+from scipy.optimize import fsolve
+
+from astr_amd import chem as AC
+from oracle import chem as OC
+
+MECH = os.path.join(os.path.dirname(__file__), "golden", "Jachimowski88_H2-9-19.xml")
+
+
+@pytest.fixture(scope="module")
+def mech():
+    return OC.Mechanism(MECH)
+
+
+def test_mechanism_contents(mech):
+    assert mech.names == ["H2", "O2", "H2O", "O", "OH", "H", "HO2", "H2O2", "N2"]
+    assert mech.nr == 19
+    assert sum(r["tb"] for r in mech.rx) == 6
+    assert mech.p0 == 1.0e5
+    assert abs(mech.W[mech.names.index("H2O")] - 18.015) < 1e-12
+
+
+def test_conservation(mech):
+    T, p, Y = AC.sample_cells(mech.names, mech.W, 20000, 3)
+    wd, h, cp, hk = mech.rates(T, p, Y)
+    scale = np.abs(wd).max(axis=0) + 1e-300
+    assert np.all(np.abs(wd.sum(axis=0)) <= 1e-13 * scale)
+    for e in "HON":
+        a = np.array([mech.comp[s].get(e, 0.0) / mech.W[k] for k, s in enumerate(mech.names)])
+        assert np.all(np.abs((a[:, None] * wd).sum(axis=0)) <= 1e-13 * scale / 1.0)
+    assert np.all(wd[mech.names.index("N2")] == 0.0)
+
+
+def test_thermo_consistency(mech):
+    """cp = dh/dT and cp/T = ds/dT (central differences), and the two NASA
+    ranges meet at T_mid: catches a mis-indexed or mis-assigned coefficient."""
+    T = np.array([500.0, 800.0, 1400.0, 2500.0])
+    dT = 1e-3
+    cpr, hrt, sr = mech.thermo(T)
+    _, hp, sp_ = mech.thermo(T + dT)
+    _, hm, sm = mech.thermo(T - dT)
+    dh = ((hp * (T + dT)[:, None]) - (hm * (T - dT)[:, None])) / (2 * dT)  # d(h/R)/dT
+    ds = (sp_ - sm) / (2 * dT)
+    assert np.abs(dh - cpr).max() < 1e-6
+    assert np.abs(ds - cpr / T[:, None]).max() < 1e-8
+    lo = mech.thermo(np.array([1000.0]))
+    hi_c = mech.coeffs(np.array([1000.0 + 1e-9]))
+    cp_hi = (hi_c[..., 0] + hi_c[..., 1] * 1000.0 + hi_c[..., 2] * 1e6 + hi_c[..., 3] * 1e9
+             + hi_c[..., 4] * 1e12)
+    assert np.abs(cp_hi - lo[0]).max() < 5e-3 * np.abs(lo[0]).max()
+
+
+def test_published_rate_constant(mech):
+    """H + O2 = OH + O of Jachimowski (1988): A = 2.6e14 cm^3/mol/s,
+    E = 16.8 kcal/mol, evaluated with R in cal/mol/K; 1 cm^3/mol = 1e-3 m^3/kmol."""
+    T = 1500.0
+    Rcal = 8.314462618 / 4.184
+    k_pub = 2.6e14 * np.exp(-16800.0 / (Rcal * T)) * 1e-3
+    r = mech.rx[1]
+    k = r["A"] * T ** r["b"] * np.exp(-r["EaR"] / T)
+    assert abs(k / k_pub - 1) < 1e-9
+
+
+def equilibrium(mech, T, p, X0):
+    """Chemical equilibrium at (T,p) from the element-potential equations:
+    mole fractions x_k = exp(-g_k - ln(p/p0) + sum_e lambda_e a_ek) with
+    sum x_k = 1 and the element totals of X0 fixed (moles N of mixture)."""
+    els = "HON"
+    A = np.array([[mech.comp[s].get(e, 0.0) for s in mech.names] for e in els])
+    b = A @ X0
+    _, hrt, sr = mech.thermo(np.array([T]))
+    g = (hrt - sr)[0] + np.log(p / mech.p0)
+
+    def res(z):
+        lam, lnN = z[:3], z[3]
+        x = np.exp(-g + A.T @ lam)
+        return np.concatenate([np.log(np.exp(lnN) * (A @ x)) - np.log(b), [np.log(x.sum())]])
+
+    # start from the potentials that reproduce a major-species guess
+    xg = np.maximum(X0, 1e-6)
+    xg = xg / xg.sum()
+    lam0 = np.linalg.lstsq(A.T, g + np.log(xg), rcond=None)[0]
+    z = fsolve(res, np.concatenate([lam0, [0.0]]), xtol=1e-15)
+    for _ in range(3):
+        z = fsolve(res, z, xtol=1e-15)
+    assert np.abs(res(z)).max() < 1e-12
+    x = np.exp(-g + A.T @ z[:3])
+    return x / x.sum()
+
+
+def test_detailed_balance_at_equilibrium(mech):
+    T, p = 2200.0, 101325.0
+    ix = {s: k for k, s in enumerate(mech.names)}
+    X0 = np.zeros(mech.ns)
+    X0[ix["H2"]], X0[ix["O2"]], X0[ix["N2"]] = 2.0, 1.0, 3.76
+    X = equilibrium(mech, T, p, X0 / X0.sum())
+    Y = X * mech.W / (X * mech.W).sum()
+    wd, _, _, _, fwd, rev = mech.rates(np.array([T]), np.array([p]), Y[:, None], parts=True)
+    ratio = rev[:, 0] / fwd[:, 0]
+    assert np.abs(ratio - 1.0).max() < 1e-7, ratio
+
+
+@pytest.mark.gpu
+def test_gpu_rates_match_oracle(mech):
+    gm = AC.Mechanism(MECH)
+    assert gm.names == mech.names and np.allclose(gm.W, mech.W, rtol=0, atol=0)
+    T, p, Y = AC.sample_cells(mech.names, mech.W, 65536, 11)
+    wd, h, cp = gm.rates(T, p, Y)
+    wd0, h0, cp0, _ = mech.rates(T, p, Y)
+    for k in range(mech.ns):
+        den = np.abs(wd0[k]).max()
+        if den == 0:
+            assert np.all(wd[k] == 0)
+        else:
+            assert np.abs(wd[k] - wd0[k]).max() / den < 1e-12, mech.names[k]
+    assert np.abs(h - h0).max() / np.abs(h0).max() < 1e-13
+    assert np.abs(cp / cp0 - 1).max() < 1e-13
+
+
+@pytest.mark.gpu
+def test_gpu_rates_density_input_and_conservation(mech):
+    gm = AC.Mechanism(MECH)
+    T, p, Y = AC.sample_cells(mech.names, mech.W, 1 << 20, 5)
+    rho = p / (OC.R_GAS * T * (Y / mech.W[:, None]).sum(axis=0))
+    wd, h, cp = gm.rates(T, rho, Y, input_is_p=False)
+    wdp, _, _ = gm.rates(T, p, Y, input_is_p=True)
+    assert np.abs(wd - wdp).max() / np.abs(wdp).max() < 1e-12
+    scale = np.abs(wd).max(axis=0) + 1e-300
+    assert np.all(np.abs(wd.sum(axis=0)) <= 1e-13 * scale)
+    assert np.all(wd[mech.names.index("N2")] == 0.0)
