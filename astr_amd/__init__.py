@@ -130,17 +130,22 @@ class Flow:
     def init_tgv(self, noise=0.0, seed=0):
         _chk(lib().astr_hip_init_tgv(self.h, noise, seed))
 
-    def upload(self, q):
-        """q: (5, km+1, jm+1, im+1) C-order == q(0:im,0:jm,0:km,1:5) Fortran."""
-        q = np.ascontiguousarray(q, dtype=np.float64)
-        if q.shape != (5,) + self.shape:
-            raise AstrError(f"q must have shape {(5,) + self.shape}")
-        _chk(lib().astr_hip_upload(self.h, _p(q), 0))
+    def halo_shape(self, hm):
+        return tuple(s + 2 * hm for s in self.shape)
 
-    def download(self, prims=False):
-        q = np.empty((5,) + self.shape)
-        pr = np.empty((6,) + self.shape) if prims else None
-        _chk(lib().astr_hip_download(self.h, _p(q), _p(pr), 0))
+    def upload(self, q, hm=0):
+        """q: (5, km+1+2hm, jm+1+2hm, im+1+2hm) C-order, i.e. the reference's
+        q(-hm:im+hm,-hm:jm+hm,-hm:km+hm,1:5); halo cells are not read."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != (5,) + self.halo_shape(hm):
+            raise AstrError(f"q must have shape {(5,) + self.halo_shape(hm)}")
+        _chk(lib().astr_hip_upload(self.h, _p(q), hm))
+
+    def download(self, prims=False, hm=0):
+        """q (and rho,u,v,w,p,T) with halo hm; halo cells hold the periodic copies."""
+        q = np.empty((5,) + self.halo_shape(hm))
+        pr = np.empty((6,) + self.halo_shape(hm)) if prims else None
+        _chk(lib().astr_hip_download(self.h, _p(q), _p(pr), hm))
         return (q, pr) if prims else q
 
     def set_deltat(self, dt):

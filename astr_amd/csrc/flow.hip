@@ -617,33 +617,48 @@ __global__ void k_init_tgv(FlowArgs a, double amp, uint64_t seed) {
   }
 }
 
-// dense (halo hm) Fortran-layout host buffer <-> padded device fields
+// periodic index of a host-layout halo cell: q(-h)=q(n-h), q(n+h)=q(h)
+// (bchomovec, tgvsolver.F90:371-440); a direction with n = 0 (2-D) has one plane
+__device__ __forceinline__ int wrap_index(int i, int n) {
+  if (n == 0) return 0;
+  return i < 0 ? i + n : (i > n ? i - n : i);
+}
+
+// dense Fortran-layout host buffer f(-hm:im+hm,-hm:jm+hm,-hm:km+hm,nf) <->
+// padded device fields.  Upload reads the computed lattice 0..n only; download
+// writes every cell, halo cells holding their periodic copies.  The grid spans
+// the dense box (i fastest).
 __global__ void k_pack(const double* __restrict__ src, double* __restrict__ dst, FlowArgs a,
                        int nf, int hm, int to_device) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i > a.n[0]) return;
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jj = blockIdx.y, kk = blockIdx.z;
   const long long di = a.n[0] + 1 + 2 * hm, dj = a.n[1] + 1 + 2 * hm;
   const long long dk = a.n[2] + 1 + 2 * hm;
+  if (ii >= di) return;
+  const int i = ii - hm, j = jj - hm, k = kk - hm;
+  const bool inside = i >= 0 && i <= a.n[0] && j >= 0 && j <= a.n[1] && k >= 0 && k <= a.n[2];
+  if (to_device && !inside) return;
   const long long dfs = di * dj * dk;
-  const long long od = (long long)(k + hm) * di * dj + (long long)(j + hm) * di + (i + hm);
-  const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+  const long long od = (long long)kk * di * dj + (long long)jj * di + ii;
+  const long long o = (long long)wrap_index(k, a.n[2]) * a.plane +
+                      (long long)wrap_index(j, a.n[1]) * a.pitch + wrap_index(i, a.n[0]);
   for (int m = 0; m < nf; m++) {
     if (to_device) dst[m * a.fs + o] = src[m * dfs + od];
     else dst[m * dfs + od] = src[m * a.fs + o];
   }
 }
 
-// primitives rho,u,v,w,p,T (q2fvar) into 6 dense fields
+// primitives rho,u,v,w,p,T (q2fvar) into 6 dense fields, halo cells included
 __global__ void k_prims(FlowArgs a, double* __restrict__ out, int hm) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y, k = blockIdx.z;
-  if (i > a.n[0]) return;
+  const int ii = blockIdx.x * blockDim.x + threadIdx.x;
+  const int jj = blockIdx.y, kk = blockIdx.z;
   const long long di = a.n[0] + 1 + 2 * hm, dj = a.n[1] + 1 + 2 * hm;
   const long long dk = a.n[2] + 1 + 2 * hm;
+  if (ii >= di) return;
   const long long dfs = di * dj * dk;
-  const long long od = (long long)(k + hm) * di * dj + (long long)(j + hm) * di + (i + hm);
-  const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+  const long long od = (long long)kk * di * dj + (long long)jj * di + ii;
+  const long long o = (long long)wrap_index(kk - hm, a.n[2]) * a.plane +
+                      (long long)wrap_index(jj - hm, a.n[1]) * a.pitch + wrap_index(ii - hm, a.n[0]);
   const Prim P = prim_of(a, a.q[o], a.q[a.fs + o], a.q[2 * a.fs + o], a.q[3 * a.fs + o],
                          a.q[4 * a.fs + o]);
   out[od] = P.r;
@@ -845,6 +860,10 @@ int check_ctx(astr_hip_ctx* c) {
 }
 
 dim3 pt_grid(const FlowArgs& a) { return dim3((a.n[0] + 64) / 64, a.n[1] + 1, a.n[2] + 1); }
+// the dense host-layout box with halo hm
+dim3 dense_grid(const FlowArgs& a, int hm) {
+  return dim3((a.n[0] + 2 * hm + 64) / 64, a.n[1] + 1 + 2 * hm, a.n[2] + 1 + 2 * hm);
+}
 
 // launch one pass; with profiling on, bracket it with events on the stream
 int pass(astr_hip_ctx* c, int prog, int d) {
@@ -998,13 +1017,22 @@ int astr_hip_init_tgv(astr_hip_ctx* c, double noise_amp, unsigned long long seed
   return 0;
 }
 
+// halo depth of a host buffer: 0..8 and at most the shortest direction, so a
+// halo cell wraps once
+static bool bad_halo(const FlowArgs& a, int hm) {
+  if (hm < 0 || hm > 8) return true;
+  for (int d = 0; d < a.ndir; d++)
+    if (hm > a.n[d]) return true;
+  return false;
+}
+
 static int xfer(astr_hip_ctx* c, double* host, int nf, double* dev, int hm, int to_dev) {
   const FlowArgs& a = c->a;
   const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
   double* tmp = nullptr;
   HIP_OK(hipMalloc(&tmp, dn * nf * sizeof(double)));
   if (to_dev) HIP_OK(hipMemcpyAsync(tmp, host, dn * nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_pack, pt_grid(a), dim3(64), 0, c->stream, to_dev ? tmp : dev,
+  hipLaunchKernelGGL(k_pack, dense_grid(a, hm), dim3(64), 0, c->stream, to_dev ? tmp : dev,
                      to_dev ? dev : tmp, a, nf, hm, to_dev);
   HIP_OK(hipGetLastError());
   if (!to_dev) HIP_OK(hipMemcpyAsync(host, tmp, dn * nf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1015,20 +1043,20 @@ static int xfer(astr_hip_ctx* c, double* host, int nf, double* dev, int hm, int 
 
 int astr_hip_upload(astr_hip_ctx* c, const double* q, int hm) {
   if (check_ctx(c)) return -1;
-  if (!q || hm < 0) return fail("bad argument to astr_hip_upload");
+  if (!q || bad_halo(c->a, hm)) return fail("bad argument to astr_hip_upload");
   return xfer(c, const_cast<double*>(q), 5, c->a.q, hm, 1);
 }
 
 int astr_hip_download(astr_hip_ctx* c, double* q, double* prims, int hm) {
   if (check_ctx(c)) return -1;
-  if (hm < 0) return fail("bad argument to astr_hip_download");
+  if (bad_halo(c->a, hm)) return fail("bad argument to astr_hip_download");
   if (q && xfer(c, q, 5, c->a.q, hm, 0)) return -1;
   if (prims) {
     const FlowArgs& a = c->a;
     const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
     double* tmp = nullptr;
     HIP_OK(hipMalloc(&tmp, dn * 6 * sizeof(double)));
-    hipLaunchKernelGGL(k_prims, pt_grid(a), dim3(64), 0, c->stream, a, tmp, hm);
+    hipLaunchKernelGGL(k_prims, dense_grid(a, hm), dim3(64), 0, c->stream, a, tmp, hm);
     HIP_OK(hipGetLastError());
     HIP_OK(hipMemcpyAsync(prims, tmp, dn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));

@@ -10,11 +10,14 @@ stores inputs and outputs as .npz data:
                    the reference miniapp (tgvsolver.F90) on a seeded TGV field
   rk3_16.npz       10 RK3 steps of the miniapp at 16^3, dt=2e-3: stacal
                    history and final q
-  hist64.npz       BASELINE config 1: TGV 64^3, dt=0.005, 200 steps, KE and
-                   enstrophy history (+ final-q checksums)
+  hist64.npz       TGV 64^3 from the closed-form field, dt=0.005, 200 steps:
+                   KE and enstrophy history (+ final-q checksums)
+  hist128_noise.npz TGV 128^3 with seeded 1e-3 Gaussian velocity noise
+                   (Philox4x32-10 + Box-Muller, seed 2024), dt=1e-3,
+                   1000 steps: KE and enstrophy history (+ final-q checksums)
 
 Only run where the reference tree and its build exist:
-    make -C oracle all && python tests/golden/make_golden.py [--skip-hist64]
+    make -C oracle all && python tests/golden/make_golden.py [--skip-hist64] [--skip-hist128]
 """
 import os
 import sys
@@ -66,10 +69,25 @@ def hist64():
                         q_sum=qo.reshape(5, -1).sum(axis=1), q_absmax=np.abs(qo).reshape(5, -1).max(axis=1))
 
 
+def hist128_noise():
+    q = seeded_q(128, 128, 128, 1e-3, 2024)
+    r = O.run_ref("rk3", dims=[128, 128, 128], dt=1e-3, q=q, nsteps=1000, timeout=4 * 3600)
+    qo = r["q_out"]
+    np.savez_compressed(os.path.join(HERE, "hist128_noise.npz"), hist=r["hist"],
+                        q_sum=qo.reshape(5, -1).sum(axis=1), q_absmax=np.abs(qo).reshape(5, -1).max(axis=1),
+                        seconds=r["seconds"])
+
+
 if __name__ == "__main__":
+    only = [a.split("=", 1)[1] for a in sys.argv[1:] if a.startswith("--only=")]
+    if only:
+        globals()[only[0]]()
+        sys.exit(0)
     lines()
     stage()
     rk3_16()
     if "--skip-hist64" not in sys.argv:
         hist64()
+    if "--skip-hist128" not in sys.argv:
+        hist128_noise()
     print("fixtures written to", HERE)

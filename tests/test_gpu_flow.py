@@ -63,6 +63,23 @@ def test_upload_download_roundtrip():
     assert relmax(pr, s.prims) < 1e-14
 
 
+@pytest.mark.parametrize("dims", [(20, 18, 16), (24, 22, 0)])
+def test_halo_download(dims):
+    """hm=5 host buffers: upload ignores the halo cells, download fills them
+    with the periodic copies q(-h)=q(n-h), q(n+h)=q(h)."""
+    f, s = make_pair(*dims)
+    hm = 5
+    big = np.full((5,) + f.halo_shape(hm), np.nan)
+    big[(slice(None),) + tuple(slice(hm, hm + n) for n in f.shape)] = s.q
+    f.upload(big, hm)
+    q, pr = f.download(prims=True, hm=hm)
+    idx = [np.array([(x - hm) % (n - 1) if n > 1 and not 0 <= x - hm < n else
+                     (0 if n == 1 else x - hm) for x in range(n + 2 * hm)]) for n in f.shape]
+    want = s.q[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]
+    assert np.array_equal(q, want)
+    assert relmax(pr, s.prims[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]) < 1e-14
+
+
 def test_init_tgv_matches_oracle():
     for amp in (0.0, 1e-3):
         f = A.Flow(A.tgv_config(24))
@@ -120,3 +137,34 @@ def test_state_ref_128():
     ke, en = f.rk3(100)
     assert relmax(ke, ref[:, 2]) < HIST_TOL
     assert relmax(en, ref[:, 3]) < HIST_TOL
+
+
+def _hist_check(name, n, dt, amp, seed):
+    z = np.load(os.path.join(GOLD, name))
+    hist = z["hist"]
+    f = A.Flow(A.tgv_config(n, deltat=dt))
+    f.init_tgv(amp, seed)
+    ke, en = f.rk3(hist.shape[0])
+    assert relmax(ke, hist[:, 2]) < HIST_TOL
+    assert relmax(en, hist[:, 3]) < HIST_TOL
+    q = f.download()
+    absmax = np.abs(q).reshape(5, -1).max(axis=1)
+    assert np.all(np.abs(absmax - z["q_absmax"]) <= HIST_TOL * z["q_absmax"])
+    qsum = q.reshape(5, -1).sum(axis=1)
+    for m in (0, 4):  # density and energy totals (momentum totals are ~0)
+        assert abs(qsum[m] - z["q_sum"][m]) <= HIST_TOL * abs(z["q_sum"][m])
+
+
+def test_hist64_reference():
+    """TGV 64^3 from the closed-form field, dt=0.005, 200 steps, against the
+    history the reference Fortran (tgvsolver.F90 stacal) wrote for the same
+    case; also the final state's per-variable max and density/energy totals."""
+    _hist_check("hist64.npz", 64, 0.005, 0.0, 0)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(GOLD, "hist128_noise.npz")),
+                    reason="fixture hist128_noise.npz not generated")
+def test_hist128_noise_1000_steps():
+    """TGV 128^3 with seeded 1e-3 Gaussian velocity noise, dt=1e-3, 1000 RK3
+    steps against the reference Fortran run on the identical seeded field."""
+    _hist_check("hist128_noise.npz", 128, 1e-3, 1e-3, 2024)
