@@ -47,8 +47,10 @@ EXPORTS = [
     "astr_hip_set_deltat", "astr_hip_field_info", "astr_hip_init_tgv", "astr_hip_upload",
     "astr_hip_download", "astr_hip_filterq", "astr_hip_gradcal", "astr_hip_rhscal",
     "astr_hip_rk3_steps", "astr_hip_sync", "astr_hip_line_batch",
-    "astr_hip_profile_begin", "astr_hip_profile_end",
+    "astr_hip_profile_begin", "astr_hip_profile_end", "astr_hip_autotune",
+    "astr_hip_get_pass_config", "astr_hip_set_pass_config",
 ]
+SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5)}
 PROF_SLOTS = 21
 PASS_NAMES = ["gc_first", "gc_mid", "gc_last", "visc_mid", "visc_last_rhs",
               "visc_last_rk3_filter", "filter"]
@@ -81,6 +83,10 @@ def lib():
                                           C.c_double, C.c_double]
         L.astr_hip_profile_begin.argtypes = [C.c_void_p, C.c_int]
         L.astr_hip_profile_end.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
+        L.astr_hip_autotune.argtypes = [C.c_void_p, C.c_int]
+        L.astr_hip_get_pass_config.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                               C.POINTER(C.c_int), C.POINTER(C.c_float)]
+        L.astr_hip_set_pass_config.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -190,6 +196,30 @@ class Flow:
 
     def sync(self):
         _chk(lib().astr_hip_sync(self.h))
+
+    # -- launch selection ------------------------------------------------
+    def autotune(self, reps=3):
+        """Time every fitting (shape, register budget) of every pass and keep
+        the fastest; the state is left as it was."""
+        _chk(lib().astr_hip_autotune(self.h, reps))
+        return self.pass_configs()
+
+    def pass_configs(self):
+        """{(pass_name, dir): (shape, minb, tuned_ms)} of the passes this case launches."""
+        nd = 3 if self.cfg.km > 0 else 2
+        used = {0: [0], 1: list(range(1, nd - 1)), 2: [nd - 1], 3: list(range(nd - 1)),
+                4: [nd - 1], 5: [nd - 1], 6: list(range(nd))}
+        out = {}
+        for prog, dirs in used.items():
+            for d in dirs:
+                sh, mb, ms = C.c_int(), C.c_int(), C.c_float()
+                _chk(lib().astr_hip_get_pass_config(self.h, prog, d, C.byref(sh), C.byref(mb),
+                                                    C.byref(ms)))
+                out[(PASS_NAMES[prog], "xyz"[d])] = (sh.value, mb.value, float(ms.value))
+        return out
+
+    def set_pass_config(self, prog, d, shape, minb):
+        _chk(lib().astr_hip_set_pass_config(self.h, prog, d, shape, minb))
 
 
 def line_batch(op, f, alfa_filter=0.49, beta_halo=1.11):

@@ -11,6 +11,8 @@ import sys
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 SOURCES = ["flow.hip", "chem.hip"]
+# passes.hip is compiled once per launch shape (-DASTR_SHAPE_ID=k)
+PASS_SHAPES = 5
 OUT = os.path.join(PKG, "libastr_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
@@ -26,16 +28,38 @@ FLAGS = [
 ]
 
 
+def _compile(job):
+    src, obj, defs = job
+    cmd = [HIPCC, *[f for f in FLAGS if f != "-shared"], *defs, "-c", "-o", obj, src]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    return cmd, r
+
+
 def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    """Compile the translation units in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    objdir = os.path.join(CSRC, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    jobs = [(os.path.join(CSRC, s), os.path.join(objdir, s.replace(".hip", ".o")), [])
+            for s in SOURCES]
+    jobs += [(os.path.join(CSRC, "passes.hip"), os.path.join(objdir, f"passes_s{k}.o"),
+              [f"-DASTR_SHAPE_ID={k}"]) for k in range(PASS_SHAPES)]
+    workers = max(1, min(len(jobs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4))))
+    with ThreadPoolExecutor(workers) as ex:
+        results = list(ex.map(_compile, jobs))
+    for cmd, r in results:
+        if r.returncode != 0:
+            sys.stderr.write(r.stdout + r.stderr)
+            raise RuntimeError("hipcc failed: " + " ".join(cmd))
+        if verbose:
+            sys.stderr.write(r.stderr)
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, *FLAGS, "-o", tmp, *srcs]
+    cmd = [HIPCC, "--offload-arch=gfx950", "-fPIC", "-shared", "-o", tmp, *[j[1] for j in jobs]]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed for " + " ".join(SOURCES))
-    if verbose:
-        sys.stderr.write(r.stderr)
+        raise RuntimeError("link failed: " + " ".join(cmd))
     os.replace(tmp, OUT)
     return OUT
 

@@ -34,7 +34,7 @@
 #include <vector>
 
 #include "../../include/astr_hip.h"
-#include "lines.hpp"
+#include "flow_dev.hpp"
 
 using namespace astr;
 
@@ -53,484 +53,6 @@ int fail(const std::string& m) {
     if (e_ != hipSuccess)                                                          \
       return fail(std::string(#x) + ": " + hipGetErrorString(e_));                 \
   } while (0)
-
-// ---------------------------------------------------------------------------
-// device-side description of one case
-// ---------------------------------------------------------------------------
-struct FlowArgs {
-  int n[3];             // im, jm, km
-  int ndir;             // 3, or 2 when km==0
-  int pitch;            // doubles between rows (j)
-  long long plane;      // doubles between planes (k)
-  long long fs;         // doubles between fields
-  double inv_d[3];      // 1/dx, 1/dy, 1/dz
-  double const2, const5, const6;
-  double reynolds, prandtl, suth, suth1;
-  double rk_a, rk_b, rk_c, dt;
-  int trash;            // a point index inside every field's padding
-  double alfa, bhalo;   // filter alpha and halo-row beta (LHS of the filter rows)
-  int stage;            // 0,1,2
-  int dostats;          // stage-1 KE/enstrophy partial sums
-  int store_last_grad;  // last GC pass also stores its gradients
-  double* q;            // 5 fields
-  double* qsave;        // 5 fields
-  double* qrhs;         // 5 fields
-  double* dv;           // 12 fields: dvel(m,d) at m+3d, dtmp(d) at 9+d
-  double* sg;           // 9 fields: sigma 11,12,13,22,23,33, qflux 1,2,3
-  double* stats;        // 2 doubles per tile
-  const double* dac[3]; // derivative LU tables [ac1|ac2|ac3] per direction
-  const double* fac[3]; // filter LU tables
-  FilterCoef fc;
-};
-
-enum Prog { GC_FIRST = 0, GC_MID, GC_LAST, V_MID, V_LAST_RHS, V_LAST_RK, F_ONLY };
-
-// ntiles of direction dir for lines-per-tile TL
-__host__ __device__ inline int tiles_of(const int n[3], int dir, int TL) {
-  const int ni = (n[0] + TL) / TL;  // ceil((im+1)/TL)
-  if (dir == 0) return ((n[1] + 1) * (n[2] + 1) + TL - 1) / TL;
-  if (dir == 1) return ni * (n[2] + 1);
-  return ni * (n[1] + 1);
-}
-
-// Blocks are handed to the 8 XCDs round robin; give each XCD a contiguous
-// range of tiles so tiles sharing cache lines share an L2.
-__device__ __forceinline__ int xcd_tile(int b, int nb) {
-  const int x = b & 7, s = b >> 3, q = nb >> 3, r = nb & 7;
-  return x * q + (x < r ? x : r) + s;
-}
-
-template <class C>
-struct Slots {
-  int g[C::NPT];  // offset inside a field; the trash point for lines past the
-                  // end of the lattice (partial last tile)
-  int s[C::NPT];  // index of the point in a line buffer (l*LT + r)
-  int h[C::NPT];  // line-buffer index of the slot's periodic halo copy
-  int interior;   // bit t: i,j,k >= 1 (stats domain)
-};
-
-// LDS of a pass: the output lines X (TL x LT), then the input lines T
-// (TL x LT), then a write-only dummy cell.  A slot holding no point (past the
-// tile) or no halo copy points s/h at index TL*LT: its T writes land in the
-// dummy cell (T + kLineOff + TL*LT) and its X reads return T's zero padding
-// (X + TL*LT = row -8 of T's first line), so no slot needs a guard.
-template <class C>
-constexpr int lds_doubles() {
-  return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
-}
-
-// Map the tile's points onto slots.  dir 0: lines are (j,k), rows i; dir 1:
-// lines (i,k) rows j; dir 2: lines (i,j) rows k.
-template <class C>
-__device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int dir, int tile) {
-  const int n = a.n[dir];
-  const int np = C::TL * (n + 1);
-  const int ni = (a.n[0] + C::TL) / C::TL;
-  const int dummy = C::TL * C::LT;
-  S.interior = 0;
-#pragma unroll
-  for (int t = 0; t < C::NPT; t++) {
-    const int e = threadIdx.x + t * C::NT;
-    S.g[t] = a.trash;
-    S.s[t] = dummy;
-    S.h[t] = dummy;
-    if (e < np) {
-      int l, r, i, j, k;
-      bool ok;
-      if (dir == 0) {
-        l = e / (n + 1);
-        r = e - l * (n + 1);
-        const int L = tile * C::TL + l;
-        j = L % (a.n[1] + 1);
-        k = L / (a.n[1] + 1);
-        i = r;
-        ok = L < (a.n[1] + 1) * (a.n[2] + 1);
-      } else {
-        l = e & (C::TL - 1);
-        r = e / C::TL;
-        const int it = tile % ni, o = tile / ni;
-        i = it * C::TL + l;
-        ok = i <= a.n[0];
-        if (dir == 1) { j = r; k = o; } else { j = o; k = r; }
-      }
-      S.s[t] = l * C::LT + r;
-      // rows n-5..n-1 are copied to the halo rows -5..-1, rows 1..5 to
-      // n+1..n+5 (disjoint for n >= 11)
-      if (r >= n - 5 && r <= n - 1) S.h[t] = S.s[t] - n;
-      if (r >= 1 && r <= 5) S.h[t] = S.s[t] + n;
-      if (ok) {
-        S.g[t] = (int)((long long)k * a.plane + (long long)j * a.pitch + i);
-        const bool in = i >= 1 && j >= 1 && (a.ndir == 2 || k >= 1);
-        if (in) S.interior |= 1 << t;
-      }
-    }
-  }
-}
-
-// store slot t's value into its input line with the periodic halo copies
-// q(-h)=q(n-h), q(n+h)=q(h) (tgvsolver.F90:371-440)
-template <class C>
-__device__ __forceinline__ void put(double* T, const Slots<C>& S, int t, double v) {
-  T[kLineOff + S.s[t]] = v;
-  T[kLineOff + S.h[t]] = v;
-}
-
-// zero the padding rows -8..-6 and n+6..P*R+2 of every input line (never
-// written by put; read by the chunks at the ends of a line)
-template <class C>
-__device__ __forceinline__ void zero_pad(double* T, int n) {
-  const int npad = 3 + (C::P * C::R + 2 - (n + 6) + 1);
-  for (int e = threadIdx.x; e < C::TL * npad; e += C::NT) {
-    const int l = e / npad, k = e - l * npad;
-    const int row = k < 3 ? k - 8 : n + 6 + (k - 3);
-    T[l * C::LT + kLineOff + row] = 0.0;
-  }
-}
-
-// q2fvar + thermal_scar (tgvsolver.F90:73-121).  Pointwise thermodynamics is
-// evaluated without FMA contraction and in the reference's operation order:
-// p ~ 1/(gamma Ma^2) is large against its gradient, so one rounding step in p
-// becomes a visible error in grad p.
-__device__ __forceinline__ double pressure_of(const FlowArgs& a, double q1, double q2, double q3,
-                                              double q4, double q5) {
-#pragma clang fp contract(off)
-  const double u = q2 / q1, v = q3 / q1, w = q4 / q1;
-  return (q5 - 0.5 * q1 * (u * u + v * v + w * w)) / a.const6;
-}
-
-struct Prim {
-  double r, u, v, w, p, t;
-};
-
-__device__ __forceinline__ Prim prim_of(const FlowArgs& a, double q1, double q2, double q3,
-                                        double q4, double q5) {
-#pragma clang fp contract(off)
-  Prim P;
-  P.r = q1;
-  P.u = q2 / q1;
-  P.v = q3 / q1;
-  P.w = q4 / q1;
-  P.p = (q5 - 0.5 * q1 * (P.u * P.u + P.v * P.v + P.w * P.w)) / a.const6;
-  P.t = P.p / q1 * a.const2;
-  return P;
-}
-
-// convective flux component m along dir (tgvsolver.F90:559-564, 586-591,
-// 613-618) from q_m, the velocity along dir and p
-__device__ __forceinline__ double conv_flux(int m, int dir, double qm, double q5, double uu,
-                                            double p) {
-#pragma clang fp contract(off)
-  if (m == 4) return (q5 + p) * uu;
-  double f = qm * uu;
-  if (m == dir + 1) f = f + p;
-  return f;
-}
-
-// Sutherland (tgvsolver.F90:123-140)
-__device__ __forceinline__ double miucal(const FlowArgs& a, double t) {
-  return t * sqrt(t) * a.suth1 / (t + a.suth);
-}
-
-// q <- a*qsave + b*q + c*qrhs*dt in the reference's order (tgvsolver.F90:1055-1057)
-__device__ __forceinline__ double rk_update(const FlowArgs& a, double qs, double qo, double qr) {
-#pragma clang fp contract(off)
-  return a.rk_a * qs + a.rk_b * qo + a.rk_c * qr * a.dt;
-}
-
-// stress, heat flux (tgvsolver.F90:663-691); g[m][d] = d u_m / d x_d
-__device__ __forceinline__ void visc_flux(const FlowArgs& a, const double g[3][3],
-                                          const double gt[3], const Prim& P, double out[9]) {
-#pragma clang fp contract(off)
-  const double s11 = g[0][0];
-  const double s12 = 0.5 * (g[0][1] + g[1][0]);
-  const double s13 = 0.5 * (g[0][2] + g[2][0]);
-  const double s22 = g[1][1];
-  const double s23 = 0.5 * (g[1][2] + g[2][1]);
-  const double s33 = g[2][2];
-  const double skk = (1.0 / 3.0) * (s11 + s22 + s33);
-  const double miu = miucal(a, P.t) / a.reynolds;
-  const double miu2 = 2.0 * miu;
-  const double hcc = (miu / a.prandtl) / a.const5;
-  out[0] = miu2 * (s11 - skk);
-  out[1] = miu2 * s12;
-  out[2] = miu2 * s13;
-  out[3] = miu2 * (s22 - skk);
-  out[4] = miu2 * s23;
-  out[5] = miu2 * (s33 - skk);
-  out[6] = hcc * gt[0] + out[0] * P.u + out[1] * P.v + out[2] * P.w;
-  out[7] = hcc * gt[1] + out[1] * P.u + out[3] * P.v + out[4] * P.w;
-  out[8] = hcc * gt[2] + out[2] * P.u + out[4] * P.v + out[5] * P.w;
-}
-
-// ---------------------------------------------------------------------------
-// the pass kernel
-// ---------------------------------------------------------------------------
-#ifndef ASTR_MINB_GC
-#define ASTR_MINB_GC 2
-#endif
-#ifndef ASTR_MINB_V
-#define ASTR_MINB_V 1
-#endif
-#ifndef ASTR_MINB_F
-#define ASTR_MINB_F 1
-#endif
-// workgroups per CU the register allocation is held to, per pass kind
-template <int PROG>
-constexpr int pass_minb() {
-  return PROG <= GC_LAST ? ASTR_MINB_GC : (PROG == F_ONLY ? ASTR_MINB_F : ASTR_MINB_V);
-}
-
-template <class C, int PROG>
-__global__ __launch_bounds__(C::NT, pass_minb<PROG>()) void k_pass(FlowArgs a, int dir) {
-  extern __shared__ double smem[];
-  constexpr int NPT = C::NPT;
-  const int n = a.n[dir];
-  double* X = smem;                 // output lines, TL x LT (rows at offset 0)
-  double* T = smem + C::TL * C::LT; // input lines, TL x LT
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  Slots<C> S;
-  make_slots<C>(S, a, dir, tile);
-  zero_pad<C>(T, n);
-  const int ln = threadIdx.x / C::P, p = threadIdx.x % C::P;
-  double* Tl = T + ln * C::LT;
-  double* Xl = X + ln * C::LT;
-  const double idn = a.inv_d[dir];
-  const long long fs = a.fs;
-
-  if constexpr (PROG == GC_FIRST || PROG == GC_MID || PROG == GC_LAST) {
-    constexpr bool LAST = PROG == GC_LAST;
-    ChunkCoef<C::R, true> CD;
-    load_chunk_coef<C::R, true>(CD, a.dac[dir], n + 3, p * C::R);
-    double qv[5][NPT];
-#pragma unroll
-    for (int t = 0; t < NPT; t++)
-#pragma unroll
-      for (int m = 0; m < 5; m++) qv[m][t] = a.q[m * fs + S.g[t]];
-
-    // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
-    double gz[4][NPT];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-        {
-          double f = 0.0;
-          {
-            if (c < 3) {
-              f = qv[c + 1][t] / qv[0][t];
-            } else {
-              const double pr = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
-              f = pr / qv[0][t] * a.const2;
-            }
-          }
-          put<C>(T, S, t, f);
-        }
-      }
-      __syncthreads();
-      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
-      __syncthreads();
-      const int fld = c < 3 ? c + 3 * dir : 9 + dir;
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-        {
-          const double v = X[S.s[t]] * idn;
-          if (!LAST || a.store_last_grad) a.dv[fld * fs + S.g[t]] = v;
-          gz[c][t] = v;
-        }
-      }
-    }
-
-    if constexpr (LAST) {
-      // stress and heat flux (diffusion, tgvsolver.F90:659-694) and the
-      // stage-1 KE / enstrophy partial sums (stacal, :893-942)
-      double sk = 0.0, se = 0.0;
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-        const int o = S.g[t];
-        double g[3][3], gt[3];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-          if (d == dir) {
-            g[0][d] = gz[0][t]; g[1][d] = gz[1][t]; g[2][d] = gz[2][t]; gt[d] = gz[3][t];
-          } else if (d < a.ndir) {
-            g[0][d] = a.dv[(0 + 3 * d) * fs + o];
-            g[1][d] = a.dv[(1 + 3 * d) * fs + o];
-            g[2][d] = a.dv[(2 + 3 * d) * fs + o];
-            gt[d] = a.dv[(9 + d) * fs + o];
-          } else {
-            g[0][d] = 0.0; g[1][d] = 0.0; g[2][d] = 0.0; gt[d] = 0.0;
-          }
-        }
-        const Prim P = prim_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
-        double f[9];
-        visc_flux(a, g, gt, P, f);
-#pragma unroll
-        for (int m = 0; m < 9; m++) a.sg[m * fs + o] = f[m];
-        if (a.dostats && ((S.interior >> t) & 1)) {
-          sk += P.r * (P.u * P.u + P.v * P.v + P.w * P.w);
-          const double o1 = g[2][1] - g[1][2];
-          const double o2 = g[0][2] - g[2][0];
-          const double o3 = g[1][0] - g[0][1];
-          se += P.r * (o1 * o1 + o2 * o2 + o3 * o3);
-        }
-      }
-      if (a.dostats) {
-        // deterministic block reduction into this tile's slot
-        __syncthreads();
-        X[threadIdx.x] = sk;
-        X[C::NT + threadIdx.x] = se;
-        __syncthreads();
-        for (int h = C::NT / 2; h > 0; h >>= 1) {
-          if ((int)threadIdx.x < h) {
-            X[threadIdx.x] += X[threadIdx.x + h];
-            X[C::NT + threadIdx.x] += X[C::NT + threadIdx.x + h];
-          }
-          __syncthreads();
-        }
-        if (threadIdx.x == 0) {
-          a.stats[2 * tile] = X[0];
-          a.stats[2 * tile + 1] = X[C::NT];
-        }
-        __syncthreads();
-      }
-    }
-
-    // convective flux derivatives along dir (convection, tgvsolver.F90:539-633)
-    double ud[NPT], pp[NPT];
-#pragma unroll
-    for (int t = 0; t < NPT; t++) {
-      pp[t] = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
-      ud[t] = (dir == 0 ? qv[1][t] : (dir == 1 ? qv[2][t] : qv[3][t])) / qv[0][t];
-    }
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-#pragma unroll
-      for (int t = 0; t < NPT; t++)
-        put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
-      double* dst = a.qrhs + m * fs;
-      double qr[NPT];
-      if (PROG != GC_FIRST) {
-        // issued before the solve so the load latency overlaps it
-#pragma unroll
-        for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
-      }
-      __syncthreads();
-      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-        {
-          const double v = X[S.s[t]] * idn;
-          if (PROG == GC_FIRST) dst[S.g[t]] = v;
-          else if (PROG == GC_MID) dst[S.g[t]] = qr[t] + v;
-          else dst[S.g[t]] = -(qr[t] + v);
-        }
-      }
-    }
-  } else if constexpr (PROG == V_MID || PROG == V_LAST_RHS || PROG == V_LAST_RK) {
-    // viscous-flux derivatives (diffusion, tgvsolver.F90:700-787); columns
-    // per direction: x (s11,s12,s13,q1), y (s12,s22,s23,q2), z (s13,s23,s33,q3)
-    ChunkCoef<C::R, true> CD;
-    load_chunk_coef<C::R, true>(CD, a.dac[dir], n + 3, p * C::R);
-    const int c0 = dir == 0 ? 0 : (dir == 1 ? 1 : 2);
-    const int c1 = dir == 0 ? 1 : (dir == 1 ? 3 : 4);
-    const int c2 = dir == 0 ? 2 : (dir == 1 ? 4 : 5);
-    double acc[4][NPT];
-    double nx[NPT];
-#pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = a.sg[c0 * fs + S.g[t]];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-#pragma unroll
-      for (int t = 0; t < NPT; t++)
-        put<C>(T, S, t, nx[t]);
-      // loads for this column's accumulation and the next column's input are
-      // issued before the solve so their latency overlaps it
-      double qr[NPT];
-      double* dst = a.qrhs + (1 + c) * fs;
-#pragma unroll
-      for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
-      if (c < 3) {
-        const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
-        const double* src = a.sg + col * fs;
-#pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = src[S.g[t]];
-      }
-      __syncthreads();
-      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-        {
-          const double v = qr[t] + X[S.s[t]] * idn;
-          if (PROG == V_LAST_RK) acc[c][t] = v;
-          else dst[S.g[t]] = v;
-        }
-      }
-    }
-    if constexpr (PROG == V_LAST_RK) {
-      // RK3 stage update (tgvsolver.F90:1043-1059), then the filter along dir
-      double qn[5][NPT];
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-#pragma unroll
-        for (int m = 0; m < 5; m++) qn[m][t] = 0.0;
-        const int o = S.g[t];
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-          const double qr = m == 0 ? a.qrhs[o] : acc[m - 1][t];
-          const double qo = a.q[m * fs + o];
-          double qs;
-          if (a.stage == 0) {
-            a.qsave[m * fs + o] = qo;
-            qs = qo;
-          } else {
-            qs = a.qsave[m * fs + o];
-          }
-          qn[m][t] = rk_update(a, qs, qo, qr);
-        }
-      }
-      ChunkCoef<C::R, false> CF;
-      load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
-#pragma unroll
-      for (int m = 0; m < 5; m++) {
-#pragma unroll
-        for (int t = 0; t < NPT; t++)
-          put<C>(T, S, t, qn[m][t]);
-        __syncthreads();
-        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
-        __syncthreads();
-        double* dst = a.q + m * fs;
-#pragma unroll
-        for (int t = 0; t < NPT; t++)
-          dst[S.g[t]] = X[S.s[t]];
-      }
-    }
-  } else {  // F_ONLY: filter of q along dir, in place (filterq, tgvsolver.F90:791-883)
-    ChunkCoef<C::R, false> CF;
-    load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
-    double nx[NPT];
-#pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = a.q[S.g[t]];
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
-#pragma unroll
-      for (int t = 0; t < NPT; t++)
-        put<C>(T, S, t, nx[t]);
-      if (m < 4) {
-#pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = a.q[(m + 1) * fs + S.g[t]];
-      }
-      __syncthreads();
-      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
-      __syncthreads();
-      double* dst = a.q + m * fs;
-#pragma unroll
-      for (int t = 0; t < NPT; t++)
-        dst[S.g[t]] = X[S.s[t]];
-    }
-  }
-}
 
 // sum the per-tile stats in a fixed order (one block of 256 threads)
 __global__ __launch_bounds__(256) void k_stats_reduce(const double* __restrict__ part, int nt,
@@ -701,20 +223,13 @@ __global__ __launch_bounds__(C::NT) void k_lines(const double* __restrict__ f, d
 // ---------------------------------------------------------------------------
 // launch-shape table
 // ---------------------------------------------------------------------------
-using CfgA = LineCfg<256, 4, 5, 5>;    // P=64, n <= 313
-using CfgB = LineCfg<256, 4, 9, 9>;    // P=64, n <= 569
-using CfgC = LineCfg<256, 16, 5, 5>;   // P=16, n <= 73
-using CfgD = LineCfg<256, 8, 9, 9>;    // P=32, n <= 281
-using CfgE = LineCfg<256, 8, 5, 5>;    // P=32, n <= 153
-
 struct ShapeInfo {
   int id, NT, TL, P, R, NPT;
 };
-const ShapeInfo kShapes[] = {
+const ShapeInfo kShapes[kNumShapes] = {
     {0, 256, 4, 64, 5, 5}, {1, 256, 4, 64, 9, 9}, {2, 256, 16, 16, 5, 5},
     {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5},
 };
-constexpr int kNumShapes = sizeof(kShapes) / sizeof(kShapes[0]);
 
 bool shape_fits(const ShapeInfo& s, int n) {
   return s.P * s.R >= n + 7 && s.NT * s.NPT >= s.TL * (n + 1) && n >= 8;
@@ -734,34 +249,19 @@ int pick_shape(int n) {
   return best;
 }
 
-template <class C>
-hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
-  const int nt = tiles_of(a.n, dir, C::TL);
-  const size_t lds = (size_t)lds_doubles<C>() * sizeof(double);
-  dim3 g(nt), b(C::NT);
-  switch (prog) {
-    case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST>), g, b, lds, st, a, dir); break;
-    case GC_MID: hipLaunchKernelGGL((k_pass<C, GC_MID>), g, b, lds, st, a, dir); break;
-    case GC_LAST: hipLaunchKernelGGL((k_pass<C, GC_LAST>), g, b, lds, st, a, dir); break;
-    case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID>), g, b, lds, st, a, dir); break;
-    case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS>), g, b, lds, st, a, dir); break;
-    case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK>), g, b, lds, st, a, dir); break;
-    default: hipLaunchKernelGGL((k_pass<C, F_ONLY>), g, b, lds, st, a, dir); break;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_pass(int shape, int prog, const FlowArgs& a, int dir, hipStream_t st) {
+hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir, hipStream_t st) {
   switch (shape) {
-    case 0: return launch_pass_c<CfgA>(prog, a, dir, st);
-    case 1: return launch_pass_c<CfgB>(prog, a, dir, st);
-    case 2: return launch_pass_c<CfgC>(prog, a, dir, st);
-    case 3: return launch_pass_c<CfgD>(prog, a, dir, st);
-    default: return launch_pass_c<CfgE>(prog, a, dir, st);
+    case 0: return launch_pass_s0(prog, minb, a, dir, st);
+    case 1: return launch_pass_s1(prog, minb, a, dir, st);
+    case 2: return launch_pass_s2(prog, minb, a, dir, st);
+    case 3: return launch_pass_s3(prog, minb, a, dir, st);
+    default: return launch_pass_s4(prog, minb, a, dir, st);
   }
 }
 
-int shape_tl(int shape) { return kShapes[shape].TL; }
+// register budget a pass uses until tuned: two workgroups per CU for the
+// gradient/convective passes, one for the others
+int default_minb(int prog) { return prog <= GC_LAST ? 2 : 1; }
 
 }  // namespace
 
@@ -771,8 +271,11 @@ int shape_tl(int shape) { return kShapes[shape].TL; }
 struct astr_hip_ctx {
   astr_hip_config cfg;
   FlowArgs a;
-  int shape[3];
-  int ntiles[3];
+  // launch selection per (pass program, direction): shape index and register
+  // budget MINB; ms = the autotuner's measured time per launch (0: untuned)
+  int shape[7][3];
+  int minb[7][3];
+  float tuned_ms[7][3];
   double* base = nullptr;   // one allocation for all fields
   double* coef = nullptr;   // LU tables
   double* stats_part = nullptr;
@@ -869,7 +372,7 @@ dim3 dense_grid(const FlowArgs& a, int hm) {
 int pass(astr_hip_ctx* c, int prog, int d) {
   const bool rec = c->prof && c->ev_used < (int)c->ev_kind.size();
   if (rec) HIP_OK(hipEventRecord(c->ev[2 * c->ev_used], c->stream));
-  HIP_OK(launch_pass(c->shape[d], prog, c->a, d, c->stream));
+  HIP_OK(launch_pass(c->shape[prog][d], prog, c->minb[prog][d], c->a, d, c->stream));
   if (rec) {
     HIP_OK(hipEventRecord(c->ev[2 * c->ev_used + 1], c->stream));
     c->ev_kind[c->ev_used++] = prog * 3 + d;
@@ -937,9 +440,13 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   filter_coefs(cfg->alfa_filter, cfg->beta_halo, a.fc, h33);
   c->ke_scale = 1.0 / ((double)cfg->im * cfg->jm * (cfg->km > 0 ? cfg->km : 1));
   for (int d = 0; d < a.ndir; d++) {
-    c->shape[d] = pick_shape(a.n[d]);
-    if (c->shape[d] < 0) { delete c; return fail("no launch shape fits this line length"); }
-    c->ntiles[d] = tiles_of(a.n, d, shape_tl(c->shape[d]));
+    const int sh = pick_shape(a.n[d]);
+    if (sh < 0) { delete c; return fail("no launch shape fits this line length"); }
+    for (int pr = 0; pr < 7; pr++) {
+      c->shape[pr][d] = sh;
+      c->minb[pr][d] = default_minb(pr);
+      c->tuned_ms[pr][d] = 0.f;
+    }
   }
   // fields: q 5, qsave 5, qrhs 5, dv 12, sg 9
   const long long nfield = 36;
@@ -972,7 +479,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
     a.dac[d] = c->coef + off[d];
     a.fac[d] = c->coef + off[3 + d];
   }
-  const int ntl = c->ntiles[a.ndir - 1];
+  // per-tile stats slots for the smallest tile any shape uses
+  const int ntl = tiles_of(a.n, a.ndir - 1, 4);
   if (hipMalloc(&c->stats_part, 2 * (size_t)ntl * sizeof(double)) != hipSuccess) {
     (void)hipFree(c->base); (void)hipFree(c->coef); delete c; return fail("hipMalloc of stats failed");
   }
@@ -1099,7 +607,7 @@ int astr_hip_rk3_steps(astr_hip_ctx* c, int nsteps, double* ke, double* ens) {
     c->stats_cap = nsteps;
   }
   FlowArgs& a = c->a;
-  const int ntl = c->ntiles[a.ndir - 1];
+  const int ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[GC_LAST][a.ndir - 1]].TL);
   for (int it = 0; it < nsteps; it++) {
     for (int s = 0; s < 3; s++) {
       a.stage = s;
@@ -1174,6 +682,97 @@ int astr_hip_profile_end(astr_hip_ctx* c, double* ms, int* count) {
   c->ev.clear();
   c->ev_kind.clear();
   c->ev_used = 0;
+  return 0;
+}
+
+// pass programs a case of ndir directions launches, per direction
+static bool pass_used(int prog, int d, int nd) {
+  switch (prog) {
+    case GC_FIRST: return d == 0;
+    case GC_MID: return d > 0 && d < nd - 1;
+    case GC_LAST: return d == nd - 1;
+    case V_MID: return d < nd - 1;
+    case V_LAST_RHS:
+    case V_LAST_RK: return d == nd - 1;
+    default: return d < nd;
+  }
+}
+
+int astr_hip_autotune(astr_hip_ctx* c, int reps) {
+  if (check_ctx(c)) return -1;
+  if (reps < 1 || reps > 1000) return fail("autotune reps must be 1..1000");
+  FlowArgs& a = c->a;
+  const FlowArgs keep = a;
+  // the passes overwrite q and qsave: keep a copy and put it back afterwards
+  const size_t qbytes = (size_t)10 * a.fs * sizeof(double);
+  double* snap = nullptr;
+  HIP_OK(hipMalloc(&snap, qbytes));
+  HIP_OK(hipMemcpyAsync(snap, a.q, qbytes, hipMemcpyDeviceToDevice, c->stream));
+  hipEvent_t e0, e1;
+  HIP_OK(hipEventCreate(&e0));
+  HIP_OK(hipEventCreate(&e1));
+  a.stage = 0;
+  a.rk_a = 1.0; a.rk_b = 0.0; a.rk_c = 1.0;
+  a.dostats = 0;
+  a.store_last_grad = 0;
+  int rc = 0;
+  for (int pr = 0; pr < 7 && !rc; pr++) {
+    for (int d = 0; d < a.ndir && !rc; d++) {
+      if (!pass_used(pr, d, a.ndir)) continue;
+      float best = 1e30f;
+      for (int sh = 0; sh < kNumShapes && !rc; sh++) {
+        if (!shape_fits(kShapes[sh], a.n[d])) continue;
+        for (int mb = 1; mb <= kMinbVariants && !rc; mb++) {
+          hipError_t e = launch_pass(sh, pr, mb, a, d, c->stream);
+          if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
+          for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_pass(sh, pr, mb, a, d, c->stream);
+          if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) {
+            rc = fail(std::string("autotune launch: ") + hipGetErrorString(e));
+            break;
+          }
+          ms /= (float)reps;
+          if (ms < best) {
+            best = ms;
+            c->shape[pr][d] = sh;
+            c->minb[pr][d] = mb;
+            c->tuned_ms[pr][d] = ms;
+          }
+        }
+      }
+    }
+  }
+  a = keep;
+  hipError_t e = hipMemcpyAsync(a.q, snap, qbytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  (void)hipFree(snap);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (!rc && e != hipSuccess) rc = fail(std::string("autotune restore: ") + hipGetErrorString(e));
+  return rc;
+}
+
+int astr_hip_get_pass_config(astr_hip_ctx* c, int prog, int dir, int* shape, int* minb, float* ms) {
+  if (check_ctx(c)) return -1;
+  if (prog < 0 || prog > 6 || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
+  if (shape) *shape = c->shape[prog][dir];
+  if (minb) *minb = c->minb[prog][dir];
+  if (ms) *ms = c->tuned_ms[prog][dir];
+  return 0;
+}
+
+int astr_hip_set_pass_config(astr_hip_ctx* c, int prog, int dir, int shape, int minb) {
+  if (check_ctx(c)) return -1;
+  if (prog < 0 || prog > 6 || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
+  if (shape < 0 || shape >= kNumShapes || !shape_fits(kShapes[shape], c->a.n[dir]))
+    return fail("launch shape does not fit this line length");
+  if (minb < 1 || minb > kMinbVariants) return fail("register budget must be 1 or 2");
+  c->shape[prog][dir] = shape;
+  c->minb[prog][dir] = minb;
+  c->tuned_ms[prog][dir] = 0.f;
   return 0;
 }
 

@@ -1,0 +1,512 @@
+// flow_dev.hpp -- device side of the per-stage line passes of flow.hip: the
+// case description FlowArgs, the pointwise thermodynamics and fluxes, and the
+// pass kernel k_pass templated on launch shape, pass program and register
+// budget.  passes.hip instantiates it once per launch shape.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "lines.hpp"
+
+namespace astr {
+
+// ---------------------------------------------------------------------------
+// device-side description of one case
+// ---------------------------------------------------------------------------
+struct FlowArgs {
+  int n[3];             // im, jm, km
+  int ndir;             // 3, or 2 when km==0
+  int pitch;            // doubles between rows (j)
+  long long plane;      // doubles between planes (k)
+  long long fs;         // doubles between fields
+  double inv_d[3];      // 1/dx, 1/dy, 1/dz
+  double const2, const5, const6;
+  double reynolds, prandtl, suth, suth1;
+  double rk_a, rk_b, rk_c, dt;
+  int trash;            // a point index inside every field's padding
+  double alfa, bhalo;   // filter alpha and halo-row beta (LHS of the filter rows)
+  int stage;            // 0,1,2
+  int dostats;          // stage-1 KE/enstrophy partial sums
+  int store_last_grad;  // last GC pass also stores its gradients
+  double* q;            // 5 fields
+  double* qsave;        // 5 fields
+  double* qrhs;         // 5 fields
+  double* dv;           // 12 fields: dvel(m,d) at m+3d, dtmp(d) at 9+d
+  double* sg;           // 9 fields: sigma 11,12,13,22,23,33, qflux 1,2,3
+  double* stats;        // 2 doubles per tile
+  const double* dac[3]; // derivative LU tables [ac1|ac2|ac3] per direction
+  const double* fac[3]; // filter LU tables
+  FilterCoef fc;
+};
+
+enum Prog { GC_FIRST = 0, GC_MID, GC_LAST, V_MID, V_LAST_RHS, V_LAST_RK, F_ONLY };
+
+// ntiles of direction dir for lines-per-tile TL
+__host__ __device__ inline int tiles_of(const int n[3], int dir, int TL) {
+  const int ni = (n[0] + TL) / TL;  // ceil((im+1)/TL)
+  if (dir == 0) return ((n[1] + 1) * (n[2] + 1) + TL - 1) / TL;
+  if (dir == 1) return ni * (n[2] + 1);
+  return ni * (n[1] + 1);
+}
+
+// Blocks are handed to the 8 XCDs round robin; give each XCD a contiguous
+// range of tiles so tiles sharing cache lines share an L2.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int x = b & 7, s = b >> 3, q = nb >> 3, r = nb & 7;
+  return x * q + (x < r ? x : r) + s;
+}
+
+template <class C>
+struct Slots {
+  int g[C::NPT];  // offset inside a field; the trash point for lines past the
+                  // end of the lattice (partial last tile)
+  int s[C::NPT];  // index of the point in a line buffer (l*LT + r)
+  int h[C::NPT];  // line-buffer index of the slot's periodic halo copy
+  int interior;   // bit t: i,j,k >= 1 (stats domain)
+};
+
+// LDS of a pass: the output lines X (TL x LT), then the input lines T
+// (TL x LT), then a write-only dummy cell.  A slot holding no point (past the
+// tile) or no halo copy points s/h at index TL*LT: its T writes land in the
+// dummy cell (T + kLineOff + TL*LT) and its X reads return T's zero padding
+// (X + TL*LT = row -8 of T's first line), so no slot needs a guard.
+template <class C>
+constexpr int lds_doubles() {
+  return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
+}
+
+// Map the tile's points onto slots.  dir 0: lines are (j,k), rows i; dir 1:
+// lines (i,k) rows j; dir 2: lines (i,j) rows k.
+template <class C>
+__device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int dir, int tile) {
+  const int n = a.n[dir];
+  const int np = C::TL * (n + 1);
+  const int ni = (a.n[0] + C::TL) / C::TL;
+  const int dummy = C::TL * C::LT;
+  S.interior = 0;
+#pragma unroll
+  for (int t = 0; t < C::NPT; t++) {
+    const int e = threadIdx.x + t * C::NT;
+    S.g[t] = a.trash;
+    S.s[t] = dummy;
+    S.h[t] = dummy;
+    if (e < np) {
+      int l, r, i, j, k;
+      bool ok;
+      if (dir == 0) {
+        l = e / (n + 1);
+        r = e - l * (n + 1);
+        const int L = tile * C::TL + l;
+        j = L % (a.n[1] + 1);
+        k = L / (a.n[1] + 1);
+        i = r;
+        ok = L < (a.n[1] + 1) * (a.n[2] + 1);
+      } else {
+        l = e & (C::TL - 1);
+        r = e / C::TL;
+        const int it = tile % ni, o = tile / ni;
+        i = it * C::TL + l;
+        ok = i <= a.n[0];
+        if (dir == 1) { j = r; k = o; } else { j = o; k = r; }
+      }
+      S.s[t] = l * C::LT + r;
+      // rows n-5..n-1 are copied to the halo rows -5..-1, rows 1..5 to
+      // n+1..n+5 (disjoint for n >= 11)
+      if (r >= n - 5 && r <= n - 1) S.h[t] = S.s[t] - n;
+      if (r >= 1 && r <= 5) S.h[t] = S.s[t] + n;
+      if (ok) {
+        S.g[t] = (int)((long long)k * a.plane + (long long)j * a.pitch + i);
+        const bool in = i >= 1 && j >= 1 && (a.ndir == 2 || k >= 1);
+        if (in) S.interior |= 1 << t;
+      }
+    }
+  }
+}
+
+// store slot t's value into its input line with the periodic halo copies
+// q(-h)=q(n-h), q(n+h)=q(h) (tgvsolver.F90:371-440)
+template <class C>
+__device__ __forceinline__ void put(double* T, const Slots<C>& S, int t, double v) {
+  T[kLineOff + S.s[t]] = v;
+  T[kLineOff + S.h[t]] = v;
+}
+
+// zero the padding rows -8..-6 and n+6..P*R+2 of every input line (never
+// written by put; read by the chunks at the ends of a line)
+template <class C>
+__device__ __forceinline__ void zero_pad(double* T, int n) {
+  const int npad = 3 + (C::P * C::R + 2 - (n + 6) + 1);
+  for (int e = threadIdx.x; e < C::TL * npad; e += C::NT) {
+    const int l = e / npad, k = e - l * npad;
+    const int row = k < 3 ? k - 8 : n + 6 + (k - 3);
+    T[l * C::LT + kLineOff + row] = 0.0;
+  }
+}
+
+// q2fvar + thermal_scar (tgvsolver.F90:73-121).  Pointwise thermodynamics is
+// evaluated without FMA contraction and in the reference's operation order:
+// p ~ 1/(gamma Ma^2) is large against its gradient, so one rounding step in p
+// becomes a visible error in grad p.
+__device__ __forceinline__ double pressure_of(const FlowArgs& a, double q1, double q2, double q3,
+                                              double q4, double q5) {
+#pragma clang fp contract(off)
+  const double u = q2 / q1, v = q3 / q1, w = q4 / q1;
+  return (q5 - 0.5 * q1 * (u * u + v * v + w * w)) / a.const6;
+}
+
+struct Prim {
+  double r, u, v, w, p, t;
+};
+
+__device__ __forceinline__ Prim prim_of(const FlowArgs& a, double q1, double q2, double q3,
+                                        double q4, double q5) {
+#pragma clang fp contract(off)
+  Prim P;
+  P.r = q1;
+  P.u = q2 / q1;
+  P.v = q3 / q1;
+  P.w = q4 / q1;
+  P.p = (q5 - 0.5 * q1 * (P.u * P.u + P.v * P.v + P.w * P.w)) / a.const6;
+  P.t = P.p / q1 * a.const2;
+  return P;
+}
+
+// convective flux component m along dir (tgvsolver.F90:559-564, 586-591,
+// 613-618) from q_m, the velocity along dir and p
+__device__ __forceinline__ double conv_flux(int m, int dir, double qm, double q5, double uu,
+                                            double p) {
+#pragma clang fp contract(off)
+  if (m == 4) return (q5 + p) * uu;
+  double f = qm * uu;
+  if (m == dir + 1) f = f + p;
+  return f;
+}
+
+// Sutherland (tgvsolver.F90:123-140)
+__device__ __forceinline__ double miucal(const FlowArgs& a, double t) {
+  return t * sqrt(t) * a.suth1 / (t + a.suth);
+}
+
+// q <- a*qsave + b*q + c*qrhs*dt in the reference's order (tgvsolver.F90:1055-1057)
+__device__ __forceinline__ double rk_update(const FlowArgs& a, double qs, double qo, double qr) {
+#pragma clang fp contract(off)
+  return a.rk_a * qs + a.rk_b * qo + a.rk_c * qr * a.dt;
+}
+
+// stress, heat flux (tgvsolver.F90:663-691); g[m][d] = d u_m / d x_d
+__device__ __forceinline__ void visc_flux(const FlowArgs& a, const double g[3][3],
+                                          const double gt[3], const Prim& P, double out[9]) {
+#pragma clang fp contract(off)
+  const double s11 = g[0][0];
+  const double s12 = 0.5 * (g[0][1] + g[1][0]);
+  const double s13 = 0.5 * (g[0][2] + g[2][0]);
+  const double s22 = g[1][1];
+  const double s23 = 0.5 * (g[1][2] + g[2][1]);
+  const double s33 = g[2][2];
+  const double skk = (1.0 / 3.0) * (s11 + s22 + s33);
+  const double miu = miucal(a, P.t) / a.reynolds;
+  const double miu2 = 2.0 * miu;
+  const double hcc = (miu / a.prandtl) / a.const5;
+  out[0] = miu2 * (s11 - skk);
+  out[1] = miu2 * s12;
+  out[2] = miu2 * s13;
+  out[3] = miu2 * (s22 - skk);
+  out[4] = miu2 * s23;
+  out[5] = miu2 * (s33 - skk);
+  out[6] = hcc * gt[0] + out[0] * P.u + out[1] * P.v + out[2] * P.w;
+  out[7] = hcc * gt[1] + out[1] * P.u + out[3] * P.v + out[4] * P.w;
+  out[8] = hcc * gt[2] + out[2] * P.u + out[4] * P.v + out[5] * P.w;
+}
+
+// ---------------------------------------------------------------------------
+// the pass kernel
+// ---------------------------------------------------------------------------
+// MINB: workgroups per CU the register allocation is held to (each
+// workgroup puts one wave on each SIMD, so MINB waves per SIMD)
+template <class C, int PROG, int MINB>
+__global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
+  extern __shared__ double smem[];
+  constexpr int NPT = C::NPT;
+  const int n = a.n[dir];
+  double* X = smem;                 // output lines, TL x LT (rows at offset 0)
+  double* T = smem + C::TL * C::LT; // input lines, TL x LT
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  Slots<C> S;
+  make_slots<C>(S, a, dir, tile);
+  zero_pad<C>(T, n);
+  const int ln = threadIdx.x / C::P, p = threadIdx.x % C::P;
+  double* Tl = T + ln * C::LT;
+  double* Xl = X + ln * C::LT;
+  const double idn = a.inv_d[dir];
+  const long long fs = a.fs;
+
+  if constexpr (PROG == GC_FIRST || PROG == GC_MID || PROG == GC_LAST) {
+    constexpr bool LAST = PROG == GC_LAST;
+    ChunkCoef<C::R, true> CD;
+    load_chunk_coef<C::R, true>(CD, a.dac[dir], n + 3, p * C::R);
+    double qv[5][NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++)
+#pragma unroll
+      for (int m = 0; m < 5; m++) qv[m][t] = a.q[m * fs + S.g[t]];
+
+    // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
+    double gz[4][NPT];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        {
+          double f = 0.0;
+          {
+            if (c < 3) {
+              f = qv[c + 1][t] / qv[0][t];
+            } else {
+              const double pr = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
+              f = pr / qv[0][t] * a.const2;
+            }
+          }
+          put<C>(T, S, t, f);
+        }
+      }
+      __syncthreads();
+      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      __syncthreads();
+      const int fld = c < 3 ? c + 3 * dir : 9 + dir;
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        {
+          const double v = X[S.s[t]] * idn;
+          if (!LAST || a.store_last_grad) a.dv[fld * fs + S.g[t]] = v;
+          gz[c][t] = v;
+        }
+      }
+    }
+
+    if constexpr (LAST) {
+      // stress and heat flux (diffusion, tgvsolver.F90:659-694) and the
+      // stage-1 KE / enstrophy partial sums (stacal, :893-942)
+      double sk = 0.0, se = 0.0;
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        const int o = S.g[t];
+        double g[3][3], gt[3];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+          if (d == dir) {
+            g[0][d] = gz[0][t]; g[1][d] = gz[1][t]; g[2][d] = gz[2][t]; gt[d] = gz[3][t];
+          } else if (d < a.ndir) {
+            g[0][d] = a.dv[(0 + 3 * d) * fs + o];
+            g[1][d] = a.dv[(1 + 3 * d) * fs + o];
+            g[2][d] = a.dv[(2 + 3 * d) * fs + o];
+            gt[d] = a.dv[(9 + d) * fs + o];
+          } else {
+            g[0][d] = 0.0; g[1][d] = 0.0; g[2][d] = 0.0; gt[d] = 0.0;
+          }
+        }
+        const Prim P = prim_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
+        double f[9];
+        visc_flux(a, g, gt, P, f);
+#pragma unroll
+        for (int m = 0; m < 9; m++) a.sg[m * fs + o] = f[m];
+        if (a.dostats && ((S.interior >> t) & 1)) {
+          sk += P.r * (P.u * P.u + P.v * P.v + P.w * P.w);
+          const double o1 = g[2][1] - g[1][2];
+          const double o2 = g[0][2] - g[2][0];
+          const double o3 = g[1][0] - g[0][1];
+          se += P.r * (o1 * o1 + o2 * o2 + o3 * o3);
+        }
+      }
+      if (a.dostats) {
+        // deterministic block reduction into this tile's slot
+        __syncthreads();
+        X[threadIdx.x] = sk;
+        X[C::NT + threadIdx.x] = se;
+        __syncthreads();
+        for (int h = C::NT / 2; h > 0; h >>= 1) {
+          if ((int)threadIdx.x < h) {
+            X[threadIdx.x] += X[threadIdx.x + h];
+            X[C::NT + threadIdx.x] += X[C::NT + threadIdx.x + h];
+          }
+          __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+          a.stats[2 * tile] = X[0];
+          a.stats[2 * tile + 1] = X[C::NT];
+        }
+        __syncthreads();
+      }
+    }
+
+    // convective flux derivatives along dir (convection, tgvsolver.F90:539-633)
+    double ud[NPT], pp[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++) {
+      pp[t] = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
+      ud[t] = (dir == 0 ? qv[1][t] : (dir == 1 ? qv[2][t] : qv[3][t])) / qv[0][t];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+#pragma unroll
+      for (int t = 0; t < NPT; t++)
+        put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
+      double* dst = a.qrhs + m * fs;
+      double qr[NPT];
+      if (PROG != GC_FIRST) {
+        // issued before the solve so the load latency overlaps it
+#pragma unroll
+        for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
+      }
+      __syncthreads();
+      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        {
+          const double v = X[S.s[t]] * idn;
+          if (PROG == GC_FIRST) dst[S.g[t]] = v;
+          else if (PROG == GC_MID) dst[S.g[t]] = qr[t] + v;
+          else dst[S.g[t]] = -(qr[t] + v);
+        }
+      }
+    }
+  } else if constexpr (PROG == V_MID || PROG == V_LAST_RHS || PROG == V_LAST_RK) {
+    // viscous-flux derivatives (diffusion, tgvsolver.F90:700-787); columns
+    // per direction: x (s11,s12,s13,q1), y (s12,s22,s23,q2), z (s13,s23,s33,q3)
+    ChunkCoef<C::R, true> CD;
+    load_chunk_coef<C::R, true>(CD, a.dac[dir], n + 3, p * C::R);
+    const int c0 = dir == 0 ? 0 : (dir == 1 ? 1 : 2);
+    const int c1 = dir == 0 ? 1 : (dir == 1 ? 3 : 4);
+    const int c2 = dir == 0 ? 2 : (dir == 1 ? 4 : 5);
+    double acc[4][NPT];
+    double nx[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++) nx[t] = a.sg[c0 * fs + S.g[t]];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+#pragma unroll
+      for (int t = 0; t < NPT; t++)
+        put<C>(T, S, t, nx[t]);
+      // loads for this column's accumulation and the next column's input are
+      // issued before the solve so their latency overlaps it
+      double qr[NPT];
+      double* dst = a.qrhs + (1 + c) * fs;
+#pragma unroll
+      for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
+      if (c < 3) {
+        const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
+        const double* src = a.sg + col * fs;
+#pragma unroll
+        for (int t = 0; t < NPT; t++) nx[t] = src[S.g[t]];
+      }
+      __syncthreads();
+      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        {
+          const double v = qr[t] + X[S.s[t]] * idn;
+          if (PROG == V_LAST_RK) acc[c][t] = v;
+          else dst[S.g[t]] = v;
+        }
+      }
+    }
+    if constexpr (PROG == V_LAST_RK) {
+      // RK3 stage update (tgvsolver.F90:1043-1059), then the filter along dir
+      double qn[5][NPT];
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+#pragma unroll
+        for (int m = 0; m < 5; m++) qn[m][t] = 0.0;
+        const int o = S.g[t];
+#pragma unroll
+        for (int m = 0; m < 5; m++) {
+          const double qr = m == 0 ? a.qrhs[o] : acc[m - 1][t];
+          const double qo = a.q[m * fs + o];
+          double qs;
+          if (a.stage == 0) {
+            a.qsave[m * fs + o] = qo;
+            qs = qo;
+          } else {
+            qs = a.qsave[m * fs + o];
+          }
+          qn[m][t] = rk_update(a, qs, qo, qr);
+        }
+      }
+      ChunkCoef<C::R, false> CF;
+      load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+#pragma unroll
+        for (int t = 0; t < NPT; t++)
+          put<C>(T, S, t, qn[m][t]);
+        __syncthreads();
+        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
+        __syncthreads();
+        double* dst = a.q + m * fs;
+#pragma unroll
+        for (int t = 0; t < NPT; t++)
+          dst[S.g[t]] = X[S.s[t]];
+      }
+    }
+  } else {  // F_ONLY: filter of q along dir, in place (filterq, tgvsolver.F90:791-883)
+    ChunkCoef<C::R, false> CF;
+    load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
+    double nx[NPT];
+#pragma unroll
+    for (int t = 0; t < NPT; t++) nx[t] = a.q[S.g[t]];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+#pragma unroll
+      for (int t = 0; t < NPT; t++)
+        put<C>(T, S, t, nx[t]);
+      if (m < 4) {
+#pragma unroll
+        for (int t = 0; t < NPT; t++) nx[t] = a.q[(m + 1) * fs + S.g[t]];
+      }
+      __syncthreads();
+      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
+      __syncthreads();
+      double* dst = a.q + m * fs;
+#pragma unroll
+      for (int t = 0; t < NPT; t++)
+        dst[S.g[t]] = X[S.s[t]];
+    }
+  }
+}
+
+template <class C, int MINB>
+hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
+  const int nt = tiles_of(a.n, dir, C::TL);
+  const size_t lds = (size_t)lds_doubles<C>() * sizeof(double);
+  dim3 g(nt), b(C::NT);
+  switch (prog) {
+    case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST, MINB>), g, b, lds, st, a, dir); break;
+    case GC_MID: hipLaunchKernelGGL((k_pass<C, GC_MID, MINB>), g, b, lds, st, a, dir); break;
+    case GC_LAST: hipLaunchKernelGGL((k_pass<C, GC_LAST, MINB>), g, b, lds, st, a, dir); break;
+    case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID, MINB>), g, b, lds, st, a, dir); break;
+    case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS, MINB>), g, b, lds, st, a, dir); break;
+    case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB>), g, b, lds, st, a, dir); break;
+    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB>), g, b, lds, st, a, dir); break;
+  }
+  return hipGetLastError();
+}
+
+// launch shapes (threads, lines per tile, chunk length, point slots per thread)
+using CfgA = LineCfg<256, 4, 5, 5>;    // P=64, n <= 313
+using CfgB = LineCfg<256, 4, 9, 9>;    // P=64, n <= 569
+using CfgC = LineCfg<256, 16, 5, 5>;   // P=16, n <= 73
+using CfgD = LineCfg<256, 8, 9, 9>;    // P=32, n <= 281
+using CfgE = LineCfg<256, 8, 5, 5>;    // P=32, n <= 153
+constexpr int kNumShapes = 5;
+constexpr int kMinbVariants = 2;       // MINB = 1, 2
+
+// one per shape, each in its own translation unit (passes.hip)
+hipError_t launch_pass_s0(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
+hipError_t launch_pass_s1(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
+hipError_t launch_pass_s2(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
+hipError_t launch_pass_s3(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
+hipError_t launch_pass_s4(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
+
+}  // namespace astr

@@ -88,5 +88,10 @@ class Event:
         return float(t.value)
 
 
+def copy_device(dst, src, nbytes):
+    """Device-to-device copy (the runtime's own copy kernel)."""
+    _ok(rt().hipMemcpy(dst.ptr, src.ptr, int(nbytes), 3), "hipMemcpy D2D")
+
+
 def synchronize():
     _ok(rt().hipDeviceSynchronize(), "hipDeviceSynchronize")

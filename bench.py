@@ -116,6 +116,25 @@ def chem_batch(ncells=1 << 20, reps=20):
             "mechanism": "Jachimowski88_H2-9-19 (9 species, 19 reactions)"}
 
 
+def copy_probe(nbytes, reps=3):
+    """A plain device-to-device copy of nbytes, reps times: the calibration
+    kernel for the FETCH_SIZE / WRITE_SIZE counters (it reads and writes
+    exactly nbytes per launch)."""
+    from astr_amd import hip
+
+    a = hip.DeviceArray((nbytes // 8,))
+    b = hip.DeviceArray((nbytes // 8,))
+    hip.synchronize()
+    e0, e1 = hip.Event(), hip.Event()
+    e0.record()
+    for _ in range(reps):
+        hip.copy_device(b, a, nbytes)
+    e1.record()
+    ms = e0.elapsed_ms(e1) / reps
+    return {"bytes": nbytes, "launches": reps, "ms_per_copy": round(ms, 4),
+            "GBs_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+
+
 def load_traffic(n):
     path = os.path.join(ROOT, "profiles", f"traffic_tgv{n}.json")
     if os.path.exists(path):
@@ -132,6 +151,10 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch selection instead of timing the candidates")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="also time a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the whole {args.n}^3 domain sits in one MI355X's HBM; "
@@ -144,6 +167,13 @@ def main():
     dt = 5e-4 * 256.0 / n
     f = A.Flow(A.tgv_config(n, deltat=dt))
     f.init_tgv(1e-3, 2024)
+    tune_s = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        f.autotune(reps=3)
+        tune_s = time.perf_counter() - t0
+    launch = {f"{k[0]}_{k[1]}": {"shape": A.SHAPES[v[0]], "minb": v[1]}
+              for k, v in f.pass_configs().items()}
     if args.warmup:
         f.rk3(args.warmup, stats=True)
     f.sync()
@@ -202,9 +232,14 @@ def main():
         "physical_point_rate": round(n ** 3 * 3 * args.steps / wall / 1e6, 3),
         "kernels_ms_per_launch": {f"{k[0]}_{k[1]}": round(v[0] / v[1], 4) for k, v in prof.items()},
         "ke_last": float(ke[-1]),
+        "launch": {"autotune_s": None if tune_s is None else round(tune_s, 2),
+                   "shape_is": "(lines per workgroup, chunks per line, unknowns per chunk)",
+                   "passes": launch},
     }
     if not args.no_chem:
         out["chem_batch"] = chem_batch()
+    if args.copy_probe:
+        out["copy_probe"] = copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

@@ -100,6 +100,20 @@ int astr_hip_sync(astr_hip_ctx *ctx);
 int astr_hip_profile_begin(astr_hip_ctx *ctx, int max_launches);
 int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
 
+/* Launch selection of the line passes (no reference counterpart: the
+ * reference's loops have no launch parameters).  A pass is identified by its
+ * program (0..6 as in the profile slots) and direction; its selection is a
+ * launch shape and a register budget minb (1 or 2 workgroups per CU).  The
+ * shapes (256 threads each) are 0: 4 lines x 64 chunks of 5 unknowns,
+ * 1: 4 x 64 x 9, 2: 16 x 16 x 5, 3: 8 x 32 x 9, 4: 8 x 32 x 5; a shape fits a
+ * line of n intervals when its chunks hold n+7 unknowns.  astr_hip_autotune times
+ * every fitting selection of every pass the case launches (reps launches
+ * each, on the context's stream, state restored afterwards) and keeps the
+ * fastest; get/set read and override single selections. */
+int astr_hip_autotune(astr_hip_ctx *ctx, int reps);
+int astr_hip_get_pass_config(astr_hip_ctx *ctx, int prog, int dir, int *shape, int *minb, float *ms);
+int astr_hip_set_pass_config(astr_hip_ctx *ctx, int prog, int dir, int shape, int minb);
+
 /* fds%central / compact_filter on a batch of lines: derivative.F90:171-198
  * (op 0, df_compact, scheme 643c) and filter.F90:112-144 (op 1), ntype=3.
  * f holds nlines lines of f(-5:n+5) (n+11 values each), out nlines x (n+1). */

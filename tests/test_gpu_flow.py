@@ -168,3 +168,43 @@ def test_hist128_noise_1000_steps():
     """TGV 128^3 with seeded 1e-3 Gaussian velocity noise, dt=1e-3, 1000 RK3
     steps against the reference Fortran run on the identical seeded field."""
     _hist_check("hist128_noise.npz", 128, 1e-3, 1e-3, 2024)
+
+
+@pytest.mark.parametrize("dims", [(64, 64, 64), (40, 36, 28)])
+def test_every_launch_selection(dims):
+    """Every launch shape and register budget, forced on all passes, gives
+    the oracle's RHS and RK3 steps."""
+    f0, s = make_pair(*dims)
+    q0 = s.q.copy()
+    s.rhscal()
+    rhs_ref = s.qrhs.copy()
+    s.rk3(2)
+    q_ref = s.q.copy()
+    tried = 0
+    for shape, (tl, p, r) in A.SHAPES.items():
+        if p * r < max(dims) + 7 or 256 * r < tl * (max(dims) + 1):
+            continue
+        for minb in (1, 2):
+            f = A.Flow(A.tgv_config(*dims, deltat=1e-3))
+            for prog in range(7):
+                for d in range(3):
+                    f.set_pass_config(prog, d, shape, minb)
+            f.upload(q0)
+            got = f.rhscal()
+            assert max(per_var(got, rhs_ref)) < RHS_TOL, (shape, minb)
+            f.upload(q0)
+            f.rk3(2)
+            assert max(state_err(f.download(), q_ref)) < RHS_TOL, (shape, minb)
+            tried += 1
+    assert tried >= 4
+
+
+def test_autotune_restores_state():
+    f, s = make_pair(48, 48, 48)
+    before = f.download()
+    cfg = f.autotune(reps=1)
+    assert np.array_equal(f.download(), before)
+    assert all(ms > 0 for (_, _, ms) in cfg.values())
+    f.rk3(3)
+    s.rk3(3)
+    assert max(state_err(f.download(), s.q)) < RHS_TOL
