@@ -48,7 +48,9 @@ EXPORTS = [
     "astr_hip_download", "astr_hip_filterq", "astr_hip_gradcal", "astr_hip_rhscal",
     "astr_hip_rk3_steps", "astr_hip_sync", "astr_hip_line_batch",
     "astr_hip_profile_begin", "astr_hip_profile_end", "astr_hip_autotune",
-    "astr_hip_get_pass_config", "astr_hip_set_pass_config",
+    "astr_hip_get_pass_config", "astr_hip_set_pass_config", "astr_hip_init_vortex",
+    "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
+    "astr_hip_batch_rk3_steps", "astr_hip_batch_info",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5)}
 PROF_SLOTS = 21
@@ -87,6 +89,12 @@ def lib():
         L.astr_hip_get_pass_config.argtypes = [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_int),
                                                C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.astr_hip_set_pass_config.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.astr_hip_init_vortex.argtypes = [C.c_void_p] + [C.c_double] * 5
+        L.astr_hip_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
+        L.astr_hip_batch_destroy.argtypes = [C.c_void_p]
+        L.astr_hip_batch_autotune.argtypes = [C.c_void_p, C.c_int]
+        L.astr_hip_batch_rk3_steps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
+        L.astr_hip_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         _lib = L
     return _lib
 
@@ -135,6 +143,12 @@ class Flow:
     # -- state ---------------------------------------------------------
     def init_tgv(self, noise=0.0, seed=0):
         _chk(lib().astr_hip_init_tgv(self.h, noise, seed))
+
+    def init_vortex(self, beta=5.0, xc=None, yc=None, u0=1.0, v0=0.0):
+        """Two-dimensional isentropic vortex (see include/astr_hip.h)."""
+        xc = self.cfg.lx / 2 if xc is None else xc
+        yc = self.cfg.ly / 2 if yc is None else yc
+        _chk(lib().astr_hip_init_vortex(self.h, beta, xc, yc, u0, v0))
 
     def halo_shape(self, hm):
         return tuple(s + 2 * hm for s in self.shape)
@@ -220,6 +234,60 @@ class Flow:
 
     def set_pass_config(self, prog, d, shape, minb):
         _chk(lib().astr_hip_set_pass_config(self.h, prog, d, shape, minb))
+
+
+def vortex_config(im, jm=None, mach=0.5, reynolds=1.0e5, length=10.0, cfl=0.67, deltat=None):
+    """Two-dimensional periodic isentropic-vortex case on [0,length]^2 (km = 0).
+    deltat defaults to the summed-CFL limit cfl for the free stream (1,0) plus
+    the vortex's peak swirl (0.8): dt = cfl / sum_d (|u_d| + c)/dx_d."""
+    jm = im if jm is None else jm
+    c = 1.0 / mach
+    if deltat is None:
+        deltat = cfl / ((1.0 + 0.8 + c) / (length / im) + (0.8 + c) / (length / jm))
+    return Config(im=im, jm=jm, km=0, semantics=0, alfa_filter=0.49, beta_halo=1.11,
+                  beta_bound=1.09, gamma=1.4, mach=mach, reynolds=reynolds, prandtl=0.72,
+                  ref_tem=273.15, sutherland=110.4, lx=length, ly=length, lz=length,
+                  deltat=deltat)
+
+
+class Batch:
+    """Many independent cases advanced together (persistent batched passes,
+    optionally one hipGraph per RK3 step).  The Flow objects stay owned by the
+    caller and must outlive the batch."""
+
+    def __init__(self, flows):
+        self.flows = list(flows)
+        arr = (C.c_void_p * len(self.flows))(*[f.h.value for f in self.flows])
+        h = C.c_void_p()
+        _chk(lib().astr_hip_batch_create(arr, len(self.flows), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().astr_hip_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def autotune(self, reps=3):
+        _chk(lib().astr_hip_batch_autotune(self.h, reps))
+
+    def rk3(self, nsteps, graph=True, stats=True):
+        """nsteps RK3 steps of every case; (ke, ens) of shape (ncases, nsteps)."""
+        n = len(self.flows)
+        ke = np.empty((n, nsteps)) if stats else None
+        en = np.empty((n, nsteps)) if stats else None
+        _chk(lib().astr_hip_batch_rk3_steps(self.h, nsteps, 1 if graph else 0, _p(ke), _p(en)))
+        return (ke, en) if stats else None
+
+    def info(self):
+        g, l = C.c_int(), C.c_int()
+        _chk(lib().astr_hip_batch_info(self.h, C.byref(g), C.byref(l)))
+        return {"groups": g.value, "launches_per_step": l.value}
 
 
 def line_batch(op, f, alfa_filter=0.49, beta_halo=1.11):

@@ -35,6 +35,17 @@ def _bind():
     L.astr_hip_chem_rates_device.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p,
                                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.c_void_p]
+    L.astr_hip_reactor_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
+                                          C.POINTER(C.c_void_p)]
+    L.astr_hip_reactor_destroy.argtypes = [C.c_void_p]
+    L.astr_hip_reactor_set.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, _dp,
+                                       C.c_double, C.c_double]
+    L.astr_hip_reactor_advance.argtypes = [C.c_void_p, C.c_int]
+    L.astr_hip_reactor_sync.argtypes = [C.c_void_p]
+    L.astr_hip_reactor_status.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.astr_hip_reactor_get.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int)]
+    L.astr_hip_reactor_release.argtypes = [C.c_void_p, C.c_int]
     L._chem_bound = True
     return L
 
@@ -134,3 +145,66 @@ def sample_cells(names, W, n, seed):
     phi = rng.uniform(0.3, 3.0, n)
     c = rng.uniform(0.0, 1.0, n)
     return T, p, mixture(names, W, phi, c)
+
+
+class ReactorPool:
+    """Slots of 0-D constant-pressure adiabatic reactors integrated on the GPU
+    (astr_hip_reactor_*, include/astr_hip.h).  Status per slot: 0 free,
+    1 running, 2 done, 3 failed."""
+
+    FREE, RUNNING, DONE, FAILED = 0, 1, 2, 3
+
+    def __init__(self, mech: Mechanism, capacity, rtol=1e-6, atol=1e-12):
+        self.mech = mech
+        self.capacity = capacity
+        h = C.c_void_p()
+        _chk(_bind().astr_hip_reactor_create(mech.h, capacity, rtol, atol, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            _bind().astr_hip_reactor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set(self, slot, T0, p, Y0, t_end, h0=1e-10):
+        Y0 = np.ascontiguousarray(Y0, dtype=np.float64)
+        if Y0.shape != (self.mech.ns,):
+            raise AstrError("Y0 must hold one mass fraction per species")
+        _chk(_bind().astr_hip_reactor_set(self.h, slot, T0, p, _p(Y0), t_end, h0))
+
+    def advance(self, max_tries=200):
+        _chk(_bind().astr_hip_reactor_advance(self.h, max_tries))
+
+    def sync(self):
+        _chk(_bind().astr_hip_reactor_sync(self.h))
+
+    def status(self):
+        st = np.zeros(self.capacity, dtype=np.int32)
+        _chk(_bind().astr_hip_reactor_status(self.h, st.ctypes.data_as(C.POINTER(C.c_int))))
+        return st
+
+    def get(self, slot):
+        """(t, T, Y, accepted steps, rejected steps) of a slot."""
+        t = C.c_double()
+        yv = np.zeros(self.mech.ns + 1)
+        ns_, nr_ = C.c_int(), C.c_int()
+        _chk(_bind().astr_hip_reactor_get(self.h, slot, C.byref(t), _p(yv), C.byref(ns_),
+                                          C.byref(nr_)))
+        return t.value, yv[0], yv[1:], ns_.value, nr_.value
+
+    def release(self, slot):
+        _chk(_bind().astr_hip_reactor_release(self.h, slot))
+
+    def run(self, max_tries=200):
+        """Advance until no slot is running; returns the final status array."""
+        while True:
+            self.advance(max_tries)
+            st = self.status()
+            if not np.any(st == self.RUNNING):
+                return st

@@ -71,6 +71,58 @@ __device__ __forceinline__ double conc_pow(double c, double nu) {
   return nu == 1.0 ? c : (nu == 2.0 ? c * c : pow(c, nu));
 }
 
+// Thermodynamics and kinetics of one state: molar concentrations from rho and
+// Y; per species cp/R (cpr), h/RT (hrt); net molar production rates w
+// (kmol/m^3/s).  GasKinetics: kf = A T^b exp(-E/RT), kr = kf / Kc with
+// Kc = exp(-dG/RT) (p0/RT)^dnu, three-body [M] = sum eff_k C_k.
+__device__ __forceinline__ void kinetics(const Mech* __restrict__ M, double T, double rho,
+                                         const double* y, double* w, double* cpr, double* hrt) {
+  const int ns = M->ns, nr = M->nr;
+  double c[NSMAX], g[NSMAX];
+  const double logT = log(T), invT = 1.0 / T;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  for (int k = 0; k < ns; k++) {
+    c[k] = rho * y[k] / M->W[k];
+    const double* a = T <= M->tmid[k] ? M->lo[k] : M->hi[k];
+    // NASA 7-coefficient polynomials
+    cpr[k] = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
+    hrt[k] = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 + a[4] * T4 / 5.0 +
+             a[5] * invT;
+    const double sr = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 +
+                      a[4] * T4 / 4.0 + a[6];
+    g[k] = hrt[k] - sr;
+    w[k] = 0.0;
+  }
+  const double lc0 = log(M->p0 / (kRgas * T));  // ln of the standard concentration
+  for (int r = 0; r < nr; r++) {
+    const double kf = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT);
+    double fwd = kf;
+    double dg = 0.0;
+    for (int j = 0; j < M->nre[r]; j++) {
+      fwd *= conc_pow(c[M->rsp[r][j]], M->rnu[r][j]);
+      dg -= M->rnu[r][j] * g[M->rsp[r][j]];
+    }
+    double rv = 0.0;
+    if (M->rev[r]) {
+      double pp = 1.0;
+      for (int j = 0; j < M->npr[r]; j++) {
+        pp *= conc_pow(c[M->psp[r][j]], M->pnu[r][j]);
+        dg += M->pnu[r][j] * g[M->psp[r][j]];
+      }
+      const double kr = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT + dg - M->dnu[r] * lc0);
+      rv = kr * pp;
+    }
+    double q = fwd - rv;
+    if (M->tb[r]) {
+      double cm = 0.0;
+      for (int k = 0; k < ns; k++) cm += M->eff[r][k] * c[k];
+      q *= cm;
+    }
+    for (int j = 0; j < M->nre[r]; j++) w[M->rsp[r][j]] -= M->rnu[r][j] * q;
+    for (int j = 0; j < M->npr[r]; j++) w[M->psp[r][j]] += M->pnu[r][j] * q;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
                                               const double* __restrict__ Tin,
                                               const double* __restrict__ RPin, int input_is_p,
@@ -139,6 +191,165 @@ __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long l
   for (int k = 0; k < ns; k++) wdot[k * n + i] = w[k] * M->W[k];
   if (hmix) hmix[i] = h;
   if (cpmix) cpmix[i] = cp * kRgas;
+}
+
+// ---------------------------------------------------------------------------
+// 0-D constant-pressure adiabatic reactor (no reference counterpart: the
+// reference's 0dreactor is constant-volume).  State y = (T, Y_1..Y_ns):
+//   dY_k/dt = wdot_k W_k / rho,  dT/dt = -sum_k h_k dY_k/dt / cp,
+// rho = p / (R T sum Y_k/W_k), h_k = (h/RT)_k R T / W_k, cp = R sum Y_k cpr_k/W_k,
+// so the mixture enthalpy is conserved.  Integrated by ROS2, the two-stage
+// L-stable Rosenbrock method of Verwer, Spee, Blom and Hundsdorfer (1999):
+//   (I - g h J) k1 = f(y),  (I - g h J) k2 = f(y + h k1) - 2 k1,  g = 1 + 1/sqrt(2)
+//   y+ = y + 3/2 h k1 + 1/2 h k2,  error estimate h/2 (k1 + k2)
+// with a forward-difference Jacobian and step-size control.
+// ---------------------------------------------------------------------------
+constexpr int NYMAX = NSMAX + 1;
+
+__device__ void reactor_rhs(const Mech* __restrict__ M, double p, const double* yv, double* f) {
+  const int ns = M->ns;
+  const double T = yv[0];
+  const double* Y = yv + 1;
+  double w[NSMAX], cpr[NSMAX], hrt[NSMAX];
+  double winv = 0.0;
+  for (int k = 0; k < ns; k++) winv += Y[k] / M->W[k];
+  const double rho = p / (kRgas * T * winv);
+  kinetics(M, T, rho, Y, w, cpr, hrt);
+  double cp = 0.0, dh = 0.0;
+  for (int k = 0; k < ns; k++) {
+    cp += Y[k] * cpr[k] / M->W[k];
+    const double dY = w[k] * M->W[k] / rho;
+    f[1 + k] = dY;
+    dh += hrt[k] * T / M->W[k] * dY;  // (h_k / R) dY_k/dt
+  }
+  f[0] = -dh / cp;                    // R cancels between h_k and cp
+}
+
+// LU with partial pivoting of the n x n matrix a (row-major, stride NYMAX)
+__device__ bool lu_factor(double* a, int* piv, int n) {
+  for (int k = 0; k < n; k++) {
+    int pk = k;
+    double m = fabs(a[k * NYMAX + k]);
+    for (int i = k + 1; i < n; i++) {
+      const double v = fabs(a[i * NYMAX + k]);
+      if (v > m) { m = v; pk = i; }
+    }
+    piv[k] = pk;
+    if (!(m > 0.0)) return false;
+    if (pk != k)
+      for (int j = 0; j < n; j++) {
+        const double t = a[k * NYMAX + j];
+        a[k * NYMAX + j] = a[pk * NYMAX + j];
+        a[pk * NYMAX + j] = t;
+      }
+    const double inv = 1.0 / a[k * NYMAX + k];
+    for (int i = k + 1; i < n; i++) {
+      const double l = a[i * NYMAX + k] * inv;
+      a[i * NYMAX + k] = l;
+      for (int j = k + 1; j < n; j++) a[i * NYMAX + j] -= l * a[k * NYMAX + j];
+    }
+  }
+  return true;
+}
+
+__device__ void lu_solve(const double* a, const int* piv, int n, double* b) {
+  for (int k = 0; k < n; k++) {
+    if (piv[k] != k) {
+      const double t = b[k];
+      b[k] = b[piv[k]];
+      b[piv[k]] = t;
+    }
+    for (int i = k + 1; i < n; i++) b[i] -= a[i * NYMAX + k] * b[k];
+  }
+  for (int k = n - 1; k >= 0; k--) {
+    double s = b[k];
+    for (int j = k + 1; j < n; j++) s -= a[k * NYMAX + j] * b[j];
+    b[k] = s / a[k * NYMAX + k];
+  }
+}
+
+struct ReactorPool {
+  double* y;     // NYMAX x cap: T then Y_k
+  double* t;     // time reached
+  double* h;     // next step size
+  double* tend;
+  double* p;
+  int* status;   // 0 free, 1 running, 2 done, 3 failed
+  int* nstep;    // accepted steps
+  int* nrej;     // rejected steps
+  int cap;
+};
+
+enum { RX_FREE = 0, RX_RUN = 1, RX_DONE = 2, RX_FAIL = 3 };
+
+// Advance every running reactor by up to max_tries step attempts (or to its
+// end time).  A reactor's steps depend only on its own state, so splitting a
+// run into launches, or sharing a launch with other reactors, does not change
+// its result.
+__global__ __launch_bounds__(64) void k_reactor(const Mech* __restrict__ M, ReactorPool P,
+                                                int max_tries, double rtol, double atol) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.cap || P.status[i] != RX_RUN) return;
+  const int n = M->ns + 1;
+  const double gam = 1.0 + 0.70710678118654752440;
+  double y[NYMAX], f0[NYMAX], f1[NYMAX], k1[NYMAX], k2[NYMAX], yt[NYMAX];
+  double a[NYMAX * NYMAX];
+  int piv[NYMAX];
+  for (int k = 0; k < n; k++) y[k] = P.y[(size_t)k * P.cap + i];
+  double t = P.t[i], h = P.h[i];
+  const double tend = P.tend[i], p = P.p[i];
+  int nstep = P.nstep[i], nrej = P.nrej[i], st = RX_RUN;
+  for (int it = 0; it < max_tries; it++) {
+    if (t >= tend) { st = RX_DONE; break; }
+    if (h > tend - t) h = tend - t;
+    if (!(h > 1e-18)) { st = RX_FAIL; break; }
+    reactor_rhs(M, p, y, f0);
+    // forward-difference Jacobian, column by column
+    for (int j = 0; j < n; j++) {
+      for (int k = 0; k < n; k++) yt[k] = y[k];
+      const double d = 1.4901161193847656e-08 * fmax(fabs(y[j]), j == 0 ? 1.0 : 1e-8);
+      yt[j] = y[j] + d;
+      reactor_rhs(M, p, yt, f1);
+      for (int k = 0; k < n; k++) a[k * NYMAX + j] = -gam * h * (f1[k] - f0[k]) / d;
+    }
+    for (int k = 0; k < n; k++) a[k * NYMAX + k] += 1.0;
+    bool ok = lu_factor(a, piv, n);
+    double err = 2.0;
+    if (ok) {
+      for (int k = 0; k < n; k++) k1[k] = f0[k];
+      lu_solve(a, piv, n, k1);
+      for (int k = 0; k < n; k++) yt[k] = y[k] + h * k1[k];
+      reactor_rhs(M, p, yt, f1);
+      for (int k = 0; k < n; k++) k2[k] = f1[k] - 2.0 * k1[k];
+      lu_solve(a, piv, n, k2);
+      err = 0.0;
+      for (int k = 0; k < n; k++) {
+        const double yn = y[k] + 1.5 * h * k1[k] + 0.5 * h * k2[k];
+        const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
+        const double e = fabs(0.5 * h * (k1[k] + k2[k])) / sc;
+        if (!(e <= err)) err = e;  // a NaN makes err NaN and rejects the step
+        yt[k] = yn;
+      }
+    }
+    if (err <= 1.0) {
+      t += h;
+      for (int k = 0; k < n; k++) y[k] = yt[k];
+      nstep++;
+      const double fac = 0.9 / sqrt(fmax(err, 1e-10));
+      h *= fmin(5.0, fmax(0.2, fac));
+    } else {
+      nrej++;
+      const double fac = err < 1e300 ? 0.9 / sqrt(err) : 0.2;
+      h *= fmax(0.2, fmin(0.9, fac));
+    }
+    if (t >= tend) { st = RX_DONE; break; }
+  }
+  for (int k = 0; k < n; k++) P.y[(size_t)k * P.cap + i] = y[k];
+  P.t[i] = t;
+  P.h[i] = h;
+  P.nstep[i] = nstep;
+  P.nrej[i] = nrej;
+  P.status[i] = st;
 }
 
 // ---------------------------------------------------------------------------
@@ -382,6 +593,14 @@ struct astr_chem {
   int device = 0;
 };
 
+struct astr_reactor_pool {
+  astr_chem* chem = nullptr;
+  ReactorPool P{};
+  void* base = nullptr;
+  double rtol = 1e-6, atol = 1e-12;
+  hipStream_t stream = nullptr;
+};
+
 extern "C" {
 
 const char* astr_hip_chem_last_error(void) { return g_cerr.c_str(); }
@@ -488,6 +707,125 @@ int astr_hip_chem_rates(astr_chem* c, long long n, const double* T, const double
   }
   (void)hipFree(d);
   return rc;
+}
+
+int astr_hip_reactor_create(astr_chem* chem, int capacity, double rtol, double atol,
+                            astr_reactor_pool** out) {
+  if (!chem || !out || capacity < 1) return cfail("bad argument to astr_hip_reactor_create");
+  if (!(rtol > 0.0) || !(atol > 0.0)) return cfail("tolerances must be positive");
+  *out = nullptr;
+  CHIP_OK(hipSetDevice(chem->device));
+  astr_reactor_pool* r = new astr_reactor_pool();
+  r->chem = chem;
+  r->rtol = rtol;
+  r->atol = atol;
+  const size_t cap = (size_t)capacity;
+  const size_t bytes = (NYMAX + 4) * cap * sizeof(double) + 4 * cap * sizeof(int);
+  if (hipMalloc(&r->base, bytes) != hipSuccess) { delete r; return cfail("hipMalloc of reactor pool failed"); }
+  double* d = (double*)r->base;
+  r->P.y = d;
+  r->P.t = d + NYMAX * cap;
+  r->P.h = r->P.t + cap;
+  r->P.tend = r->P.h + cap;
+  r->P.p = r->P.tend + cap;
+  int* iv = (int*)(r->P.p + cap);
+  r->P.status = iv;
+  r->P.nstep = iv + cap;
+  r->P.nrej = iv + 2 * cap;
+  r->P.cap = capacity;
+  if (hipMemset(r->base, 0, bytes) != hipSuccess || hipStreamCreate(&r->stream) != hipSuccess) {
+    (void)hipFree(r->base);
+    delete r;
+    return cfail("reactor pool set-up failed");
+  }
+  *out = r;
+  return 0;
+}
+
+int astr_hip_reactor_destroy(astr_reactor_pool* r) {
+  if (!r) return 0;
+  (void)hipSetDevice(r->chem->device);
+  (void)hipStreamSynchronize(r->stream);
+  (void)hipFree(r->base);
+  (void)hipStreamDestroy(r->stream);
+  delete r;
+  return 0;
+}
+
+int astr_hip_reactor_set(astr_reactor_pool* r, int slot, double T0, double p, const double* Y0,
+                         double t_end, double h0) {
+  if (!r || !Y0) return cfail("null argument");
+  if (slot < 0 || slot >= r->P.cap) return cfail("reactor slot out of range");
+  if (!(T0 > 0.0) || !(p > 0.0) || !(t_end >= 0.0) || !(h0 > 0.0)) return cfail("bad reactor state");
+  CHIP_OK(hipSetDevice(r->chem->device));
+  const int ns = r->chem->host.ns;
+  const size_t cap = (size_t)r->P.cap;
+  double yv[NYMAX];
+  yv[0] = T0;
+  for (int k = 0; k < ns; k++) yv[1 + k] = Y0[k];
+  for (int k = 0; k <= ns; k++)
+    CHIP_OK(hipMemcpyAsync(r->P.y + k * cap + slot, &yv[k], sizeof(double), hipMemcpyHostToDevice, r->stream));
+  const double sc[4] = {0.0, h0, t_end, p};
+  double* dst[4] = {r->P.t, r->P.h, r->P.tend, r->P.p};
+  for (int q = 0; q < 4; q++)
+    CHIP_OK(hipMemcpyAsync(dst[q] + slot, &sc[q], sizeof(double), hipMemcpyHostToDevice, r->stream));
+  const int iv[3] = {RX_RUN, 0, 0};
+  int* idst[3] = {r->P.status, r->P.nstep, r->P.nrej};
+  for (int q = 0; q < 3; q++)
+    CHIP_OK(hipMemcpyAsync(idst[q] + slot, &iv[q], sizeof(int), hipMemcpyHostToDevice, r->stream));
+  CHIP_OK(hipStreamSynchronize(r->stream));
+  return 0;
+}
+
+int astr_hip_reactor_advance(astr_reactor_pool* r, int max_tries) {
+  if (!r) return cfail("null pool");
+  if (max_tries < 1) return cfail("max_tries must be positive");
+  CHIP_OK(hipSetDevice(r->chem->device));
+  const int bs = 64;
+  hipLaunchKernelGGL(k_reactor, dim3((r->P.cap + bs - 1) / bs), dim3(bs), 0, r->stream, r->chem->dev,
+                     r->P, max_tries, r->rtol, r->atol);
+  CHIP_OK(hipGetLastError());
+  return 0;
+}
+
+int astr_hip_reactor_sync(astr_reactor_pool* r) {
+  if (!r) return cfail("null pool");
+  CHIP_OK(hipStreamSynchronize(r->stream));
+  return 0;
+}
+
+int astr_hip_reactor_status(astr_reactor_pool* r, int* status) {
+  if (!r || !status) return cfail("null argument");
+  CHIP_OK(hipSetDevice(r->chem->device));
+  CHIP_OK(hipMemcpyAsync(status, r->P.status, r->P.cap * sizeof(int), hipMemcpyDeviceToHost, r->stream));
+  CHIP_OK(hipStreamSynchronize(r->stream));
+  return 0;
+}
+
+int astr_hip_reactor_get(astr_reactor_pool* r, int slot, double* t, double* yv, int* nstep, int* nrej) {
+  if (!r) return cfail("null pool");
+  if (slot < 0 || slot >= r->P.cap) return cfail("reactor slot out of range");
+  CHIP_OK(hipSetDevice(r->chem->device));
+  const int n = r->chem->host.ns + 1;
+  const size_t cap = (size_t)r->P.cap;
+  if (yv)
+    for (int k = 0; k < n; k++)
+      CHIP_OK(hipMemcpyAsync(&yv[k], r->P.y + k * cap + slot, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+  if (t) CHIP_OK(hipMemcpyAsync(t, r->P.t + slot, sizeof(double), hipMemcpyDeviceToHost, r->stream));
+  if (nstep) CHIP_OK(hipMemcpyAsync(nstep, r->P.nstep + slot, sizeof(int), hipMemcpyDeviceToHost, r->stream));
+  if (nrej) CHIP_OK(hipMemcpyAsync(nrej, r->P.nrej + slot, sizeof(int), hipMemcpyDeviceToHost, r->stream));
+  CHIP_OK(hipStreamSynchronize(r->stream));
+  return 0;
+}
+
+int astr_hip_reactor_release(astr_reactor_pool* r, int slot) {
+  if (!r) return cfail("null pool");
+  if (slot < 0 || slot >= r->P.cap) return cfail("reactor slot out of range");
+  CHIP_OK(hipSetDevice(r->chem->device));
+  const int v = RX_FREE;
+  CHIP_OK(hipMemcpyAsync(r->P.status + slot, &v, sizeof(int), hipMemcpyHostToDevice, r->stream));
+  CHIP_OK(hipStreamSynchronize(r->stream));
+  return 0;
 }
 
 }  // extern "C"

@@ -139,6 +139,39 @@ __global__ void k_init_tgv(FlowArgs a, double amp, uint64_t seed) {
   }
 }
 
+// Isentropic vortex (2-D, build-defined; no reference counterpart): free
+// stream rho=1, T=1, velocity (u0,v0); vortex of strength beta centred at
+// (xc,yc) with unit radius:
+//   du = -beta/(2 pi) (y-yc) e,  dv = beta/(2 pi) (x-xc) e,  e = exp((1-r^2)/2)
+//   T  = 1 - (gamma-1) Ma^2 beta^2/(8 pi^2) e^2   (cyclostrophic balance)
+//   rho = T^(1/(gamma-1)),  p = rho T/(gamma Ma^2)  (isentropic)
+// evaluated at i mod im, j mod jm so the duplicate planes equal planes 0.
+__global__ void k_init_vortex(FlowArgs a, double beta, double xc, double yc, double u0, double v0,
+                              double lx, double ly, double gm1, double ma2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i > a.n[0]) return;
+  const double pi = 3.141592653589793238462643383279502884;
+  const double x = lx * (double)(i % a.n[0]) / (double)a.n[0];
+  const double y = ly * (double)(j % a.n[1]) / (double)a.n[1];
+  const double dx = x - xc, dy = y - yc;
+  const double e = exp(0.5 * (1.0 - (dx * dx + dy * dy)));
+  const double u = u0 - beta / (2.0 * pi) * dy * e;
+  const double v = v0 + beta / (2.0 * pi) * dx * e;
+  const double t = 1.0 - gm1 * ma2 * beta * beta / (8.0 * pi * pi) * e * e;
+  const double rho = pow(t, 1.0 / gm1);
+  const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+  {
+#pragma clang fp contract(off)
+    const double p = rho * t / a.const2;
+    a.q[o] = rho;
+    a.q[a.fs + o] = rho * u;
+    a.q[2 * a.fs + o] = rho * v;
+    a.q[3 * a.fs + o] = 0.0;
+    a.q[4 * a.fs + o] = p * a.const6 + rho * 0.5 * (u * u + v * v);
+  }
+}
+
 // periodic index of a host-layout halo cell: q(-h)=q(n-h), q(n+h)=q(h)
 // (bchomovec, tgvsolver.F90:371-440); a direction with n = 0 (2-D) has one plane
 __device__ __forceinline__ int wrap_index(int i, int n) {
@@ -525,6 +558,16 @@ int astr_hip_init_tgv(astr_hip_ctx* c, double noise_amp, unsigned long long seed
   return 0;
 }
 
+int astr_hip_init_vortex(astr_hip_ctx* c, double beta, double xc, double yc, double u0, double v0) {
+  if (check_ctx(c)) return -1;
+  if (c->a.ndir != 2) return fail("the isentropic vortex is a two-dimensional case (km = 0)");
+  const double g = c->cfg.gamma, ma = c->cfg.mach;
+  hipLaunchKernelGGL(k_init_vortex, pt_grid(c->a), dim3(64), 0, c->stream, c->a, beta, xc, yc, u0,
+                     v0, c->cfg.lx, c->cfg.ly, g - 1.0, ma * ma);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
 // halo depth of a host buffer: 0..8 and at most the shortest direction, so a
 // halo cell wraps once
 static bool bad_halo(const FlowArgs& a, int hm) {
@@ -817,6 +860,442 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
   (void)hipFree(dt);
   (void)hipFree(df);
   (void)hipFree(dout);
+  return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batching: many independent cases advanced together.  Every pass of a stage
+// is launched once per group of cases sharing (program, launch shape,
+// register budget) as a persistent kernel over the groups' concatenated
+// tiles; a whole RK3 step can be captured in a hipGraph and replayed.
+// ---------------------------------------------------------------------------
+namespace {
+
+struct StatsJob {
+  const double* part;
+  int ntiles;
+  double scale;
+  double* out;  // 2 doubles per step
+};
+
+// per-case KE / enstrophy of the current step (index *step), one block per case
+__global__ __launch_bounds__(256) void k_stats_batch(const StatsJob* __restrict__ jobs,
+                                                     const int* __restrict__ step) {
+  __shared__ double sa[256], sb[256];
+  const StatsJob J = jobs[blockIdx.x];
+  double x = 0.0, y = 0.0;
+  for (int i = threadIdx.x; i < J.ntiles; i += 256) {
+    x += J.part[2 * i];
+    y += J.part[2 * i + 1];
+  }
+  sa[threadIdx.x] = x;
+  sb[threadIdx.x] = y;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) {
+      sa[threadIdx.x] += sa[threadIdx.x + h];
+      sb[threadIdx.x] += sb[threadIdx.x + h];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const int s = *step;
+    J.out[2 * s] = 0.5 * sa[0] * J.scale;
+    J.out[2 * s + 1] = 0.5 * sb[0] * J.scale;
+  }
+}
+
+__global__ void k_set_int(int* p, int v) { *p = v; }
+__global__ void k_add_int(int* p, int v) { *p += v; }
+
+hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, int dir,
+                        const StageArgs& sa, int nblocks, hipStream_t st) {
+  switch (shape) {
+    case 0: return launch_batch_s0(prog, minb, bl, dir, sa, nblocks, st);
+    case 1: return launch_batch_s1(prog, minb, bl, dir, sa, nblocks, st);
+    case 2: return launch_batch_s2(prog, minb, bl, dir, sa, nblocks, st);
+    case 3: return launch_batch_s3(prog, minb, bl, dir, sa, nblocks, st);
+    default: return launch_batch_s4(prog, minb, bl, dir, sa, nblocks, st);
+  }
+}
+
+// the eight pass slots of a stage: GC x,y,z; V x,y,z; F x,y.  The program a
+// case of nd directions runs in slot (kind, d), or -1.
+int slot_prog(int kind, int d, int nd) {
+  if (kind == 0) return d < nd ? (d == nd - 1 ? GC_LAST : (d == 0 ? GC_FIRST : GC_MID)) : -1;
+  if (kind == 1) return d < nd ? (d == nd - 1 ? V_LAST_RK : V_MID) : -1;
+  return d < nd - 1 ? F_ONLY : -1;
+}
+const int kSlotKind[8] = {0, 0, 0, 1, 1, 1, 2, 2};
+const int kSlotDir[8] = {0, 1, 2, 0, 1, 2, 0, 1};
+
+struct BatchGroup {
+  int slot, prog, dir, shape, minb, nblocks;
+  std::vector<int> members;
+  FlowArgs* d_args = nullptr;
+  int* d_tstart = nullptr;
+  BatchLaunch bl{};
+};
+
+}  // namespace
+
+struct astr_hip_batch {
+  std::vector<astr_hip_ctx*> cases;
+  int device = 0;
+  int ncu = 256;
+  hipStream_t stream = nullptr;
+  std::vector<BatchGroup> groups;  // in slot order
+  StatsJob* d_jobs = nullptr;
+  int* d_step = nullptr;
+  double* d_stats = nullptr;
+  int stats_cap = 0;
+  hipGraphExec_t graph = nullptr;
+  bool graph_valid = false;
+};
+
+namespace {
+
+void free_groups(astr_hip_batch* b) {
+  for (auto& g : b->groups) {
+    (void)hipFree(g.d_args);
+    (void)hipFree(g.d_tstart);
+  }
+  b->groups.clear();
+}
+
+void drop_graph(astr_hip_batch* b) {
+  if (b->graph) (void)hipGraphExecDestroy(b->graph);
+  b->graph = nullptr;
+  b->graph_valid = false;
+}
+
+// group the cases of every slot by (program, shape, budget) from each case's
+// launch selection; upload the groups' case arrays and tile offsets
+int build_groups(astr_hip_batch* b) {
+  drop_graph(b);
+  free_groups(b);
+  for (int sl = 0; sl < 8; sl++) {
+    const int kind = kSlotKind[sl], d = kSlotDir[sl];
+    std::vector<BatchGroup> here;
+    for (int i = 0; i < (int)b->cases.size(); i++) {
+      astr_hip_ctx* c = b->cases[i];
+      const int prog = slot_prog(kind, d, c->a.ndir);
+      if (prog < 0) continue;
+      const int sh = c->shape[prog][d], mb = c->minb[prog][d];
+      BatchGroup* g = nullptr;
+      for (auto& h : here)
+        if (h.prog == prog && h.shape == sh && h.minb == mb) g = &h;
+      if (!g) {
+        here.emplace_back();
+        g = &here.back();
+        g->slot = sl; g->prog = prog; g->dir = d; g->shape = sh; g->minb = mb;
+      }
+      g->members.push_back(i);
+    }
+    for (auto& g : here) b->groups.push_back(g);
+  }
+  for (auto& g : b->groups) {
+    const int nm = (int)g.members.size();
+    std::vector<FlowArgs> args(nm);
+    std::vector<int> ts(nm + 1, 0);
+    for (int k = 0; k < nm; k++) {
+      const FlowArgs& a = b->cases[g.members[k]]->a;
+      args[k] = a;
+      ts[k + 1] = ts[k] + tiles_of(a.n, g.dir, kShapes[g.shape].TL);
+    }
+    HIP_OK(hipMalloc(&g.d_args, nm * sizeof(FlowArgs)));
+    HIP_OK(hipMalloc(&g.d_tstart, (nm + 1) * sizeof(int)));
+    HIP_OK(hipMemcpy(g.d_args, args.data(), nm * sizeof(FlowArgs), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(g.d_tstart, ts.data(), (nm + 1) * sizeof(int), hipMemcpyHostToDevice));
+    g.bl.cases = g.d_args;
+    g.bl.tstart = g.d_tstart;
+    g.bl.ncases = nm;
+    g.bl.total = ts[nm];
+    // persistent grid: a multiple of 8 blocks, at most as many blocks as the
+    // budget keeps resident (minb per CU), never more than the tiles
+    const int cap = b->ncu * g.minb;
+    int nb = g.bl.total < cap ? g.bl.total : cap;
+    g.nblocks = (nb + 7) / 8 * 8;
+  }
+  return 0;
+}
+
+// the cases' FlowArgs can change between calls (time step); refresh the copies
+int refresh_args(astr_hip_batch* b) {
+  for (auto& g : b->groups) {
+    std::vector<FlowArgs> args(g.members.size());
+    for (size_t k = 0; k < g.members.size(); k++) args[k] = b->cases[g.members[k]]->a;
+    HIP_OK(hipMemcpyAsync(g.d_args, args.data(), args.size() * sizeof(FlowArgs),
+                          hipMemcpyHostToDevice, b->stream));
+  }
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+// stats jobs for a step capacity of cap steps
+int ensure_stats(astr_hip_batch* b, int cap) {
+  if (cap <= b->stats_cap && b->d_jobs) return 0;
+  drop_graph(b);
+  (void)hipFree(b->d_stats);
+  (void)hipFree(b->d_jobs);
+  b->d_stats = nullptr;
+  b->d_jobs = nullptr;
+  const int nc = (int)b->cases.size();
+  HIP_OK(hipMalloc(&b->d_stats, 2 * (size_t)nc * cap * sizeof(double)));
+  HIP_OK(hipMalloc(&b->d_jobs, nc * sizeof(StatsJob)));
+  b->stats_cap = cap;
+  return 0;
+}
+
+int upload_jobs(astr_hip_batch* b) {
+  const int nc = (int)b->cases.size();
+  std::vector<StatsJob> jobs(nc);
+  for (int i = 0; i < nc; i++) {
+    astr_hip_ctx* c = b->cases[i];
+    const int nd = c->a.ndir;
+    jobs[i].part = c->stats_part;
+    jobs[i].ntiles = tiles_of(c->a.n, nd - 1, kShapes[c->shape[GC_LAST][nd - 1]].TL);
+    jobs[i].scale = c->ke_scale;
+    jobs[i].out = b->d_stats + 2 * (size_t)i * b->stats_cap;
+  }
+  HIP_OK(hipMemcpy(b->d_jobs, jobs.data(), nc * sizeof(StatsJob), hipMemcpyHostToDevice));
+  return 0;
+}
+
+// enqueue one RK3 step of every case on the batch stream
+int enqueue_step(astr_hip_batch* b) {
+  static const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  for (int s = 0; s < 3; s++) {
+    StageArgs sa{s, s == 0 ? 1 : 0, 0, rk[s][0], rk[s][1], rk[s][2]};
+    StageArgs sv = sa;
+    sv.dostats = 0;
+    for (int sl = 0; sl < 8; sl++) {
+      for (auto& g : b->groups) {
+        if (g.slot != sl) continue;
+        HIP_OK(launch_batch(g.shape, g.prog, g.minb, g.bl, g.dir, g.prog == GC_LAST ? sa : sv,
+                            g.nblocks, b->stream));
+      }
+      if (sl == 2 && s == 0) {
+        hipLaunchKernelGGL(k_stats_batch, dim3((unsigned)b->cases.size()), dim3(256), 0, b->stream,
+                           b->d_jobs, b->d_step);
+        HIP_OK(hipGetLastError());
+      }
+    }
+  }
+  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(1), 0, b->stream, b->d_step, 1);
+  HIP_OK(hipGetLastError());
+  return 0;
+}
+
+int capture_step(astr_hip_batch* b) {
+  drop_graph(b);
+  hipGraph_t g = nullptr;
+  HIP_OK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_step(b);
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return -1;
+  }
+  HIP_OK(e);
+  e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  HIP_OK(e);
+  b->graph_valid = true;
+  return 0;
+}
+
+int batch_sync_cases(astr_hip_batch* b) {
+  for (astr_hip_ctx* c : b->cases) HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int astr_hip_batch_create(astr_hip_ctx* const* cases, int ncases, astr_hip_batch** out) {
+  if (!cases || !out || ncases < 1) return fail("bad argument to astr_hip_batch_create");
+  *out = nullptr;
+  astr_hip_batch* b = new astr_hip_batch();
+  for (int i = 0; i < ncases; i++) {
+    if (!cases[i]) { delete b; return fail("null case"); }
+    if (i && cases[i]->device != cases[0]->device) { delete b; return fail("cases on different devices"); }
+    b->cases.push_back(cases[i]);
+  }
+  b->device = cases[0]->device;
+  int rc = 0;
+  if (hipSetDevice(b->device) != hipSuccess) rc = fail("hipSetDevice failed");
+  if (!rc && hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
+    rc = fail("device attribute query failed");
+  if (!rc && hipStreamCreate(&b->stream) != hipSuccess) rc = fail("hipStreamCreate failed");
+  if (!rc && hipMalloc(&b->d_step, sizeof(int)) != hipSuccess) rc = fail("hipMalloc failed");
+  if (!rc) rc = build_groups(b);
+  if (!rc) rc = ensure_stats(b, 16);
+  if (!rc) rc = upload_jobs(b);
+  if (rc) {
+    astr_hip_batch_destroy(b);
+    return -1;
+  }
+  *out = b;
+  return 0;
+}
+
+int astr_hip_batch_destroy(astr_hip_batch* b) {
+  if (!b) return 0;
+  (void)hipSetDevice(b->device);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  drop_graph(b);
+  free_groups(b);
+  (void)hipFree(b->d_jobs);
+  (void)hipFree(b->d_stats);
+  (void)hipFree(b->d_step);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+  return 0;
+}
+
+int astr_hip_batch_rk3_steps(astr_hip_batch* b, int nsteps, int use_graph, double* ke, double* ens) {
+  if (!b) return fail("null batch");
+  if (nsteps < 0) return fail("negative step count");
+  HIP_OK(hipSetDevice(b->device));
+  if (batch_sync_cases(b)) return -1;
+  if (nsteps == 0) return 0;
+  if (ensure_stats(b, nsteps) || upload_jobs(b) || refresh_args(b)) return -1;
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, b->stream, b->d_step, 0);
+  HIP_OK(hipGetLastError());
+  if (use_graph) {
+    if (!b->graph_valid && capture_step(b)) return -1;
+    for (int it = 0; it < nsteps; it++) HIP_OK(hipGraphLaunch(b->graph, b->stream));
+  } else {
+    for (int it = 0; it < nsteps; it++)
+      if (enqueue_step(b)) return -1;
+  }
+  const int nc = (int)b->cases.size();
+  if (ke || ens) {
+    std::vector<double> h(2 * (size_t)nc * b->stats_cap);
+    HIP_OK(hipMemcpyAsync(h.data(), b->d_stats, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+    HIP_OK(hipStreamSynchronize(b->stream));
+    for (int i = 0; i < nc; i++)
+      for (int it = 0; it < nsteps; it++) {
+        if (ke) ke[(size_t)i * nsteps + it] = h[2 * ((size_t)i * b->stats_cap + it)];
+        if (ens) ens[(size_t)i * nsteps + it] = h[2 * ((size_t)i * b->stats_cap + it) + 1];
+      }
+  }
+  HIP_OK(hipStreamSynchronize(b->stream));
+  return 0;
+}
+
+int astr_hip_batch_autotune(astr_hip_batch* b, int reps) {
+  if (!b) return fail("null batch");
+  if (reps < 1 || reps > 1000) return fail("autotune reps must be 1..1000");
+  HIP_OK(hipSetDevice(b->device));
+  if (batch_sync_cases(b)) return -1;
+  const int nc = (int)b->cases.size();
+  // snapshots of q and qsave, put back afterwards
+  std::vector<double*> snap(nc, nullptr);
+  int rc = 0;
+  for (int i = 0; i < nc && !rc; i++) {
+    const FlowArgs& a = b->cases[i]->a;
+    const size_t qb = (size_t)10 * a.fs * sizeof(double);
+    if (hipMalloc(&snap[i], qb) != hipSuccess) rc = fail("hipMalloc failed");
+    else if (hipMemcpyAsync(snap[i], a.q, qb, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+      rc = fail("snapshot copy failed");
+  }
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = fail("hipEventCreate failed");
+  const StageArgs sa{0, 0, 0, 1.0, 0.0, 1.0};
+  for (int sl = 0; sl < 8 && !rc; sl++) {
+    const int kind = kSlotKind[sl], d = kSlotDir[sl];
+    // classes: cases of this slot with the same program and the same set of
+    // fitting shapes; each class gets the fastest common selection
+    std::vector<std::pair<std::pair<int, int>, std::vector<int>>> classes;
+    for (int i = 0; i < nc; i++) {
+      astr_hip_ctx* c = b->cases[i];
+      const int prog = slot_prog(kind, d, c->a.ndir);
+      if (prog < 0) continue;
+      int mask = 0;
+      for (int sh = 0; sh < kNumShapes; sh++)
+        if (shape_fits(kShapes[sh], c->a.n[d])) mask |= 1 << sh;
+      bool placed = false;
+      for (auto& cl : classes)
+        if (cl.first.first == prog && cl.first.second == mask) { cl.second.push_back(i); placed = true; }
+      if (!placed) classes.push_back({{prog, mask}, {i}});
+    }
+    for (auto& cl : classes) {
+      if (rc) break;
+      const int prog = cl.first.first, mask = cl.first.second;
+      const int nm = (int)cl.second.size();
+      std::vector<FlowArgs> args(nm);
+      for (int k = 0; k < nm; k++) args[k] = b->cases[cl.second[k]]->a;
+      FlowArgs* dargs = nullptr;
+      int* dts = nullptr;
+      if (hipMalloc(&dargs, nm * sizeof(FlowArgs)) != hipSuccess ||
+          hipMalloc(&dts, (nm + 1) * sizeof(int)) != hipSuccess ||
+          hipMemcpy(dargs, args.data(), nm * sizeof(FlowArgs), hipMemcpyHostToDevice) != hipSuccess) {
+        rc = fail("autotune set-up failed");
+      }
+      float best = 1e30f;
+      int bsh = -1, bmb = 1;
+      for (int sh = 0; sh < kNumShapes && !rc; sh++) {
+        if (!((mask >> sh) & 1)) continue;
+        std::vector<int> ts(nm + 1, 0);
+        for (int k = 0; k < nm; k++) ts[k + 1] = ts[k] + tiles_of(args[k].n, d, kShapes[sh].TL);
+        if (hipMemcpy(dts, ts.data(), (nm + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+          rc = fail("autotune copy failed");
+          break;
+        }
+        BatchLaunch bl{dargs, dts, nm, ts[nm]};
+        for (int mb = 1; mb <= kMinbVariants && !rc; mb++) {
+          const int cap = b->ncu * mb;
+          const int nb = ((bl.total < cap ? bl.total : cap) + 7) / 8 * 8;
+          hipError_t e = launch_batch(sh, prog, mb, bl, d, sa, nb, b->stream);
+          if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+          for (int r = 0; r < reps && e == hipSuccess; r++) e = launch_batch(sh, prog, mb, bl, d, sa, nb, b->stream);
+          if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) {
+            rc = fail(std::string("batch autotune launch: ") + hipGetErrorString(e));
+            break;
+          }
+          if (ms < best) { best = ms; bsh = sh; bmb = mb; }
+        }
+      }
+      (void)hipFree(dargs);
+      (void)hipFree(dts);
+      if (!rc && bsh >= 0)
+        for (int i : cl.second) {
+          b->cases[i]->shape[prog][d] = bsh;
+          b->cases[i]->minb[prog][d] = bmb;
+          b->cases[i]->tuned_ms[prog][d] = best / reps;
+        }
+    }
+  }
+  for (int i = 0; i < nc; i++) {
+    if (!snap[i]) continue;
+    const FlowArgs& a = b->cases[i]->a;
+    const size_t qb = (size_t)10 * a.fs * sizeof(double);
+    if (!rc && hipMemcpyAsync(a.q, snap[i], qb, hipMemcpyDeviceToDevice, b->stream) != hipSuccess)
+      rc = fail("restore copy failed");
+  }
+  if (hipStreamSynchronize(b->stream) != hipSuccess && !rc) rc = fail("autotune sync failed");
+  for (double* p : snap) (void)hipFree(p);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if (build_groups(b) || upload_jobs(b)) return -1;
+  return 0;
+}
+
+int astr_hip_batch_info(astr_hip_batch* b, int* ngroups, int* launches_per_step) {
+  if (!b) return fail("null batch");
+  if (ngroups) *ngroups = (int)b->groups.size();
+  if (launches_per_step) *launches_per_step = 3 * (int)b->groups.size() + 2;
   return 0;
 }
 

@@ -188,10 +188,23 @@ __device__ __forceinline__ double miucal(const FlowArgs& a, double t) {
   return t * sqrt(t) * a.suth1 / (t + a.suth);
 }
 
+// what changes from stage to stage (the same for every case of a batch)
+struct StageArgs {
+  int stage;            // 0,1,2
+  int dostats;          // stage-1 KE/enstrophy partial sums
+  int store_last_grad;  // last GC pass also stores its gradients
+  double rk_a, rk_b, rk_c;
+};
+
+__device__ __forceinline__ StageArgs stage_of(const FlowArgs& a) {
+  return StageArgs{a.stage, a.dostats, a.store_last_grad, a.rk_a, a.rk_b, a.rk_c};
+}
+
 // q <- a*qsave + b*q + c*qrhs*dt in the reference's order (tgvsolver.F90:1055-1057)
-__device__ __forceinline__ double rk_update(const FlowArgs& a, double qs, double qo, double qr) {
+__device__ __forceinline__ double rk_update(const StageArgs& sa, double dt, double qs, double qo,
+                                            double qr) {
 #pragma clang fp contract(off)
-  return a.rk_a * qs + a.rk_b * qo + a.rk_c * qr * a.dt;
+  return sa.rk_a * qs + sa.rk_b * qo + sa.rk_c * qr * dt;
 }
 
 // stress, heat flux (tgvsolver.F90:663-691); g[m][d] = d u_m / d x_d
@@ -224,14 +237,14 @@ __device__ __forceinline__ void visc_flux(const FlowArgs& a, const double g[3][3
 // ---------------------------------------------------------------------------
 // MINB: workgroups per CU the register allocation is held to (each
 // workgroup puts one wave on each SIMD, so MINB waves per SIMD)
-template <class C, int PROG, int MINB>
-__global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
-  extern __shared__ double smem[];
+// One tile of one pass.  `a` describes the case, `sa` the stage.
+template <class C, int PROG>
+__device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa, int dir, int tile,
+                                          double* smem) {
   constexpr int NPT = C::NPT;
   const int n = a.n[dir];
   double* X = smem;                 // output lines, TL x LT (rows at offset 0)
   double* T = smem + C::TL * C::LT; // input lines, TL x LT
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
   Slots<C> S;
   make_slots<C>(S, a, dir, tile);
   zero_pad<C>(T, n);
@@ -278,7 +291,7 @@ __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
       for (int t = 0; t < NPT; t++) {
         {
           const double v = X[S.s[t]] * idn;
-          if (!LAST || a.store_last_grad) a.dv[fld * fs + S.g[t]] = v;
+          if (!LAST || sa.store_last_grad) a.dv[fld * fs + S.g[t]] = v;
           gz[c][t] = v;
         }
       }
@@ -310,7 +323,7 @@ __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
         visc_flux(a, g, gt, P, f);
 #pragma unroll
         for (int m = 0; m < 9; m++) a.sg[m * fs + o] = f[m];
-        if (a.dostats && ((S.interior >> t) & 1)) {
+        if (sa.dostats && ((S.interior >> t) & 1)) {
           sk += P.r * (P.u * P.u + P.v * P.v + P.w * P.w);
           const double o1 = g[2][1] - g[1][2];
           const double o2 = g[0][2] - g[2][0];
@@ -318,7 +331,7 @@ __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
           se += P.r * (o1 * o1 + o2 * o2 + o3 * o3);
         }
       }
-      if (a.dostats) {
+      if (sa.dostats) {
         // deterministic block reduction into this tile's slot
         __syncthreads();
         X[threadIdx.x] = sk;
@@ -425,13 +438,13 @@ __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
           const double qr = m == 0 ? a.qrhs[o] : acc[m - 1][t];
           const double qo = a.q[m * fs + o];
           double qs;
-          if (a.stage == 0) {
+          if (sa.stage == 0) {
             a.qsave[m * fs + o] = qo;
             qs = qo;
           } else {
             qs = a.qsave[m * fs + o];
           }
-          qn[m][t] = rk_update(a, qs, qo, qr);
+          qn[m][t] = rk_update(sa, a.dt, qs, qo, qr);
         }
       }
       ChunkCoef<C::R, false> CF;
@@ -476,6 +489,59 @@ __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
   }
 }
 
+// one pass of one case: block b handles tile xcd_tile(b)
+template <class C, int PROG, int MINB>
+__global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
+  extern __shared__ double smem[];
+  pass_body<C, PROG>(a, stage_of(a), dir, xcd_tile(blockIdx.x, gridDim.x), smem);
+}
+
+// The tiles of one pass over several cases: case k owns global tiles
+// tstart[k]..tstart[k+1]-1.  Device arrays, built by the batch host code.
+struct BatchLaunch {
+  const FlowArgs* cases;
+  const int* tstart;
+  int ncases;
+  int total;
+};
+
+// one pass over a batch of cases: persistent blocks (grid a multiple of 8);
+// the blocks of XCD x walk the contiguous range [x*total/8, (x+1)*total/8) so
+// neighbouring tiles share that XCD's L2
+template <class C, int PROG, int MINB>
+__global__ __launch_bounds__(C::NT, MINB) void k_pass_batch(BatchLaunch bl, int dir, StageArgs sa) {
+  extern __shared__ double smem[];
+  const int x = blockIdx.x & 7, per = gridDim.x >> 3;
+  const int lo = (int)((long long)bl.total * x / 8), hi = (int)((long long)bl.total * (x + 1) / 8);
+  for (int g = lo + (int)(blockIdx.x >> 3); g < hi; g += per) {
+    int kl = 0, kh = bl.ncases;  // tstart[kl] <= g < tstart[kh]
+    while (kh - kl > 1) {
+      const int mid = (kl + kh) >> 1;
+      if (bl.tstart[mid] <= g) kl = mid;
+      else kh = mid;
+    }
+    pass_body<C, PROG>(bl.cases[kl], sa, dir, g - bl.tstart[kl], smem);
+    __syncthreads();
+  }
+}
+
+template <class C, int MINB>
+hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageArgs& sa, int nblocks,
+                          hipStream_t st) {
+  const size_t lds = (size_t)lds_doubles<C>() * sizeof(double);
+  dim3 g(nblocks), b(C::NT);
+  switch (prog) {
+    case GC_FIRST: hipLaunchKernelGGL((k_pass_batch<C, GC_FIRST, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case GC_MID: hipLaunchKernelGGL((k_pass_batch<C, GC_MID, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case GC_LAST: hipLaunchKernelGGL((k_pass_batch<C, GC_LAST, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case V_MID: hipLaunchKernelGGL((k_pass_batch<C, V_MID, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case V_LAST_RHS: hipLaunchKernelGGL((k_pass_batch<C, V_LAST_RHS, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case V_LAST_RK: hipLaunchKernelGGL((k_pass_batch<C, V_LAST_RK, MINB>), g, b, lds, st, bl, dir, sa); break;
+    default: hipLaunchKernelGGL((k_pass_batch<C, F_ONLY, MINB>), g, b, lds, st, bl, dir, sa); break;
+  }
+  return hipGetLastError();
+}
+
 template <class C, int MINB>
 hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
   const int nt = tiles_of(a.n, dir, C::TL);
@@ -503,10 +569,15 @@ constexpr int kNumShapes = 5;
 constexpr int kMinbVariants = 2;       // MINB = 1, 2
 
 // one per shape, each in its own translation unit (passes.hip)
-hipError_t launch_pass_s0(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
-hipError_t launch_pass_s1(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
-hipError_t launch_pass_s2(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
-hipError_t launch_pass_s3(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
-hipError_t launch_pass_s4(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st);
+#define ASTR_DECLARE_SHAPE(K)                                                                 \
+  hipError_t launch_pass_s##K(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st); \
+  hipError_t launch_batch_s##K(int prog, int minb, const BatchLaunch& bl, int dir,             \
+                               const StageArgs& sa, int nblocks, hipStream_t st);
+ASTR_DECLARE_SHAPE(0)
+ASTR_DECLARE_SHAPE(1)
+ASTR_DECLARE_SHAPE(2)
+ASTR_DECLARE_SHAPE(3)
+ASTR_DECLARE_SHAPE(4)
+#undef ASTR_DECLARE_SHAPE
 
 }  // namespace astr

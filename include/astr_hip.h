@@ -73,6 +73,14 @@ int astr_hip_field_info(astr_hip_ctx *ctx, astr_hip_layout *layout);
  * the reference) on i,j,k < n, copied to the duplicate planes. */
 int astr_hip_init_tgv(astr_hip_ctx *ctx, double noise_amp, unsigned long long seed);
 
+/* Two-dimensional isentropic vortex (km = 0; no reference counterpart, a
+ * build-defined initial field for the batched and served workloads): free
+ * stream rho = 1, T = 1, velocity (u0, v0); a vortex of strength beta and unit
+ * radius at (xc, yc):  du = -beta/(2 pi) (y-yc) e,  dv = beta/(2 pi) (x-xc) e,
+ * e = exp((1-r^2)/2),  T = 1 - (gamma-1) Ma^2 beta^2/(8 pi^2) e^2,
+ * rho = T^(1/(gamma-1)),  p = rho T/(gamma Ma^2). */
+int astr_hip_init_vortex(astr_hip_ctx *ctx, double beta, double xc, double yc, double u0, double v0);
+
 /* set / read the conservative state q(-hm:..,1:5); download also returns the
  * primitives rho,u,v,w,p,T (q2fvar, tgvsolver.F90:92-121) when prims != 0. */
 int astr_hip_upload(astr_hip_ctx *ctx, const double *q, int hm);
@@ -114,6 +122,29 @@ int astr_hip_autotune(astr_hip_ctx *ctx, int reps);
 int astr_hip_get_pass_config(astr_hip_ctx *ctx, int prog, int dir, int *shape, int *minb, float *ms);
 int astr_hip_set_pass_config(astr_hip_ctx *ctx, int prog, int dir, int shape, int minb);
 
+/* ------------------------------------------------------------------------
+ * Batching (no reference counterpart: the reference runs one case per job).
+ * A batch advances many independent contexts together:
+ *   - each pass of a stage is one persistent launch per group of cases that
+ *     share program, launch shape and register budget;
+ *   - with use_graph != 0, a whole RK3 step is captured once as a hipGraph
+ *     and replayed.
+ * Each case's result equals its own astr_hip_rk3_steps under the same launch
+ * selection.
+ * Rules:
+ *   - the cases stay owned by the caller and must outlive the batch;
+ *   - astr_hip_batch_autotune picks one selection per class of cases (same
+ *     program and set of fitting shapes) per pass, by timing, and writes it
+ *     into the cases' own selections;
+ *   - ke/ens (nullable) receive ncases x nsteps values, case-major.
+ * ------------------------------------------------------------------------ */
+typedef struct astr_hip_batch astr_hip_batch;
+int astr_hip_batch_create(astr_hip_ctx *const *cases, int ncases, astr_hip_batch **out);
+int astr_hip_batch_destroy(astr_hip_batch *batch);
+int astr_hip_batch_autotune(astr_hip_batch *batch, int reps);
+int astr_hip_batch_rk3_steps(astr_hip_batch *batch, int nsteps, int use_graph, double *ke, double *ens);
+int astr_hip_batch_info(astr_hip_batch *batch, int *ngroups, int *launches_per_step);
+
 /* fds%central / compact_filter on a batch of lines: derivative.F90:171-198
  * (op 0, df_compact, scheme 643c) and filter.F90:112-144 (op 1), ntype=3.
  * f holds nlines lines of f(-5:n+5) (n+11 values each), out nlines x (n+1). */
@@ -146,6 +177,40 @@ int astr_hip_chem_rates_device(astr_chem *chem, long long n, const double *T,
                                const double *rho_or_p, int input_is_p, const double *Y,
                                double *wdot, double *h_mix, double *cp_mix, double *h_k,
                                void *stream);
+
+/* ------------------------------------------------------------------------
+ * 0-D constant-pressure adiabatic reactors (no reference counterpart: the
+ * reference's 0dreactor, src/solver.F90:208,256-258, is constant-volume).
+ *
+ * The model: state (T, Y_k), dY_k/dt = wdot_k W_k / rho,
+ * dT/dt = -sum_k h_k dY_k/dt / cp.
+ *
+ * The integrator: ROS2, the two-stage L-stable Rosenbrock method (Verwer et
+ * al. 1999), with a finite-difference Jacobian and error control at rtol,
+ * atol (mass fractions; temperature uses atol 1e-6 K).  One thread integrates
+ * one reactor.
+ *
+ * A pool holds `capacity` slots:
+ *   - set starts a reactor in a slot;
+ *   - advance launches up to max_tries step attempts for every running slot
+ *     (asynchronous, on the pool's own stream);
+ *   - status reads all slots: 0 free, 1 running, 2 done, 3 failed;
+ *   - get reads one slot's t, (T, Y) and step counts;
+ *   - release frees a slot.
+ * A reactor's result does not depend on the other slots or on how its run is
+ * split into launches.
+ * ------------------------------------------------------------------------ */
+typedef struct astr_reactor_pool astr_reactor_pool;
+int astr_hip_reactor_create(astr_chem *chem, int capacity, double rtol, double atol,
+                            astr_reactor_pool **out);
+int astr_hip_reactor_destroy(astr_reactor_pool *pool);
+int astr_hip_reactor_set(astr_reactor_pool *pool, int slot, double T0, double p, const double *Y0,
+                         double t_end, double h0);
+int astr_hip_reactor_advance(astr_reactor_pool *pool, int max_tries);
+int astr_hip_reactor_sync(astr_reactor_pool *pool);
+int astr_hip_reactor_status(astr_reactor_pool *pool, int *status);
+int astr_hip_reactor_get(astr_reactor_pool *pool, int slot, double *t, double *TY, int *nstep, int *nrej);
+int astr_hip_reactor_release(astr_reactor_pool *pool, int slot);
 
 #ifdef __cplusplus
 }

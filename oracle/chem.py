@@ -167,3 +167,29 @@ class Mechanism:
         if parts:
             return wdot, hmix, cpmix, hk, np.array(fwd_all), np.array(rev_all)
         return wdot, hmix, cpmix, hk
+
+    # ---- 0-D constant-pressure adiabatic reactor ------------------------
+    def reactor_rhs(self, yv, p):
+        """d(T, Y_k)/dt of the constant-pressure adiabatic reactor:
+        dY_k/dt = wdot_k W_k / rho, dT/dt = -sum_k h_k dY_k/dt / cp."""
+        T = np.array([yv[0]])
+        Y = np.asarray(yv[1:], dtype=float)[:, None]
+        wd, _, cp, _ = self.rates(T, np.array([p]), Y, input_is_p=True)
+        winv = (Y[:, 0] / self.W).sum()
+        rho = p / (R_GAS * T[0] * winv)
+        dY = wd[:, 0] / rho
+        _, hrt, _ = self.thermo(T)
+        hk = hrt[0] * R_GAS * T[0] / self.W
+        return np.concatenate([[-(hk * dY).sum() / cp[0]], dY])
+
+    def ignite(self, T0, p, Y0, t_end, rtol=1e-11, atol=1e-16):
+        """Reference solution of the reactor by scipy's Radau (implicit
+        Runge-Kutta, order 5) at tight tolerances; returns (T, Y) at t_end."""
+        from scipy.integrate import solve_ivp
+
+        y0 = np.concatenate([[T0], np.asarray(Y0, dtype=float)])
+        sol = solve_ivp(lambda t, y: self.reactor_rhs(y, p), (0.0, t_end), y0, method="Radau",
+                        rtol=rtol, atol=atol)
+        if not sol.success:
+            raise RuntimeError(sol.message)
+        return sol.y[:, -1]

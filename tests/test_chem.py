@@ -140,3 +140,87 @@ def test_gpu_rates_density_input_and_conservation(mech):
     scale = np.abs(wd).max(axis=0) + 1e-300
     assert np.all(np.abs(wd.sum(axis=0)) <= 1e-13 * scale)
     assert np.all(wd[mech.names.index("N2")] == 0.0)
+
+
+def _mix(mech, phi):
+    ix = {s: k for k, s in enumerate(mech.names)}
+    X = np.zeros(mech.ns)
+    X[ix["H2"]], X[ix["O2"]], X[ix["N2"]] = 2.0 * phi, 1.0, 3.76
+    Y = X * mech.W
+    return Y / Y.sum()
+
+
+def test_reactor_rhs_conserves_enthalpy_and_elements(mech):
+    """Along the reactor's trajectory the mixture enthalpy sum_k Y_k h_k(T) and
+    the element mass fractions stay constant (pins the sign and scale of dT/dt
+    and the mass-fraction rates); the mixture ignites (T rises by > 1000 K)."""
+    T0, p = 1200.0, 101325.0
+    Y0 = _mix(mech, 1.0)
+    y = mech.ignite(T0, p, Y0, 1e-3, rtol=1e-9, atol=1e-14)
+
+    def h_of(T, Y):
+        _, hrt, _ = mech.thermo(np.array([T]))
+        return float((Y * hrt[0] * OC.R_GAS * T / mech.W).sum())
+
+    h0, h1 = h_of(T0, Y0), h_of(y[0], y[1:])
+    assert abs(h1 - h0) < 1e-6 * max(abs(h0), 1e3 * 8314.0 / 29.0)
+    for e in "HON":
+        a = np.array([mech.comp[s].get(e, 0.0) / mech.W[k] for k, s in enumerate(mech.names)])
+        assert abs((a * y[1:]).sum() - (a * Y0).sum()) < 1e-9 * (a * Y0).sum()
+    assert y[0] > T0 + 1000.0
+    assert abs(y[1:].sum() - 1.0) < 1e-9
+
+
+IGN = os.path.join(os.path.dirname(__file__), "golden", "ignition_radau.npz")
+
+
+def test_ignition_fixture_matches_oracle(mech):
+    """The stored Radau reference (tests/golden/make_ignition.py) is what the
+    current restatement gives: case 1 recomputed at rtol 1e-9."""
+    z = np.load(IGN)
+    T0, p, phi = z["cases"][1]
+    y = mech.ignite(T0, p, _mix(mech, phi), 1e-3, rtol=1e-9, atol=1e-16)
+    assert abs(y[0] / z["y_end"][1, 0] - 1) < 1e-10
+    assert np.abs(y[1:] - z["y_end"][1, 1:]).max() < 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_reactor_converges_to_radau(mech):
+    """ROS2 on the GPU at rtol 1e-7 against scipy's Radau at 1e-10 (stored)
+    for lean, stoichiometric and rich mixtures at 1 and 5 atm, to t = 1 ms."""
+    z = np.load(IGN)
+    gm = AC.Mechanism(MECH)
+    cases = [tuple(c) for c in z["cases"]]
+    pool = AC.ReactorPool(gm, len(cases), rtol=1e-7, atol=1e-13)
+    for k, (T0, p, phi) in enumerate(cases):
+        pool.set(k, T0, p, _mix(mech, phi), 1e-3)
+    st = pool.run(max_tries=2000)
+    assert np.all(st == AC.ReactorPool.DONE), st
+    for k in range(len(cases)):
+        t, T, Y, nstep, nrej = pool.get(k)
+        assert t == 1e-3 and nstep > 10
+        ref = z["y_end"][k]
+        assert abs(T / ref[0] - 1) < 1e-4, (k, T, ref[0])
+        assert np.abs(Y - ref[1:]).max() < 1e-5, k
+
+
+@pytest.mark.gpu
+def test_gpu_reactor_same_alone_and_in_a_mix(mech):
+    """A reactor's result is bit-identical alone, among 63 others, and split
+    into launches of a different length."""
+    gm = AC.Mechanism(MECH)
+    rng = np.random.default_rng(4)
+    target = (1100.0, 3 * 101325.0, _mix(mech, 0.8))
+    alone = AC.ReactorPool(gm, 1, rtol=1e-6, atol=1e-12)
+    alone.set(0, target[0], target[1], target[2], 1e-3)
+    alone.run(max_tries=1000)
+    mixed = AC.ReactorPool(gm, 64, rtol=1e-6, atol=1e-12)
+    for k in range(64):
+        if k == 37:
+            mixed.set(k, target[0], target[1], target[2], 1e-3)
+        else:
+            mixed.set(k, rng.uniform(950, 1400), rng.uniform(1, 10) * 101325.0,
+                      _mix(mech, rng.uniform(0.5, 2.0)), 1e-3)
+    mixed.run(max_tries=37)
+    a, m = alone.get(0), mixed.get(37)
+    assert a[0] == m[0] and a[1] == m[1] and np.array_equal(a[2], m[2]) and a[3:] == m[3:]
