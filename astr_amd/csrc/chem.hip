@@ -733,7 +733,7 @@ int astr_hip_reactor_create(astr_chem* chem, int capacity, double rtol, double a
   r->P.nstep = iv + cap;
   r->P.nrej = iv + 2 * cap;
   r->P.cap = capacity;
-  if (hipMemset(r->base, 0, bytes) != hipSuccess || hipStreamCreate(&r->stream) != hipSuccess) {
+  if (hipMemset(r->base, 0, bytes) != hipSuccess || hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
     (void)hipFree(r->base);
     delete r;
     return cfail("reactor pool set-up failed");

@@ -1130,7 +1130,8 @@ int astr_hip_batch_create(astr_hip_ctx* const* cases, int ncases, astr_hip_batch
   if (hipSetDevice(b->device) != hipSuccess) rc = fail("hipSetDevice failed");
   if (!rc && hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess)
     rc = fail("device attribute query failed");
-  if (!rc && hipStreamCreate(&b->stream) != hipSuccess) rc = fail("hipStreamCreate failed");
+  if (!rc && hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = fail("hipStreamCreate failed");
   if (!rc && hipMalloc(&b->d_step, sizeof(int)) != hipSuccess) rc = fail("hipMalloc failed");
   if (!rc) rc = build_groups(b);
   if (!rc) rc = ensure_stats(b, 16);
