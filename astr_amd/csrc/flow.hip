@@ -242,8 +242,8 @@ __global__ __launch_bounds__(C::NT) void k_lines(const double* __restrict__ f, d
     T[l * C::LT + kLineOff - 5 + r] = (L0 + l < nlines) ? f[(L0 + l) * (n + 11) + r] : 0.0;
   }
   const int l = threadIdx.x / C::P, p = threadIdx.x % C::P;
-  ChunkCoef<C::R, !FILT> CC;
-  load_chunk_coef<C::R, !FILT>(CC, ac, n + (FILT ? 7 : 3), p * C::R);
+  ChunkCoef<C::R, FILT> CC;
+  load_chunk_coef<C::R, FILT>(CC, ac, n + (FILT ? 7 : 3), p * C::R);
   __syncthreads();
   solve_line<C::P, C::R, FILT>(T + l * C::LT, X + l * C::LT, n, p, CC, fc);
   __syncthreads();
@@ -841,7 +841,7 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
   HIP_OK(hipMemcpy(df, f, (size_t)nlines * (n + 11) * sizeof(double), hipMemcpyHostToDevice));
   const ShapeInfo& s = kShapes[sh];
   const long long nb = (nlines + s.TL - 1) / s.TL;
-  const size_t lds = (size_t)2 * s.TL * ((s.P * s.R + 11) | 1) * sizeof(double);
+  const size_t lds = (size_t)2 * s.TL * line_stride(s.P, s.R, s.TL) * sizeof(double);
   fc.ch = dt;
   const double* lu = dt + 33;
 #define LB(Cf)                                                                            \

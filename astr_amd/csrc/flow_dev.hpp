@@ -59,12 +59,38 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 
 template <class C>
 struct Slots {
-  int g[C::NPT];  // offset inside a field; the trash point for lines past the
-                  // end of the lattice (partial last tile)
+  unsigned g[C::NPT];  // byte offset inside a field; the trash point for lines
+                       // past the end of the lattice (partial last tile)
   int s[C::NPT];  // index of the point in a line buffer (l*LT + r)
   int h[C::NPT];  // line-buffer index of the slot's periodic halo copy
   int interior;   // bit t: i,j,k >= 1 (stats domain)
 };
+
+// FP64 load / store at a byte offset from a field's base pointer.  The base is
+// the same for every lane and the offset a 32-bit per-lane value, so the
+// access uses the scalar-base + vector-offset address mode and one offset
+// register per point serves every field.
+// The base passes through readfirstlane so the compiler keeps it a scalar
+// value and does not fold the field index into the per-lane address.
+template <class T>
+__device__ __forceinline__ T* uniform_ptr(T* p) {
+  const unsigned long long v = reinterpret_cast<unsigned long long>(p);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return reinterpret_cast<T*>(((unsigned long long)hi << 32) | lo);
+}
+// The accesses are made through global (address space 1) pointers so the
+// compiler emits global, not flat, instructions.
+typedef __attribute__((address_space(1))) const char gchar_c;
+typedef __attribute__((address_space(1))) char gchar;
+__device__ __forceinline__ double ldg(const double* base, unsigned off) {
+  gchar_c* b = (gchar_c*)uniform_ptr(base);
+  return *(__attribute__((address_space(1))) const double*)(b + off);
+}
+__device__ __forceinline__ void stg(double* base, unsigned off, double v) {
+  gchar* b = (gchar*)uniform_ptr(base);
+  *(__attribute__((address_space(1))) double*)(b + off) = v;
+}
 
 // LDS of a pass: the output lines X (TL x LT), then the input lines T
 // (TL x LT), then a write-only dummy cell.  A slot holding no point (past the
@@ -88,7 +114,7 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
 #pragma unroll
   for (int t = 0; t < C::NPT; t++) {
     const int e = threadIdx.x + t * C::NT;
-    S.g[t] = a.trash;
+    S.g[t] = (unsigned)a.trash * 8u;
     S.s[t] = dummy;
     S.h[t] = dummy;
     if (e < np) {
@@ -116,7 +142,7 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
       if (r >= n - 5 && r <= n - 1) S.h[t] = S.s[t] - n;
       if (r >= 1 && r <= 5) S.h[t] = S.s[t] + n;
       if (ok) {
-        S.g[t] = (int)((long long)k * a.plane + (long long)j * a.pitch + i);
+        S.g[t] = (unsigned)((long long)k * a.plane + (long long)j * a.pitch + i) * 8u;
         const bool in = i >= 1 && j >= 1 && (a.ndir == 2 || k >= 1);
         if (in) S.interior |= 1 << t;
       }
@@ -256,13 +282,13 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 
   if constexpr (PROG == GC_FIRST || PROG == GC_MID || PROG == GC_LAST) {
     constexpr bool LAST = PROG == GC_LAST;
-    ChunkCoef<C::R, true> CD;
-    load_chunk_coef<C::R, true>(CD, a.dac[dir], n + 3, p * C::R);
+    ChunkCoef<C::R, false> CD;
+    load_chunk_coef<C::R, false>(CD, a.dac[dir], n + 3, p * C::R);
     double qv[5][NPT];
 #pragma unroll
     for (int t = 0; t < NPT; t++)
 #pragma unroll
-      for (int m = 0; m < 5; m++) qv[m][t] = a.q[m * fs + S.g[t]];
+      for (int m = 0; m < 5; m++) qv[m][t] = ldg(a.q + m * fs, S.g[t]);
 
     // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
     double gz[4][NPT];
@@ -291,7 +317,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       for (int t = 0; t < NPT; t++) {
         {
           const double v = X[S.s[t]] * idn;
-          if (!LAST || sa.store_last_grad) a.dv[fld * fs + S.g[t]] = v;
+          if (!LAST || sa.store_last_grad) stg(a.dv + fld * fs, S.g[t], v);
           gz[c][t] = v;
         }
       }
@@ -303,17 +329,17 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       double sk = 0.0, se = 0.0;
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        const int o = S.g[t];
+        const unsigned o = S.g[t];
         double g[3][3], gt[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
           if (d == dir) {
             g[0][d] = gz[0][t]; g[1][d] = gz[1][t]; g[2][d] = gz[2][t]; gt[d] = gz[3][t];
           } else if (d < a.ndir) {
-            g[0][d] = a.dv[(0 + 3 * d) * fs + o];
-            g[1][d] = a.dv[(1 + 3 * d) * fs + o];
-            g[2][d] = a.dv[(2 + 3 * d) * fs + o];
-            gt[d] = a.dv[(9 + d) * fs + o];
+            g[0][d] = ldg(a.dv + (0 + 3 * d) * fs, o);
+            g[1][d] = ldg(a.dv + (1 + 3 * d) * fs, o);
+            g[2][d] = ldg(a.dv + (2 + 3 * d) * fs, o);
+            gt[d] = ldg(a.dv + (9 + d) * fs, o);
           } else {
             g[0][d] = 0.0; g[1][d] = 0.0; g[2][d] = 0.0; gt[d] = 0.0;
           }
@@ -322,7 +348,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         double f[9];
         visc_flux(a, g, gt, P, f);
 #pragma unroll
-        for (int m = 0; m < 9; m++) a.sg[m * fs + o] = f[m];
+        for (int m = 0; m < 9; m++) stg(a.sg + m * fs, o, f[m]);
         if (sa.dostats && ((S.interior >> t) & 1)) {
           sk += P.r * (P.u * P.u + P.v * P.v + P.w * P.w);
           const double o1 = g[2][1] - g[1][2];
@@ -369,7 +395,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       if (PROG != GC_FIRST) {
         // issued before the solve so the load latency overlaps it
 #pragma unroll
-        for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
+        for (int t = 0; t < NPT; t++) qr[t] = ldg(dst, S.g[t]);
       }
       __syncthreads();
       solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
@@ -378,97 +404,122 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       for (int t = 0; t < NPT; t++) {
         {
           const double v = X[S.s[t]] * idn;
-          if (PROG == GC_FIRST) dst[S.g[t]] = v;
-          else if (PROG == GC_MID) dst[S.g[t]] = qr[t] + v;
-          else dst[S.g[t]] = -(qr[t] + v);
+          if (PROG == GC_FIRST) stg(dst, S.g[t], v);
+          else if (PROG == GC_MID) stg(dst, S.g[t], qr[t] + v);
+          else stg(dst, S.g[t], -(qr[t] + v));
         }
       }
     }
   } else if constexpr (PROG == V_MID || PROG == V_LAST_RHS || PROG == V_LAST_RK) {
     // viscous-flux derivatives (diffusion, tgvsolver.F90:700-787); columns
-    // per direction: x (s11,s12,s13,q1), y (s12,s22,s23,q2), z (s13,s23,s33,q3)
-    ChunkCoef<C::R, true> CD;
-    load_chunk_coef<C::R, true>(CD, a.dac[dir], n + 3, p * C::R);
+    // per direction: x (s11,s12,s13,q1), y (s12,s22,s23,q2), z (s13,s23,s33,q3).
+    // V_LAST_RK: each variable's RK3 stage update (tgvsolver.F90:1043-1059)
+    // and its filter along dir follow its viscous sum at once, so one sum is
+    // live at a time; density, which has no viscous term, goes first.
+    constexpr bool RK = PROG == V_LAST_RK;
+    ChunkCoef<C::R, false> CD;
+    load_chunk_coef<C::R, false>(CD, a.dac[dir], n + 3, p * C::R);
     const int c0 = dir == 0 ? 0 : (dir == 1 ? 1 : 2);
     const int c1 = dir == 0 ? 1 : (dir == 1 ? 3 : 4);
     const int c2 = dir == 0 ? 2 : (dir == 1 ? 4 : 5);
-    double acc[4][NPT];
     double nx[NPT];
 #pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = a.sg[c0 * fs + S.g[t]];
+    for (int t = 0; t < NPT; t++) nx[t] = ldg(a.sg + c0 * fs, S.g[t]);
+    if constexpr (!RK) {
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-#pragma unroll
-      for (int t = 0; t < NPT; t++)
-        put<C>(T, S, t, nx[t]);
-      // loads for this column's accumulation and the next column's input are
-      // issued before the solve so their latency overlaps it
-      double qr[NPT];
-      double* dst = a.qrhs + (1 + c) * fs;
-#pragma unroll
-      for (int t = 0; t < NPT; t++) qr[t] = dst[S.g[t]];
-      if (c < 3) {
-        const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
-        const double* src = a.sg + col * fs;
-#pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = src[S.g[t]];
-      }
-      __syncthreads();
-      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
-      __syncthreads();
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-        {
-          const double v = qr[t] + X[S.s[t]] * idn;
-          if (PROG == V_LAST_RK) acc[c][t] = v;
-          else dst[S.g[t]] = v;
-        }
-      }
-    }
-    if constexpr (PROG == V_LAST_RK) {
-      // RK3 stage update (tgvsolver.F90:1043-1059), then the filter along dir
-      double qn[5][NPT];
-#pragma unroll
-      for (int t = 0; t < NPT; t++) {
-#pragma unroll
-        for (int m = 0; m < 5; m++) qn[m][t] = 0.0;
-        const int o = S.g[t];
-#pragma unroll
-        for (int m = 0; m < 5; m++) {
-          const double qr = m == 0 ? a.qrhs[o] : acc[m - 1][t];
-          const double qo = a.q[m * fs + o];
-          double qs;
-          if (sa.stage == 0) {
-            a.qsave[m * fs + o] = qo;
-            qs = qo;
-          } else {
-            qs = a.qsave[m * fs + o];
-          }
-          qn[m][t] = rk_update(sa, a.dt, qs, qo, qr);
-        }
-      }
-      ChunkCoef<C::R, false> CF;
-      load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
-#pragma unroll
-      for (int m = 0; m < 5; m++) {
+      for (int c = 0; c < 4; c++) {
 #pragma unroll
         for (int t = 0; t < NPT; t++)
-          put<C>(T, S, t, qn[m][t]);
+          put<C>(T, S, t, nx[t]);
+        // loads for this column's accumulation and the next column's input are
+        // issued before the solve so their latency overlaps it
+        double qr[NPT];
+        double* dst = a.qrhs + (1 + c) * fs;
+#pragma unroll
+        for (int t = 0; t < NPT; t++) qr[t] = ldg(dst, S.g[t]);
+        if (c < 3) {
+          const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
+#pragma unroll
+          for (int t = 0; t < NPT; t++) nx[t] = ldg(a.sg + col * fs, S.g[t]);
+        }
+        __syncthreads();
+        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NPT; t++) stg(dst, S.g[t], qr[t] + X[S.s[t]] * idn);
+      }
+    } else {
+      ChunkCoef<C::R, true> CF;
+      load_chunk_coef<C::R, true>(CF, a.fac[dir], n + 7, p * C::R);
+      // Loads run two solves ahead: variable c+1's qrhs, q, qsave are issued
+      // before variable c's filter solve, the next stress column before
+      // variable c's derivative solve.
+      double qr[NPT], qo[NPT], qs[NPT];
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        const unsigned o = S.g[t];
+        const double q0 = ldg(a.q, o);
+        double s0 = q0;
+        if (sa.stage == 0) stg(a.qsave, o, q0);
+        else s0 = ldg(a.qsave, o);
+        put<C>(T, S, t, rk_update(sa, a.dt, s0, q0, ldg(a.qrhs, o)));
+      }
+#pragma unroll
+      for (int t = 0; t < NPT; t++) {
+        qr[t] = ldg(a.qrhs + fs, S.g[t]);
+        qo[t] = ldg(a.q + fs, S.g[t]);
+        qs[t] = sa.stage == 0 ? 0.0 : ldg(a.qsave + fs, S.g[t]);
+      }
+      __syncthreads();
+      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
+      __syncthreads();
+#pragma unroll
+      for (int t = 0; t < NPT; t++) stg(a.q, S.g[t], X[S.s[t]]);
+#pragma unroll
+      for (int c = 0; c < 4; c++) {
+#pragma unroll
+        for (int t = 0; t < NPT; t++)
+          put<C>(T, S, t, nx[t]);
+        if (c < 3) {
+          const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
+#pragma unroll
+          for (int t = 0; t < NPT; t++) nx[t] = ldg(a.sg + col * fs, S.g[t]);
+        }
+        __syncthreads();
+        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NPT; t++) {
+          const double v = qr[t] + X[S.s[t]] * idn;
+          double s0 = qs[t];
+          if (sa.stage == 0) {
+            stg(a.qsave + (1 + c) * fs, S.g[t], qo[t]);
+            s0 = qo[t];
+          }
+          put<C>(T, S, t, rk_update(sa, a.dt, s0, qo[t], v));
+        }
+        if (c < 3) {
+#pragma unroll
+          for (int t = 0; t < NPT; t++) {
+            qr[t] = ldg(a.qrhs + (2 + c) * fs, S.g[t]);
+            qo[t] = ldg(a.q + (2 + c) * fs, S.g[t]);
+            qs[t] = sa.stage == 0 ? 0.0 : ldg(a.qsave + (2 + c) * fs, S.g[t]);
+          }
+        }
         __syncthreads();
         solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
         __syncthreads();
-        double* dst = a.q + m * fs;
+        double* dq = a.q + (1 + c) * fs;
 #pragma unroll
-        for (int t = 0; t < NPT; t++)
-          dst[S.g[t]] = X[S.s[t]];
+        for (int t = 0; t < NPT; t++) stg(dq, S.g[t], X[S.s[t]]);
       }
     }
   } else {  // F_ONLY: filter of q along dir, in place (filterq, tgvsolver.F90:791-883)
-    ChunkCoef<C::R, false> CF;
-    load_chunk_coef<C::R, false>(CF, a.fac[dir], n + 7, p * C::R);
+    ChunkCoef<C::R, true> CF;
+    load_chunk_coef<C::R, true>(CF, a.fac[dir], n + 7, p * C::R);
     double nx[NPT];
 #pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = a.q[S.g[t]];
+    for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q, S.g[t]);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
 #pragma unroll
@@ -476,7 +527,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         put<C>(T, S, t, nx[t]);
       if (m < 4) {
 #pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = a.q[(m + 1) * fs + S.g[t]];
+        for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q + (m + 1) * fs, S.g[t]);
       }
       __syncthreads();
       solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
@@ -484,7 +535,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       double* dst = a.q + m * fs;
 #pragma unroll
       for (int t = 0; t < NPT; t++)
-        dst[S.g[t]] = X[S.s[t]];
+        stg(dst, S.g[t], X[S.s[t]]);
     }
   }
 }

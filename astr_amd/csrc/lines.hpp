@@ -133,32 +133,47 @@ __device__ __forceinline__ double affine_scan(double& A, double& B) {
 // LU factors of one chunk, held in registers, with encodings that make the
 // recurrences branch-free: the first row, which the reference's solve leaves
 // as is, has c2=1, c3=0; the last row has c1=0; rows past the end have all
-// factors 0.  For rows with a = c = alpha the reference's factors
-// (commfunc.F90:752-774) satisfy ac1 = ac3 = alpha*ac2; the derivative's
-// first and last rows have a = c = 0, so there c1 = c3 (= 0) as well and one
-// array serves both (SAME13).
-template <int R, bool SAME13>
+// factors 0.  For rows with a = c the reference's factors (commfunc.F90:
+// 752-774) ac1 = c/den and ac3 = a/den are equal, so one array serves both
+// with two exceptions:
+//   - the first row's c3 is never used: it multiplies the zero carry that
+//     enters each line, and its product is never composed with anything to
+//     its left;
+//   - the filter's last row has a = beta_h, c = 0.  Its c3 is kept as one
+//     value, c3last, at chunk position klast (-1 if the chunk does not hold
+//     that row).
+// The derivative's last row has a = c = 0, so there no exception is needed.
+// FILT selects the filter's last-row exception.
+template <int R, bool FILT>
 struct ChunkCoef {
   double c2[R];
   double c1[R];
-  double c3[SAME13 ? 1 : R];
-  __device__ __forceinline__ double f3(int k) const { return SAME13 ? c1[k] : c3[k]; }
+  int klast = -1;
+  double c3last = 0.0;
+  __device__ __forceinline__ double f3(int k) const {
+    if constexpr (FILT) return k == klast ? c3last : c1[k];
+    else return c1[k];
+  }
 };
 
-template <int R, bool SAME13>
-__device__ __forceinline__ void load_chunk_coef(ChunkCoef<R, SAME13>& C,
+template <int R, bool FILT>
+__device__ __forceinline__ void load_chunk_coef(ChunkCoef<R, FILT>& C,
                                                 const double* __restrict__ ac, int M, int u0) {
 #pragma unroll
   for (int k = 0; k < R; k++) {
     const int u = u0 + k;
     const bool ok = u < M;
     const int uu = ok ? u : 0;
-    double a1 = ac[uu], a2 = ac[M + uu], a3 = ac[2 * M + uu];
-    if (u == 0) { a2 = 1.0; a3 = 0.0; }
+    double a1 = ac[uu], a2 = ac[M + uu];
+    if (u == 0) a2 = 1.0;
     if (u == M - 1) a1 = 0.0;
     C.c2[k] = ok ? a2 : 0.0;
     C.c1[k] = ok ? a1 : 0.0;
-    if constexpr (!SAME13) C.c3[k] = ok ? a3 : 0.0;
+  }
+  if constexpr (FILT) {
+    const int kl = M - 1 - u0;
+    C.klast = kl >= 0 && kl < R ? kl : -1;
+    C.c3last = ac[2 * M + M - 1];
   }
 }
 
@@ -172,7 +187,7 @@ __device__ __forceinline__ void load_chunk_coef(ChunkCoef<R, SAME13>& C,
 //             -3..-1, n+1..n+3 from coefh (filter.F90:206-283).
 template <int P, int R, bool FILT>
 __device__ __forceinline__ void solve_line(const double* __restrict__ T, double* __restrict__ X,
-                                           int n, int p, const ChunkCoef<R, !FILT>& C,
+                                           int n, int p, const ChunkCoef<R, FILT>& C,
                                            const FilterCoef& fc) {
   constexpr int OFF = FILT ? 3 : 1;
   constexpr int H = FILT ? 5 : 3;
@@ -294,6 +309,17 @@ __device__ __forceinline__ void solve_line(const double* __restrict__ T, double*
   }
 }
 
+// LDS stride of one line: at least rows -8..P*R+2, and = 32/TL (mod 32)
+// doubles, so that the y/z passes' point slots (TL adjacent lines x 32/TL
+// consecutive rows per half-wave) hit distinct banks; the solve's chunk reads
+// (stride R, odd) stay conflict-free for any stride.
+constexpr int line_stride(int P, int R, int TL) {
+  int L = P * R + 11;
+  const int want = (32 / TL) % 32;
+  while (L % 32 != want) L++;
+  return L;
+}
+
 // Launch shape of a line kernel.
 template <int NT_, int TL_, int R_, int NPT_>
 struct LineCfg {
@@ -302,9 +328,9 @@ struct LineCfg {
   static constexpr int P = NT_ / TL_;  // lanes (chunks) per line
   static constexpr int R = R_;     // unknowns per chunk
   static constexpr int NPT = NPT_; // point slots per thread
-  static constexpr int LT = (P * R_ + 11) | 1;  // input line stride in LDS (rows -8..P*R+2)
+  static constexpr int LT = line_stride(P, R_, TL_);  // line stride in LDS (rows -8..P*R+2)
   static_assert(P == 16 || P == 32 || P == 64, "a line's chunks must fill 16, 32 or 64 lanes");
-  static_assert((TL_ & (TL_ - 1)) == 0, "TL must be a power of two");
+  static_assert((TL_ & (TL_ - 1)) == 0 && TL_ <= 32, "TL must be a power of two, at most 32");
   static_assert(R_ >= 3, "chunk 0 must hold the three filter halo rows");
 };
 
