@@ -268,6 +268,34 @@ bool shape_fits(const ShapeInfo& s, int n) {
   return s.P * s.R >= n + 7 && s.NT * s.NPT >= s.TL * (n + 1) && n >= 8;
 }
 
+// LDS bytes of a pass in a shape (lds_doubles in flow_dev.hpp)
+size_t pass_lds_bytes(int shape, int /*prog*/) {
+  const ShapeInfo& s = kShapes[shape];
+  const long long tl = (long long)s.TL * line_stride(s.P, s.R, s.TL);
+  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8;
+  return (size_t)one * sizeof(double);
+}
+
+// per-workgroup LDS of a device, read once
+size_t lds_limit(int device) {
+  static std::mutex mu;
+  static size_t cache[64] = {0};
+  std::lock_guard<std::mutex> lk(mu);
+  if (device < 0 || device >= 64) return 65536;
+  if (!cache[device]) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || v <= 0)
+      v = 65536;
+    cache[device] = (size_t)v;
+  }
+  return cache[device];
+}
+
+// a selection a pass of line length n can launch with
+bool selection_ok(int shape, int prog, int n, int device) {
+  return shape_fits(kShapes[shape], n) && pass_lds_bytes(shape, prog) <= lds_limit(device);
+}
+
 // default: the fitting shape with the fewest idle slots+chunks
 int pick_shape(int n) {
   int best = -1;
@@ -476,7 +504,11 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
     const int sh = pick_shape(a.n[d]);
     if (sh < 0) { delete c; return fail("no launch shape fits this line length"); }
     for (int pr = 0; pr < 7; pr++) {
-      c->shape[pr][d] = sh;
+      int use = sh;
+      for (int k = 0; k < kNumShapes && !selection_ok(use, pr, a.n[d], c->device); k++)
+        use = k;
+      if (!selection_ok(use, pr, a.n[d], c->device)) { delete c; return fail("no launch shape fits the LDS"); }
+      c->shape[pr][d] = use;
       c->minb[pr][d] = default_minb(pr);
       c->tuned_ms[pr][d] = 0.f;
     }
@@ -764,7 +796,7 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
       if (!pass_used(pr, d, a.ndir)) continue;
       float best = 1e30f;
       for (int sh = 0; sh < kNumShapes && !rc; sh++) {
-        if (!shape_fits(kShapes[sh], a.n[d])) continue;
+        if (!selection_ok(sh, pr, a.n[d], c->device)) continue;
         for (int mb = 1; mb <= kMinbVariants && !rc; mb++) {
           hipError_t e = launch_pass(sh, pr, mb, a, d, c->stream);
           if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
@@ -810,8 +842,8 @@ int astr_hip_get_pass_config(astr_hip_ctx* c, int prog, int dir, int* shape, int
 int astr_hip_set_pass_config(astr_hip_ctx* c, int prog, int dir, int shape, int minb) {
   if (check_ctx(c)) return -1;
   if (prog < 0 || prog > 6 || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
-  if (shape < 0 || shape >= kNumShapes || !shape_fits(kShapes[shape], c->a.n[dir]))
-    return fail("launch shape does not fit this line length");
+  if (shape < 0 || shape >= kNumShapes || !selection_ok(shape, prog, c->a.n[dir], c->device))
+    return fail("launch shape does not fit this line length or the LDS");
   if (minb < 1 || minb > kMinbVariants) return fail("register budget must be 1 or 2");
   c->shape[prog][dir] = shape;
   c->minb[prog][dir] = minb;
@@ -1220,7 +1252,7 @@ int astr_hip_batch_autotune(astr_hip_batch* b, int reps) {
       if (prog < 0) continue;
       int mask = 0;
       for (int sh = 0; sh < kNumShapes; sh++)
-        if (shape_fits(kShapes[sh], c->a.n[d])) mask |= 1 << sh;
+        if (selection_ok(sh, prog, c->a.n[d], c->device)) mask |= 1 << sh;
       bool placed = false;
       for (auto& cl : classes)
         if (cl.first.first == prog && cl.first.second == mask) { cl.second.push_back(i); placed = true; }
