@@ -353,6 +353,230 @@ __global__ __launch_bounds__(64) void k_reactor(const Mech* __restrict__ M, Reac
 }
 
 // ---------------------------------------------------------------------------
+// The same reactor for a mechanism of exactly NS species, with every species
+// loop unrolled so the per-reactor state stays in registers (the generic
+// kernel's arrays, indexed at run time, live in scratch memory).  Species
+// picked by a reaction (uniform indices) are read and updated through select
+// chains; the Rosenbrock matrix of each thread sits in LDS, element (i,j) of
+// thread t at [(i*N + j)*64 + t].  Same method, same arithmetic order as
+// k_reactor.
+// ---------------------------------------------------------------------------
+template <int NS>
+__device__ __forceinline__ double pick(const double (&v)[NS], int idx) {
+  double r = v[0];
+#pragma unroll
+  for (int k = 1; k < NS; k++) r = idx == k ? v[k] : r;
+  return r;
+}
+
+template <int NS>
+__device__ __forceinline__ void add_at(double (&v)[NS], int idx, double x) {
+#pragma unroll
+  for (int k = 0; k < NS; k++)
+    if (idx == k) v[k] += x;
+}
+
+template <int NS>
+__device__ void reactor_rhs_n(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
+                              double (&f)[NS + 1]) {
+  const double T = yv[0];
+  double c[NS], g[NS], w[NS], cpr[NS], hrt[NS];
+  double winv = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; k++) winv += yv[1 + k] / M->W[k];
+  const double rho = p / (kRgas * T * winv);
+  const double logT = log(T), invT = 1.0 / T;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    c[k] = rho * yv[1 + k] / M->W[k];
+    const double* a = T <= M->tmid[k] ? M->lo[k] : M->hi[k];
+    cpr[k] = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
+    hrt[k] = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 + a[4] * T4 / 5.0 +
+             a[5] * invT;
+    const double sr = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 +
+                      a[4] * T4 / 4.0 + a[6];
+    g[k] = hrt[k] - sr;
+    w[k] = 0.0;
+  }
+  const double lc0 = log(M->p0 / (kRgas * T));
+  for (int r = 0; r < M->nr; r++) {
+    const double kf = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT);
+    double fwd = kf;
+    double dg = 0.0;
+    for (int j = 0; j < M->nre[r]; j++) {
+      const int sp = M->rsp[r][j];
+      fwd *= conc_pow(pick<NS>(c, sp), M->rnu[r][j]);
+      dg -= M->rnu[r][j] * pick<NS>(g, sp);
+    }
+    double rv = 0.0;
+    if (M->rev[r]) {
+      double pp = 1.0;
+      for (int j = 0; j < M->npr[r]; j++) {
+        const int sp = M->psp[r][j];
+        pp *= conc_pow(pick<NS>(c, sp), M->pnu[r][j]);
+        dg += M->pnu[r][j] * pick<NS>(g, sp);
+      }
+      const double kr = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT + dg - M->dnu[r] * lc0);
+      rv = kr * pp;
+    }
+    double q = fwd - rv;
+    if (M->tb[r]) {
+      double cm = 0.0;
+#pragma unroll
+      for (int k = 0; k < NS; k++) cm += M->eff[r][k] * c[k];
+      q *= cm;
+    }
+    for (int j = 0; j < M->nre[r]; j++) add_at<NS>(w, M->rsp[r][j], -M->rnu[r][j] * q);
+    for (int j = 0; j < M->npr[r]; j++) add_at<NS>(w, M->psp[r][j], M->pnu[r][j] * q);
+  }
+  double cp = 0.0, dh = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    cp += yv[1 + k] * cpr[k] / M->W[k];
+    const double dY = w[k] * M->W[k] / rho;
+    f[1 + k] = dY;
+    dh += hrt[k] * T / M->W[k] * dY;
+  }
+  f[0] = -dh / cp;
+}
+
+// solve with the LU factors of the per-thread LDS matrix (stride 64)
+template <int N>
+__device__ __forceinline__ void lu_solve_n(const double* a, const int (&piv)[N], double (&b)[N]) {
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    // apply the row swap of step k
+    const double bk = b[k];
+    double bp = bk;
+#pragma unroll
+    for (int r = k + 1; r < N; r++) bp = piv[k] == r ? b[r] : bp;
+#pragma unroll
+    for (int r = k + 1; r < N; r++)
+      if (piv[k] == r) b[r] = bk;
+    b[k] = bp;
+#pragma unroll
+    for (int r = k + 1; r < N; r++) b[r] -= a[(r * N + k) * 64] * b[k];
+  }
+#pragma unroll
+  for (int k = N - 1; k >= 0; k--) {
+    double sum = b[k];
+#pragma unroll
+    for (int c = k + 1; c < N; c++) sum -= a[(k * N + c) * 64] * b[c];
+    b[k] = sum / a[(k * N + k) * 64];
+  }
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, ReactorPool P,
+                                                  int max_tries, double rtol, double atol) {
+  constexpr int N = NS + 1;
+  __shared__ double sA[N * N * 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= P.cap || P.status[i] != RX_RUN) return;
+  double* a = sA + threadIdx.x;
+#define AIJ(r, c) a[((r) * N + (c)) * 64]
+  const double gam = 1.0 + 0.70710678118654752440;
+  double y[N], f0[N], f1[N], k1[N], k2[N], yt[N];
+  int piv[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) y[k] = P.y[(size_t)k * P.cap + i];
+  double t = P.t[i], h = P.h[i];
+  const double tend = P.tend[i], p = P.p[i];
+  int nstep = P.nstep[i], nrej = P.nrej[i], st = RX_RUN;
+  for (int it = 0; it < max_tries; it++) {
+    if (t >= tend) { st = RX_DONE; break; }
+    if (h > tend - t) h = tend - t;
+    if (!(h > 1e-18)) { st = RX_FAIL; break; }
+    reactor_rhs_n<NS>(M, p, y, f0);
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+#pragma unroll
+      for (int k = 0; k < N; k++) yt[k] = y[k];
+      const double d = 1.4901161193847656e-08 * fmax(fabs(y[j]), j == 0 ? 1.0 : 1e-8);
+      yt[j] = y[j] + d;
+      reactor_rhs_n<NS>(M, p, yt, f1);
+#pragma unroll
+      for (int k = 0; k < N; k++) AIJ(k, j) = -gam * h * (f1[k] - f0[k]) / d;
+    }
+#pragma unroll
+    for (int k = 0; k < N; k++) AIJ(k, k) += 1.0;
+    // LU with partial pivoting (rows swapped in LDS)
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      int pk = k;
+      double m = fabs(AIJ(k, k));
+#pragma unroll
+      for (int r = k + 1; r < N; r++) {
+        const double v = fabs(AIJ(r, k));
+        if (v > m) { m = v; pk = r; }
+      }
+      piv[k] = pk;
+      ok = ok && (m > 0.0);
+      if (pk != k) {
+        for (int c = 0; c < N; c++) {
+          const double tmp = AIJ(k, c);
+          AIJ(k, c) = AIJ(pk, c);
+          AIJ(pk, c) = tmp;
+        }
+      }
+      const double inv = 1.0 / AIJ(k, k);
+#pragma unroll
+      for (int r = k + 1; r < N; r++) {
+        const double l = AIJ(r, k) * inv;
+        AIJ(r, k) = l;
+#pragma unroll
+        for (int c = k + 1; c < N; c++) AIJ(r, c) -= l * AIJ(k, c);
+      }
+    }
+    double err = 2.0;
+    if (ok) {
+      // k1 = A^-1 f0, k2 = A^-1 (f(y + h k1) - 2 k1)
+#pragma unroll
+      for (int k = 0; k < N; k++) k1[k] = f0[k];
+      lu_solve_n<N>(a, piv, k1);
+#pragma unroll
+      for (int k = 0; k < N; k++) yt[k] = y[k] + h * k1[k];
+      reactor_rhs_n<NS>(M, p, yt, f1);
+#pragma unroll
+      for (int k = 0; k < N; k++) k2[k] = f1[k] - 2.0 * k1[k];
+      lu_solve_n<N>(a, piv, k2);
+      err = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        const double yn = y[k] + 1.5 * h * k1[k] + 0.5 * h * k2[k];
+        const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
+        const double e = fabs(0.5 * h * (k1[k] + k2[k])) / sc;
+        if (!(e <= err)) err = e;
+        yt[k] = yn;
+      }
+    }
+    if (err <= 1.0) {
+      t += h;
+#pragma unroll
+      for (int k = 0; k < N; k++) y[k] = yt[k];
+      nstep++;
+      const double fac = 0.9 / sqrt(fmax(err, 1e-10));
+      h *= fmin(5.0, fmax(0.2, fac));
+    } else {
+      nrej++;
+      const double fac = err < 1e300 ? 0.9 / sqrt(err) : 0.2;
+      h *= fmax(0.2, fmin(0.9, fac));
+    }
+    if (t >= tend) { st = RX_DONE; break; }
+  }
+#undef AIJ
+#pragma unroll
+  for (int k = 0; k < N; k++) P.y[(size_t)k * P.cap + i] = y[k];
+  P.t[i] = t;
+  P.h[i] = h;
+  P.nstep[i] = nstep;
+  P.nrej[i] = nrej;
+  P.status[i] = st;
+}
+
+// ---------------------------------------------------------------------------
 // CTML reading (the subset the reference's mechanisms use)
 // ---------------------------------------------------------------------------
 std::string trim(const std::string& s) {
@@ -782,8 +1006,13 @@ int astr_hip_reactor_advance(astr_reactor_pool* r, int max_tries) {
   if (max_tries < 1) return cfail("max_tries must be positive");
   CHIP_OK(hipSetDevice(r->chem->device));
   const int bs = 64;
-  hipLaunchKernelGGL(k_reactor, dim3((r->P.cap + bs - 1) / bs), dim3(bs), 0, r->stream, r->chem->dev,
-                     r->P, max_tries, r->rtol, r->atol);
+  const dim3 grid((r->P.cap + bs - 1) / bs);
+  if (r->chem->host.ns == 9)
+    hipLaunchKernelGGL(k_reactor_n<9>, grid, dim3(bs), 0, r->stream, r->chem->dev, r->P, max_tries,
+                       r->rtol, r->atol);
+  else
+    hipLaunchKernelGGL(k_reactor, grid, dim3(bs), 0, r->stream, r->chem->dev, r->P, max_tries,
+                       r->rtol, r->atol);
   CHIP_OK(hipGetLastError());
   return 0;
 }

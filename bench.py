@@ -135,6 +135,144 @@ def copy_probe(nbytes, reps=3):
             "GBs_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
 
 
+def batch_form(ncases=64, steps=100, seed=2026):
+    """Batching form: 64 independent periodic cases drawn from a seed, half
+    2-D isentropic vortex (grid side U{64..512}, Ma 0.5, strength 5), half 3-D
+    TGV (side U{32..128}, Re U[100,3000]), 100 RK3 steps each, FP64.  Batched:
+    one batch (autotuned launch selection, one hipGraph per step).  Sequential:
+    each case alone through astr_hip_rk3_steps (own autotuned selection).
+    Set-up and tuning are outside both timings."""
+    import astr_amd as A
+
+    rng = np.random.default_rng(seed)
+    specs = []
+    for k in range(ncases):
+        if k % 2 == 0:
+            n = int(rng.integers(64, 513))
+            specs.append(("vortex", A.vortex_config(n), None))
+        else:
+            n = int(rng.integers(32, 129))
+            re = float(rng.uniform(100.0, 3000.0))
+            specs.append(("tgv", A.tgv_config(n, deltat=0.128 / n, reynolds=re), int(rng.integers(1 << 30))))
+
+    def make():
+        flows = []
+        for kind, cfg, sd in specs:
+            f = A.Flow(cfg)
+            if kind == "vortex":
+                f.init_vortex(5.0)
+            else:
+                f.init_tgv(1e-3, sd)
+            flows.append(f)
+        return flows
+
+    flows = make()
+    b = A.Batch(flows)
+    b.autotune(reps=2)
+    b.rk3(1, graph=True)  # warm-up step (captures the graph)
+    for f, (kind, cfg, sd) in zip(flows, specs):  # same start for both paths
+        if kind == "vortex":
+            f.init_vortex(5.0)
+        else:
+            f.init_tgv(1e-3, sd)
+    A.lib().astr_hip_sync(flows[0].h)
+    t0 = time.perf_counter()
+    b.rk3(steps, graph=True, stats=False)
+    t_batch = time.perf_counter() - t0
+    info = b.info()
+    q_batch = [f.download() for f in flows]
+    b.close()
+    for f in flows:
+        f.close()
+
+    seq = make()
+    for f in seq:
+        f.autotune(reps=2)
+        f.rk3(1, stats=False)
+    for f, (kind, cfg, sd) in zip(seq, specs):
+        if kind == "vortex":
+            f.init_vortex(5.0)
+        else:
+            f.init_tgv(1e-3, sd)
+        f.sync()
+    t0 = time.perf_counter()
+    for f in seq:
+        f.rk3(steps, stats=False)
+        f.sync()
+    t_seq = time.perf_counter() - t0
+    diff = 0.0
+    for f, qb in zip(seq, q_batch):
+        qs = f.download()
+        diff = max(diff, float(np.abs(qb - qs).max() / np.abs(qs).max()))
+        f.close()
+    pts = sum((c.im + 1) * (c.jm + 1) * (c.km + 1) for _, c, _ in specs)
+    return {"cases": ncases, "steps_per_case": steps,
+            "workload": "32 x 2-D isentropic vortex (side U{64..512}) + 32 x 3-D TGV (side U{32..128}, "
+                        "Re U[100,3000]), FP64 RK3, compact 643c + filter",
+            "lattice_points": pts,
+            "batched": {"s": round(t_batch, 4), "cases_per_s": round(ncases / t_batch, 3),
+                        "Mpt_updates_per_s": round(pts * 3 * steps / t_batch / 1e6, 1),
+                        "launch_groups": info["groups"], "launches_per_step": info["launches_per_step"],
+                        "graph": True},
+            "sequential": {"s": round(t_seq, 4), "cases_per_s": round(ncases / t_seq, 3)},
+            "speedup": round(t_seq / t_batch, 3),
+            "max_rel_diff_batched_vs_sequential": diff}
+
+
+def serve_form(rates=(10.0, 100.0, 1000.0), counts=(40, 150, 300), seed=7, rtol=1e-5, atol=1e-10):
+    """Serving form: Poisson request streams, 70% 0-D constant-pressure H2-air
+    ignition to 1 ms (ROS2 at rtol/atol), 30% 2-D vortex (side U{64..256},
+    200 steps), under a 200 GB HBM budget.  Latency = completion - arrival.
+    The sequential reference serves a sample one request at a time through
+    the plain path; its cases/s is 1 / mean service time."""
+    from astr_amd import serve as S
+
+    mech = os.path.join(ROOT, "tests", "golden", "Jachimowski88_H2-9-19.xml")
+    out = {"workload": "70% 0-D ignition (T0 U[950,1400] K, p U[1,10] atm, phi U[0.5,2], t_end 1 ms), "
+                       "30% 2-D vortex (side U{64..256}, 200 steps); HBM budget 200 GB",
+           "reactor_tolerance": {"rtol": rtol, "atol": atol}, "rates": []}
+    for rate, count in zip(rates, counts):
+        reqs = S.poisson_requests(count, rate, seed + int(rate))
+        srv = S.Server(mech, hbm_budget=200e9, reactor_slots=4096, rtol=rtol, atol=atol,
+                       steps_per_tick=10, reactor_tries=100)
+        res = srv.run(reqs)
+        srv.close()
+        st = S.latency_stats(res)
+        ign = [r.latency for r in res if r.kind == "ignition"]
+        vor = [r.latency for r in res if r.kind == "vortex"]
+        out["rates"].append({"rate_req_per_s": rate, "requests": count,
+                             "cases_per_s": round(st["cases_per_s"], 3),
+                             "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
+                             "p50_ignition_s": round(float(np.percentile(ign, 50)), 4) if ign else None,
+                             "p50_vortex_s": round(float(np.percentile(vor, 50)), 4) if vor else None,
+                             "max_in_flight": srv.max_active})
+    # sequential plain path on a sample; equality of results with the served runs
+    sample = S.poisson_requests(6, 1.0, seed + 99)
+    t_serv = []
+    srv = S.Server(mech, reactor_slots=64, rtol=rtol, atol=atol, steps_per_tick=10, reactor_tries=100)
+    for r in sample:
+        r.arrival = 0.0
+    served = {r.rid: r for r in srv.run(sample)}
+    srv.close()
+    same = True
+    for r in sample:
+        t0 = time.perf_counter()
+        alone = S.run_alone(r, mech, rtol=rtol, atol=atol, reactor_tries=100)
+        t_serv.append(time.perf_counter() - t0)
+        got = served[r.rid].data
+        if r.__class__.__name__ == "Ignition":
+            same = same and got["T"] == alone["T"] and bool(np.array_equal(got["Y"], alone["Y"]))
+        else:
+            same = same and float(np.abs(got["q"] - alone["q"]).max() / np.abs(alone["q"]).max()) < 1e-12
+    mean = float(np.mean(t_serv))
+    out["sequential"] = {"sample": len(sample), "mean_service_s": round(mean, 4),
+                         "cases_per_s": round(1.0 / mean, 3)}
+    best = max(x["cases_per_s"] for x in out["rates"])
+    out["speedup_at_best_rate"] = round(best * mean, 3)
+    out["served_equals_alone"] = bool(same)
+    return out
+
+
 def load_traffic(n):
     path = os.path.join(ROOT, "profiles", f"traffic_tgv{n}.json")
     if os.path.exists(path):
@@ -151,6 +289,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chem", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving-form measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch selection instead of timing the candidates")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -238,6 +378,10 @@ def main():
     }
     if not args.no_chem:
         out["chem_batch"] = chem_batch()
+    if not args.no_batch:
+        out["batching"] = batch_form()
+    if not args.no_serve:
+        out["serving"] = serve_form()
     if args.copy_probe:
         out["copy_probe"] = copy_probe(args.copy_probe)
     if not args.no_cpu_baseline:
