@@ -50,12 +50,13 @@ EXPORTS = [
     "astr_hip_profile_begin", "astr_hip_profile_end", "astr_hip_autotune",
     "astr_hip_get_pass_config", "astr_hip_set_pass_config", "astr_hip_init_vortex",
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
-    "astr_hip_batch_rk3_steps", "astr_hip_batch_info",
+    "astr_hip_batch_rk3_steps", "astr_hip_batch_info", "astr_hip_set_last_split",
+    "astr_hip_get_last_split",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5)}
-PROF_SLOTS = 21
+PROF_SLOTS = 27
 PASS_NAMES = ["gc_first", "gc_mid", "gc_last", "visc_mid", "visc_last_rhs",
-              "visc_last_rk3_filter", "filter"]
+              "visc_last_rk3_filter", "filter", "gc_last_grad", "gc_last_conv"]
 
 _lib = None
 
@@ -90,6 +91,8 @@ def lib():
                                                C.POINTER(C.c_int), C.POINTER(C.c_float)]
         L.astr_hip_set_pass_config.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]
         L.astr_hip_init_vortex.argtypes = [C.c_void_p] + [C.c_double] * 5
+        L.astr_hip_set_last_split.argtypes = [C.c_void_p, C.c_int]
+        L.astr_hip_get_last_split.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.astr_hip_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
         L.astr_hip_batch_destroy.argtypes = [C.c_void_p]
         L.astr_hip_batch_autotune.argtypes = [C.c_void_p, C.c_int]
@@ -222,7 +225,7 @@ class Flow:
         """{(pass_name, dir): (shape, minb, tuned_ms)} of the passes this case launches."""
         nd = 3 if self.cfg.km > 0 else 2
         used = {0: [0], 1: list(range(1, nd - 1)), 2: [nd - 1], 3: list(range(nd - 1)),
-                4: [nd - 1], 5: [nd - 1], 6: list(range(nd))}
+                4: [nd - 1], 5: [nd - 1], 6: list(range(nd)), 7: [nd - 1], 8: [nd - 1]}
         out = {}
         for prog, dirs in used.items():
             for d in dirs:
@@ -234,6 +237,16 @@ class Flow:
 
     def set_pass_config(self, prog, d, shape, minb):
         _chk(lib().astr_hip_set_pass_config(self.h, prog, d, shape, minb))
+
+    @property
+    def split_last(self):
+        v = C.c_int()
+        _chk(lib().astr_hip_get_last_split(self.h, C.byref(v)))
+        return bool(v.value)
+
+    @split_last.setter
+    def split_last(self, on):
+        _chk(lib().astr_hip_set_last_split(self.h, 1 if on else 0))
 
 
 def vortex_config(im, jm=None, mach=0.5, reynolds=1.0e5, length=10.0, cfl=0.67, deltat=None):

@@ -322,7 +322,7 @@ hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir
 
 // register budget a pass uses until tuned: two workgroups per CU for the
 // gradient/convective passes, one for the others
-int default_minb(int prog) { return prog <= GC_LAST ? 2 : 1; }
+int default_minb(int prog) { return prog <= GC_LAST || prog >= GL_GRAD ? 2 : 1; }
 
 }  // namespace
 
@@ -334,9 +334,12 @@ struct astr_hip_ctx {
   FlowArgs a;
   // launch selection per (pass program, direction): shape index and register
   // budget MINB; ms = the autotuner's measured time per launch (0: untuned)
-  int shape[7][3];
-  int minb[7][3];
-  float tuned_ms[7][3];
+  int shape[kNumProgs][3];
+  int minb[kNumProgs][3];
+  float tuned_ms[kNumProgs][3];
+  // the last direction's gradient/convective work in two kernels
+  // (GL_GRAD + GL_CONV) instead of GC_LAST; chosen by the autotuner
+  int split_last = 0;
   double* base = nullptr;   // one allocation for all fields
   double* coef = nullptr;   // LU tables
   double* stats_part = nullptr;
@@ -447,6 +450,10 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad) {
   a.dostats = stats ? 1 : 0;
   a.store_last_grad = store_last_grad ? 1 : 0;
   for (int d = 0; d < nd; d++) {
+    if (d == nd - 1 && c->split_last) {
+      if (pass(c, GL_GRAD, d) || pass(c, GL_CONV, d)) return -1;
+      continue;
+    }
     const int prog = d == nd - 1 ? GC_LAST : (d == 0 ? GC_FIRST : GC_MID);
     if (pass(c, prog, d)) return -1;
   }
@@ -503,7 +510,7 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   for (int d = 0; d < a.ndir; d++) {
     const int sh = pick_shape(a.n[d]);
     if (sh < 0) { delete c; return fail("no launch shape fits this line length"); }
-    for (int pr = 0; pr < 7; pr++) {
+    for (int pr = 0; pr < kNumProgs; pr++) {
       int use = sh;
       for (int k = 0; k < kNumShapes && !selection_ok(use, pr, a.n[d], c->device); k++)
         use = k;
@@ -682,7 +689,8 @@ int astr_hip_rk3_steps(astr_hip_ctx* c, int nsteps, double* ke, double* ens) {
     c->stats_cap = nsteps;
   }
   FlowArgs& a = c->a;
-  const int ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[GC_LAST][a.ndir - 1]].TL);
+  const int stats_prog = c->split_last ? GL_GRAD : GC_LAST;
+  const int ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[stats_prog][a.ndir - 1]].TL);
   for (int it = 0; it < nsteps; it++) {
     for (int s = 0; s < 3; s++) {
       a.stage = s;
@@ -765,7 +773,9 @@ static bool pass_used(int prog, int d, int nd) {
   switch (prog) {
     case GC_FIRST: return d == 0;
     case GC_MID: return d > 0 && d < nd - 1;
-    case GC_LAST: return d == nd - 1;
+    case GC_LAST:
+    case GL_GRAD:
+    case GL_CONV: return d == nd - 1;
     case V_MID: return d < nd - 1;
     case V_LAST_RHS:
     case V_LAST_RK: return d == nd - 1;
@@ -791,7 +801,7 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   a.dostats = 0;
   a.store_last_grad = 0;
   int rc = 0;
-  for (int pr = 0; pr < 7 && !rc; pr++) {
+  for (int pr = 0; pr < kNumProgs && !rc; pr++) {
     for (int d = 0; d < a.ndir && !rc; d++) {
       if (!pass_used(pr, d, a.ndir)) continue;
       float best = 1e30f;
@@ -820,6 +830,10 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
       }
     }
   }
+  if (!rc) {
+    const int dl = a.ndir - 1;
+    c->split_last = c->tuned_ms[GL_GRAD][dl] + c->tuned_ms[GL_CONV][dl] < c->tuned_ms[GC_LAST][dl];
+  }
   a = keep;
   hipError_t e = hipMemcpyAsync(a.q, snap, qbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
@@ -832,7 +846,7 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
 
 int astr_hip_get_pass_config(astr_hip_ctx* c, int prog, int dir, int* shape, int* minb, float* ms) {
   if (check_ctx(c)) return -1;
-  if (prog < 0 || prog > 6 || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
+  if (prog < 0 || prog >= kNumProgs || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
   if (shape) *shape = c->shape[prog][dir];
   if (minb) *minb = c->minb[prog][dir];
   if (ms) *ms = c->tuned_ms[prog][dir];
@@ -841,13 +855,25 @@ int astr_hip_get_pass_config(astr_hip_ctx* c, int prog, int dir, int* shape, int
 
 int astr_hip_set_pass_config(astr_hip_ctx* c, int prog, int dir, int shape, int minb) {
   if (check_ctx(c)) return -1;
-  if (prog < 0 || prog > 6 || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
+  if (prog < 0 || prog >= kNumProgs || dir < 0 || dir >= c->a.ndir) return fail("bad pass or direction");
   if (shape < 0 || shape >= kNumShapes || !selection_ok(shape, prog, c->a.n[dir], c->device))
     return fail("launch shape does not fit this line length or the LDS");
   if (minb < 1 || minb > kMinbVariants) return fail("register budget must be 1 or 2");
   c->shape[prog][dir] = shape;
   c->minb[prog][dir] = minb;
   c->tuned_ms[prog][dir] = 0.f;
+  return 0;
+}
+
+int astr_hip_set_last_split(astr_hip_ctx* c, int split) {
+  if (check_ctx(c)) return -1;
+  c->split_last = split ? 1 : 0;
+  return 0;
+}
+
+int astr_hip_get_last_split(astr_hip_ctx* c, int* split) {
+  if (check_ctx(c) || !split) return fail("bad argument");
+  *split = c->split_last;
   return 0;
 }
 

@@ -40,7 +40,10 @@ struct FlowArgs {
   FilterCoef fc;
 };
 
-enum Prog { GC_FIRST = 0, GC_MID, GC_LAST, V_MID, V_LAST_RHS, V_LAST_RK, F_ONLY };
+// Pass programs.  GL_GRAD + GL_CONV are GC_LAST split in two kernels (the
+// gradient/stress half and the convective half), each with fewer registers.
+enum Prog { GC_FIRST = 0, GC_MID, GC_LAST, V_MID, V_LAST_RHS, V_LAST_RK, F_ONLY, GL_GRAD, GL_CONV };
+constexpr int kNumProgs = 9;
 
 // ntiles of direction dir for lines-per-tile TL
 __host__ __device__ inline int tiles_of(const int n[3], int dir, int TL) {
@@ -280,8 +283,11 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
   const double idn = a.inv_d[dir];
   const long long fs = a.fs;
 
-  if constexpr (PROG == GC_FIRST || PROG == GC_MID || PROG == GC_LAST) {
-    constexpr bool LAST = PROG == GC_LAST;
+  if constexpr (PROG == GC_FIRST || PROG == GC_MID || PROG == GC_LAST || PROG == GL_GRAD ||
+                PROG == GL_CONV) {
+    constexpr bool LAST = PROG == GC_LAST || PROG == GL_GRAD || PROG == GL_CONV;
+    constexpr bool DO_GRAD = PROG != GL_CONV;
+    constexpr bool DO_CONV = PROG != GL_GRAD;
     ChunkCoef<C::R, false> CD;
     load_chunk_coef<C::R, false>(CD, a.dac[dir], n + 3, p * C::R);
     double qv[5][NPT];
@@ -290,6 +296,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
       for (int m = 0; m < 5; m++) qv[m][t] = ldg(a.q + m * fs, S.g[t]);
 
+    if constexpr (DO_GRAD) {
     // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
     double gz[4][NPT];
 #pragma unroll
@@ -377,7 +384,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         __syncthreads();
       }
     }
+    }
 
+    if constexpr (DO_CONV) {
     // convective flux derivatives along dir (convection, tgvsolver.F90:539-633)
     double ud[NPT], pp[NPT];
 #pragma unroll
@@ -409,6 +418,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           else stg(dst, S.g[t], -(qr[t] + v));
         }
       }
+    }
     }
   } else if constexpr (PROG == V_MID || PROG == V_LAST_RHS || PROG == V_LAST_RK) {
     // viscous-flux derivatives (diffusion, tgvsolver.F90:700-787); columns
@@ -588,6 +598,8 @@ hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageA
     case V_MID: hipLaunchKernelGGL((k_pass_batch<C, V_MID, MINB>), g, b, lds, st, bl, dir, sa); break;
     case V_LAST_RHS: hipLaunchKernelGGL((k_pass_batch<C, V_LAST_RHS, MINB>), g, b, lds, st, bl, dir, sa); break;
     case V_LAST_RK: hipLaunchKernelGGL((k_pass_batch<C, V_LAST_RK, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case GL_GRAD: hipLaunchKernelGGL((k_pass_batch<C, GL_GRAD, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case GL_CONV: hipLaunchKernelGGL((k_pass_batch<C, GL_CONV, MINB>), g, b, lds, st, bl, dir, sa); break;
     default: hipLaunchKernelGGL((k_pass_batch<C, F_ONLY, MINB>), g, b, lds, st, bl, dir, sa); break;
   }
   return hipGetLastError();
@@ -605,6 +617,8 @@ hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
     case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID, MINB>), g, b, lds, st, a, dir); break;
     case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS, MINB>), g, b, lds, st, a, dir); break;
     case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB>), g, b, lds, st, a, dir); break;
+    case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB>), g, b, lds, st, a, dir); break;
+    case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB>), g, b, lds, st, a, dir); break;
     default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB>), g, b, lds, st, a, dir); break;
   }
   return hipGetLastError();

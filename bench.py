@@ -40,8 +40,12 @@ HBM_PEAK_GBS = 8000.0
 # Summed over a 3-D stage: 20+25+50+12+12+42+10+10 = 181, plus the
 # reference's updatefvar (5+6 = 11) that the passes make unnecessary:
 # 192 words = 1536 B per point per stage.
+# When the last gradient/convective pass runs split, its halves carry the
+# gradients + stress/heat flux (4+4 + 16+9 = 33) and the convective
+# derivatives (12+5 = 17) of the same 50 words.
 PASS_WORDS = {"gc_first": 20, "gc_mid": 25, "gc_last": 50, "visc_mid": 12,
-              "visc_last_rhs": 12, "visc_last_rk3_filter": 42, "filter": 10}
+              "visc_last_rhs": 12, "visc_last_rk3_filter": 42, "filter": 10,
+              "gc_last_grad": 33, "gc_last_conv": 17}
 STAGE_BYTES = 1536
 
 
@@ -314,6 +318,7 @@ def main():
         tune_s = time.perf_counter() - t0
     launch = {f"{k[0]}_{k[1]}": {"shape": A.SHAPES[v[0]], "minb": v[1]}
               for k, v in f.pass_configs().items()}
+    launch["last_pass_split"] = f.split_last
     if args.warmup:
         f.rk3(args.warmup, stats=True)
     f.sync()

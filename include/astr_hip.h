@@ -102,9 +102,11 @@ int astr_hip_sync(astr_hip_ctx *ctx);
 
 /* Per-pass timing with HIP events on the context's stream (no reference
  * counterpart; the reference's timerept accumulators, src/utility.F90:27-297,
- * are no-ops).  Slot = pass*3 + direction, pass 0..6 = convective+gradient
- * first/middle/last, viscous middle/last(rhs)/last(RK3+filter), filter. */
-#define ASTR_PROF_SLOTS 21
+ * are no-ops).  Slot = pass*3 + direction, pass 0..8 = convective+gradient
+ * first/middle/last, viscous middle/last(rhs)/last(RK3+filter), filter, and
+ * the two halves of the split last convective+gradient pass (gradients and
+ * stress, convective). */
+#define ASTR_PROF_SLOTS 27
 int astr_hip_profile_begin(astr_hip_ctx *ctx, int max_launches);
 int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
 
@@ -119,6 +121,11 @@ int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
  * each, on the context's stream, state restored afterwards) and keeps the
  * fastest; get/set read and override single selections. */
 int astr_hip_autotune(astr_hip_ctx *ctx, int reps);
+/* The last direction's gradient/convective work as one pass (split = 0) or as
+ * two lighter passes (split = 1, programs 7 and 8); astr_hip_autotune sets
+ * it from the timings. */
+int astr_hip_set_last_split(astr_hip_ctx *ctx, int split);
+int astr_hip_get_last_split(astr_hip_ctx *ctx, int *split);
 int astr_hip_get_pass_config(astr_hip_ctx *ctx, int prog, int dir, int *shape, int *minb, float *ms);
 int astr_hip_set_pass_config(astr_hip_ctx *ctx, int prog, int dir, int shape, int minb);
 

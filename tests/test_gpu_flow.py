@@ -186,7 +186,8 @@ def test_every_launch_selection(dims):
             continue
         for minb in (1, 2):
             f = A.Flow(A.tgv_config(*dims, deltat=1e-3))
-            for prog in range(7):
+            f.split_last = (shape + minb) % 2 == 1   # both forms of the last pass
+            for prog in range(len(A.PASS_NAMES)):
                 for d in range(3):
                     f.set_pass_config(prog, d, shape, minb)
             f.upload(q0)
@@ -197,6 +198,23 @@ def test_every_launch_selection(dims):
             assert max(state_err(f.download(), q_ref)) < RHS_TOL, (shape, minb)
             tried += 1
     assert tried >= 4
+
+
+@pytest.mark.parametrize("split", [False, True])
+def test_last_pass_split_matches_oracle(split):
+    f, s = make_pair(36, 32, 40)
+    f.split_last = split
+    got = f.rhscal()
+    s.rhscal()
+    assert max(per_var(got, s.qrhs)) < RHS_TOL
+    dvel, dtmp = f.gradcal()
+    assert max(per_var(dvel, s.dvel)) < RHS_TOL
+    assert max(per_var(dtmp, s.dtmp)) < 1e-11
+    f.upload(s.q.copy())
+    ke, en = f.rk3(3)
+    ke0, en0 = s.rk3(3)
+    assert max(state_err(f.download(), s.q)) < RHS_TOL
+    assert relmax(ke, ke0) < HIST_TOL and relmax(en, en0) < HIST_TOL
 
 
 def test_autotune_restores_state():
