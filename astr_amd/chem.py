@@ -35,6 +35,9 @@ def _bind():
     L.astr_hip_chem_rates_device.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p,
                                              C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_void_p, C.c_void_p]
+    L.astr_hip_chem_temperature.argtypes = [C.c_void_p, C.c_longlong, _dp, _dp, _dp]
+    L.astr_hip_chem_temperature_device.argtypes = [C.c_void_p, C.c_longlong, C.c_void_p, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p]
     L.astr_hip_reactor_create.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double,
                                           C.POINTER(C.c_void_p)]
     L.astr_hip_reactor_destroy.argtypes = [C.c_void_p]
@@ -145,6 +148,18 @@ def sample_cells(names, W, n, seed):
     phi = rng.uniform(0.3, 3.0, n)
     c = rng.uniform(0.0, 1.0, n)
     return T, p, mixture(names, W, phi, c)
+
+
+def temperature(mech, Y, e, T0):
+    """temperature_calc: T from internal energy e (J/kg) and Y (ns, n), from
+    the initial guesses T0 (thermchem.F90:1775-1786)."""
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64)
+    T = np.array(T0, dtype=np.float64, copy=True).reshape(-1)
+    if Y.shape != (mech.ns, T.size) or e.shape != (T.size,):
+        raise AstrError("shapes: Y (ns, n), e (n,), T0 (n,)")
+    _chk(_bind().astr_hip_chem_temperature(mech.h, T.size, _p(Y), _p(e), _p(T)))
+    return T
 
 
 class ReactorPool:

@@ -194,6 +194,54 @@ __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long l
 }
 
 // ---------------------------------------------------------------------------
+// temperature_calc (thermchem.F90:1775-1786): T from the mass-specific
+// internal energy e at fixed density and composition (Cantera setState_UV;
+// for an ideal gas e does not depend on density).  Newton on
+//   e(T) = sum_k Y_k (h_k(T) - R T / W_k),  de/dT = cv = cp - R sum Y_k/W_k
+// with the NASA-7 polynomials and Cantera's R, from the guess in T.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void e_cv_of(const Mech* __restrict__ M, const double* y, double T,
+                                        double& e, double& cv) {
+  const int ns = M->ns;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  double h = 0.0, cp = 0.0, winv = 0.0;
+  for (int k = 0; k < ns; k++) {
+    const double* a = T <= M->tmid[k] ? M->lo[k] : M->hi[k];
+    const double cpr = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
+    const double hrt = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 + a[4] * T4 / 5.0 +
+                       a[5] / T;
+    const double yw = y[k] / M->W[k];
+    h += yw * hrt * T;
+    cp += yw * cpr;
+    winv += yw;
+  }
+  e = kRgas * (h - winv * T);
+  cv = kRgas * (cp - winv);
+}
+
+__global__ __launch_bounds__(256) void k_temperature(const Mech* __restrict__ M, long long n,
+                                                     const double* __restrict__ Y,
+                                                     const double* __restrict__ E,
+                                                     double* __restrict__ T, int* __restrict__ iters) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double y[NSMAX];
+  for (int k = 0; k < M->ns; k++) y[k] = Y[k * n + i];
+  const double et = E[i];
+  double t = T[i];
+  int it = 0;
+  for (; it < 50; it++) {
+    double e, cv;
+    e_cv_of(M, y, t, e, cv);
+    const double dt = (et - e) / cv;
+    t += dt;
+    if (fabs(dt) <= 1e-13 * fabs(t)) break;
+  }
+  T[i] = t;
+  if (iters) iters[i] = it + 1;
+}
+
+// ---------------------------------------------------------------------------
 // 0-D constant-pressure adiabatic reactor (no reference counterpart: the
 // reference's 0dreactor is constant-volume).  State y = (T, Y_1..Y_ns):
 //   dY_k/dt = wdot_k W_k / rho,  dT/dt = -sum_k h_k dY_k/dt / cp,
@@ -1055,6 +1103,45 @@ int astr_hip_reactor_release(astr_reactor_pool* r, int slot) {
   CHIP_OK(hipMemcpyAsync(r->P.status + slot, &v, sizeof(int), hipMemcpyHostToDevice, r->stream));
   CHIP_OK(hipStreamSynchronize(r->stream));
   return 0;
+}
+
+int astr_hip_chem_temperature_device(astr_chem* c, long long n, const double* Y, const double* e,
+                                     double* T, void* stream) {
+  if (!c) return cfail("null mechanism");
+  if (n < 0) return cfail("negative cell count");
+  if (n == 0) return 0;
+  if (!Y || !e || !T) return cfail("null buffer");
+  CHIP_OK(hipSetDevice(c->device));
+  const int bs = 256;
+  hipLaunchKernelGGL(k_temperature, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0,
+                     (hipStream_t)stream, c->dev, n, Y, e, T, nullptr);
+  CHIP_OK(hipGetLastError());
+  return 0;
+}
+
+int astr_hip_chem_temperature(astr_chem* c, long long n, const double* Y, const double* e, double* T) {
+  if (!c) return cfail("null mechanism");
+  if (n < 0) return cfail("negative cell count");
+  if (n == 0) return 0;
+  if (!Y || !e || !T) return cfail("null buffer");
+  CHIP_OK(hipSetDevice(c->device));
+  const int ns = c->host.ns;
+  const size_t b1 = (size_t)n * sizeof(double);
+  double* d = nullptr;
+  CHIP_OK(hipMalloc(&d, (2 + (size_t)ns) * b1));
+  double* dY = d;
+  double* dE = dY + (size_t)ns * n;
+  double* dT = dE + n;
+  int rc = 0;
+  if (hipMemcpy(dY, Y, b1 * ns, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dE, e, b1, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dT, T, b1, hipMemcpyHostToDevice) != hipSuccess)
+    rc = cfail("copy to device failed");
+  if (!rc) rc = astr_hip_chem_temperature_device(c, n, dY, dE, dT, nullptr);
+  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = cfail("temperature kernel failed");
+  if (!rc && hipMemcpy(T, dT, b1, hipMemcpyDeviceToHost) != hipSuccess) rc = cfail("copy from device failed");
+  (void)hipFree(d);
+  return rc;
 }
 
 }  // extern "C"

@@ -13,9 +13,11 @@
 !                        rk3, steploop.  The fields live in HBM; the host
 !                        arrays are filled when a subroutine's outputs are
 !                        the reference's module arrays.
-! module astr_hip_thermchem  chemread, chemrate, cpeval, enthpy with the
+! module astr_hip_thermchem  chemread, chemrate, cpeval, enthpy,
+!                        temperature_calc, rgcmix_scar, convertxiyi with the
 !                        reference's argument lists (src/thermchem.F90:59,
-!                        :1553, :1342, :1498) plus array forms over many cells
+!                        :1553, :1342, :1498, :1775, :1281, :1639) plus an
+!                        array form of chemrate over many cells
 !-----------------------------------------------------------------------
 module astr_hip_capi
   use iso_c_binding
@@ -121,6 +123,13 @@ module astr_hip_capi
       type(c_ptr), value :: chem, nsp, nrx, mw, names
       integer(c_int), value :: name_len
       integer(c_int) :: astr_hip_chem_info
+    end function
+    function astr_hip_chem_temperature(chem, n, y, e, t) bind(C, name='astr_hip_chem_temperature')
+      import :: c_int, c_ptr, c_long_long
+      type(c_ptr), value :: chem
+      integer(c_long_long), value :: n
+      type(c_ptr), value :: y, e, t
+      integer(c_int) :: astr_hip_chem_temperature
     end function
     function astr_hip_chem_rates(chem, n, t, rp, input_is_p, y, wdot, hmix, cpmix, hk) &
         bind(C, name='astr_hip_chem_rates')
@@ -441,6 +450,39 @@ contains
                                        c_loc(w), c_null_ptr, c_null_ptr, c_loc(hk)), 'enthpy')
     hi = hk
   end subroutine enthpy
+
+  ! temperature_calc (thermchem.F90:1775-1786): T from (den, spc, eint);
+  ! tmp holds the initial guess on entry
+  subroutine temperature_calc(tmp, den, spc, eint)
+    real(8), intent(in) :: den, spc(:), eint
+    real(8), intent(inout) :: tmp
+    real(8), target :: y(num_species), e(1), t(1)
+    y = spc(1:num_species); e(1) = eint; t(1) = tmp
+    call chemcheck(astr_hip_chem_temperature(mixture, 1_c_long_long, c_loc(y), c_loc(e), c_loc(t)), &
+                   'temperature_calc')
+    tmp = t(1)
+  end subroutine temperature_calc
+
+  ! rgcmix (thermchem.F90:1281-1289): mixture gas constant, R = 8314.2
+  function rgcmix_scar(spc) result(vout)
+    real(8), intent(in) :: spc(:)
+    real(8) :: vout
+    vout = sum(spc(1:num_species) * rguniv / wmolar)
+  end function rgcmix_scar
+
+  ! convertxiyi (thermchem.F90:1639-1657): mole <-> mass fractions
+  subroutine convertxiyi(fracin, fracout, mode)
+    character(len=*), intent(in) :: mode
+    real(8), intent(in) :: fracin(:)
+    real(8), intent(out) :: fracout(:)
+    if (mode == 'X2Y') then
+      fracout(1:num_species) = (fracin(1:num_species) * wmolar) / sum(fracin(1:num_species) * wmolar)
+    elseif (mode == 'Y2X') then
+      fracout(1:num_species) = (fracin(1:num_species) / wmolar) / sum(fracin(1:num_species) / wmolar)
+    else
+      stop ' !!Error - wrong mode given in convertxiyi!!'
+    end if
+  end subroutine convertxiyi
 
   subroutine chemend
     if (c_associated(mixture)) call chemcheck(astr_hip_chem_destroy(mixture), 'chemend')

@@ -6,9 +6,11 @@
 ! usage: chemcells_hip MECHANISM.xml CELLS.txt OUT.txt
 !   CELLS.txt: first line n; then n lines "den tmp Y(1..num_species)"
 !   OUT.txt:   per cell one line
-!              "wi(1..num_species) cp eng hi(1..num_species)"
+!              "wi(1..num_species) cp eng hi(1..num_species) T_e rgc Xmole(1..)"
 !              wi from one chemrate_cells call over all cells; cp, eng and hi
-!              from per-cell cpeval / enthpy calls
+!              from per-cell cpeval / enthpy calls; T_e the temperature that
+!              temperature_calc recovers from eng starting at 1000 K; rgc
+!              from rgcmix_scar; Xmole from convertxiyi(Y2X)
 !-----------------------------------------------------------------------
 program chemcells_hip
   use astr_hip_thermchem
@@ -16,7 +18,8 @@ program chemcells_hip
   character(len=512) :: mech, fin, fout
   integer :: n, i, k
   real(8), allocatable :: den(:), tmp(:), spc(:,:), wi(:,:), hi(:), row(:)
-  real(8) :: cp, eng
+  real(8) :: cp, eng, te
+  real(8), allocatable :: xm(:)
 
   call get_command_argument(1, mech)
   call get_command_argument(2, fin)
@@ -36,13 +39,18 @@ program chemcells_hip
   close(21)
 
   call chemrate_cells(n, den, tmp, spc, wi)
+  allocate(xm(num_species))
 
   open(22, file=trim(fout), status='replace')
   do i = 1, n
     call cpeval(tmp(i), spc(i, :), cp=cp)
     call cpeval(tmp(i), spc(i, :), eng=eng)
     call enthpy(tmp(i), hi)
-    write(22, '(*(ES25.17E3,1X))') (wi(i, k), k = 1, num_species), cp, eng, (hi(k), k = 1, num_species)
+    te = 1000.d0
+    call temperature_calc(te, den(i), spc(i, :), eng)
+    call convertxiyi(spc(i, :), xm, 'Y2X')
+    write(22, '(*(ES25.17E3,1X))') (wi(i, k), k = 1, num_species), cp, eng, (hi(k), k = 1, num_species), &
+        te, rgcmix_scar(spc(i, :)), (xm(k), k = 1, num_species)
   end do
   close(22)
   call chemend

@@ -185,6 +185,14 @@ int astr_hip_chem_rates_device(astr_chem *chem, long long n, const double *T,
                                double *wdot, double *h_mix, double *cp_mix, double *h_k,
                                void *stream);
 
+/* temperature_calc (thermchem.F90:1775-1786; Cantera setState_UV): T from
+ * the mass-specific internal energy e (J/kg) and Y (nsp x n, SoA); T holds
+ * the initial guesses on entry and the temperatures on return.  Newton on
+ * e(T) = sum_k Y_k (h_k(T) - R T/W_k) to |dT| <= 1e-13 T. */
+int astr_hip_chem_temperature(astr_chem *chem, long long n, const double *Y, const double *e, double *T);
+int astr_hip_chem_temperature_device(astr_chem *chem, long long n, const double *Y, const double *e,
+                                     double *T, void *stream);
+
 /* ------------------------------------------------------------------------
  * 0-D constant-pressure adiabatic reactors (no reference counterpart: the
  * reference's 0dreactor, src/solver.F90:208,256-258, is constant-volume).

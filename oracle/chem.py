@@ -193,3 +193,40 @@ class Mechanism:
         if not sol.success:
             raise RuntimeError(sol.message)
         return sol.y[:, -1]
+
+    # ---- thermodynamic helpers (thermchem.F90:1281-1329, 1639-1657, 1775-1786)
+    def e_mass(self, T, Y):
+        """Mass-specific internal energy sum_k Y_k (h_k - R T / W_k), J/kg."""
+        T = np.asarray(T, dtype=float)
+        Y = np.asarray(Y, dtype=float)
+        _, hrt, _ = self.thermo(T)
+        return R_GAS * ((Y / self.W[:, None]) * (hrt.T * T - T)).sum(axis=0)
+
+    def temperature(self, e, Y, T0, tol=1e-13, maxit=50):
+        """Inverse of e_mass by Newton iteration from T0 (setState_UV)."""
+        T = np.array(T0, dtype=float)
+        Y = np.asarray(Y, dtype=float)
+        for _ in range(maxit):
+            cpr, hrt, _ = self.thermo(T)
+            winv = (Y / self.W[:, None]).sum(axis=0)
+            cv = R_GAS * ((Y / self.W[:, None]) * cpr.T).sum(axis=0) - R_GAS * winv
+            dT = (e - self.e_mass(T, Y)) / cv
+            T = T + dT
+            if np.all(np.abs(dT) <= tol * np.abs(T)):
+                break
+        return T
+
+    def rgcmix(self, Y):
+        """Mixture gas constant with the reference's R = 8314.2 (rgcmix)."""
+        return (np.asarray(Y, dtype=float) * (R_REF / self.W)[:, None]).sum(axis=0)
+
+    def convertxiyi(self, frac, mode):
+        """Mole <-> mass fractions (convertxiyi), frac of shape (ns, n)."""
+        frac = np.asarray(frac, dtype=float)
+        if mode == "X2Y":
+            v = frac * self.W[:, None]
+        elif mode == "Y2X":
+            v = frac / self.W[:, None]
+        else:
+            raise ValueError("mode must be X2Y or Y2X")
+        return v / v.sum(axis=0)

@@ -62,6 +62,28 @@ def test_thermo_consistency(mech):
     assert np.abs(cp_hi - lo[0]).max() < 5e-3 * np.abs(lo[0]).max()
 
 
+def test_temperature_inverts_internal_energy(mech):
+    """e(T) and its Newton inverse: round trip from guesses 500 K off, and
+    cv = de/dT (central differences) so the Newton slope is the true one."""
+    T, p, Y = AC.sample_cells(mech.names, mech.W, 200, 17)
+    e = mech.e_mass(T, Y)
+    T1 = mech.temperature(e, Y, T + 500.0)
+    assert np.abs(T1 / T - 1).max() < 1e-13
+    cpr, _, _ = mech.thermo(T)
+    cv = OC.R_GAS * ((Y / mech.W[:, None]) * (cpr.T - 1.0)).sum(axis=0)
+    de = (mech.e_mass(T + 1e-3, Y) - mech.e_mass(T - 1e-3, Y)) / 2e-3
+    assert np.abs(de / cv - 1).max() < 1e-8
+
+
+def test_mixture_helpers(mech):
+    T, p, Y = AC.sample_cells(mech.names, mech.W, 50, 18)
+    X = mech.convertxiyi(Y, "Y2X")
+    assert np.abs(X.sum(axis=0) - 1).max() < 1e-15
+    assert np.abs(mech.convertxiyi(X, "X2Y") - Y).max() < 1e-15
+    # R_mix = R / W_mix with W_mix = sum X_k W_k
+    assert np.abs(mech.rgcmix(Y) * (X * mech.W[:, None]).sum(axis=0) / OC.R_REF - 1).max() < 1e-14
+
+
 def test_published_rate_constant(mech):
     """H + O2 = OH + O of Jachimowski (1988): A = 2.6e14 cm^3/mol/s,
     E = 16.8 kcal/mol, evaluated with R in cal/mol/K; 1 cm^3/mol = 1e-3 m^3/kmol."""
@@ -127,6 +149,15 @@ def test_gpu_rates_match_oracle(mech):
             assert np.abs(wd[k] - wd0[k]).max() / den < 1e-12, mech.names[k]
     assert np.abs(h - h0).max() / np.abs(h0).max() < 1e-13
     assert np.abs(cp / cp0 - 1).max() < 1e-13
+
+
+@pytest.mark.gpu
+def test_gpu_temperature_calc(mech):
+    gm = AC.Mechanism(MECH)
+    T, p, Y = AC.sample_cells(mech.names, mech.W, 65536, 19)
+    e = mech.e_mass(T, Y)
+    T1 = AC.temperature(gm, Y, e, np.full(T.size, 1000.0))
+    assert np.abs(T1 / T - 1).max() < 1e-12
 
 
 @pytest.mark.gpu

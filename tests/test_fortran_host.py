@@ -74,7 +74,8 @@ def test_chemcells_hip_matches_oracle(tmp_path):
     _run([os.path.join(FDIR, "chemcells_hip"), MECH, "cells.txt", "out.txt"], tmp_path)
     out = np.loadtxt(tmp_path / "out.txt")
     ns = m.ns
-    wi, cp, eng, hi = out[:, :ns].T, out[:, ns], out[:, ns + 1], out[:, ns + 2:].T
+    wi, cp, eng, hi = out[:, :ns].T, out[:, ns], out[:, ns + 1], out[:, ns + 2:2 * ns + 2].T
+    te, rgc, xm = out[:, 2 * ns + 2], out[:, 2 * ns + 3], out[:, 2 * ns + 4:].T
     wd0, _, cp0, hk0 = m.rates(T, rho, Y, input_is_p=False)
     for k in range(ns):
         den = np.abs(wd0[k]).max()
@@ -87,3 +88,7 @@ def test_chemcells_hip_matches_oracle(tmp_path):
     e0 = (Y * (hk0 * (OC.R_GAS / OC.R_REF) - OC.R_GAS * T / m.W[:, None])).sum(axis=0)
     assert np.abs(eng - e0).max() / np.abs(e0).max() < 1e-13
     assert np.abs(hi - hk0).max() / np.abs(hk0).max() < 1e-13
+    # temperature_calc inverts eng back to T; rgcmix; convertxiyi
+    assert np.abs(te / T - 1).max() < 1e-12
+    assert np.abs(rgc / m.rgcmix(Y) - 1).max() < 1e-14
+    assert np.abs(xm - m.convertxiyi(Y, "Y2X")).max() < 1e-15
