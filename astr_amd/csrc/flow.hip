@@ -47,6 +47,19 @@ int fail(const std::string& m) {
   return -1;
 }
 
+// device scratch released on every return path (hipFree waits for the
+// device, so no kernel still uses it)
+struct DevBuf {
+  double* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
+};
+
 #define HIP_OK(x)                                                                  \
   do {                                                                             \
     hipError_t e_ = (x);                                                           \
@@ -490,6 +503,11 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   a.plane = (long long)a.pitch * (cfg->jm + 1);
   a.trash = (int)(a.plane * (cfg->km + 1));
   a.fs = (a.plane * (cfg->km + 1) + 1 + 63) / 64 * 64;
+  // point offsets inside a field are 32-bit byte offsets (flow_dev.hpp ldg/stg)
+  if ((unsigned long long)a.fs * sizeof(double) >= (1ull << 32)) {
+    delete c;
+    return fail("grid too large: a field must stay below 4 GiB");
+  }
   const double g = cfg->gamma, ma = cfg->mach;
   a.const2 = g * ma * ma;
   a.const5 = (g - 1.0) * ma * ma;
@@ -619,15 +637,14 @@ static bool bad_halo(const FlowArgs& a, int hm) {
 static int xfer(astr_hip_ctx* c, double* host, int nf, double* dev, int hm, int to_dev) {
   const FlowArgs& a = c->a;
   const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
-  double* tmp = nullptr;
-  HIP_OK(hipMalloc(&tmp, dn * nf * sizeof(double)));
-  if (to_dev) HIP_OK(hipMemcpyAsync(tmp, host, dn * nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_pack, dense_grid(a, hm), dim3(64), 0, c->stream, to_dev ? tmp : dev,
-                     to_dev ? dev : tmp, a, nf, hm, to_dev);
+  DevBuf tmp;
+  HIP_OK(tmp.alloc(dn * nf * sizeof(double)));
+  if (to_dev) HIP_OK(hipMemcpyAsync(tmp.p, host, dn * nf * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_pack, dense_grid(a, hm), dim3(64), 0, c->stream, to_dev ? tmp.p : dev,
+                     to_dev ? dev : tmp.p, a, nf, hm, to_dev);
   HIP_OK(hipGetLastError());
-  if (!to_dev) HIP_OK(hipMemcpyAsync(host, tmp, dn * nf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  if (!to_dev) HIP_OK(hipMemcpyAsync(host, tmp.p, dn * nf * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  HIP_OK(hipFree(tmp));
   return 0;
 }
 
@@ -644,13 +661,12 @@ int astr_hip_download(astr_hip_ctx* c, double* q, double* prims, int hm) {
   if (prims) {
     const FlowArgs& a = c->a;
     const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
-    double* tmp = nullptr;
-    HIP_OK(hipMalloc(&tmp, dn * 6 * sizeof(double)));
-    hipLaunchKernelGGL(k_prims, dense_grid(a, hm), dim3(64), 0, c->stream, a, tmp, hm);
+    DevBuf tmp;
+    HIP_OK(tmp.alloc(dn * 6 * sizeof(double)));
+    hipLaunchKernelGGL(k_prims, dense_grid(a, hm), dim3(64), 0, c->stream, a, tmp.p, hm);
     HIP_OK(hipGetLastError());
-    HIP_OK(hipMemcpyAsync(prims, tmp, dn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipMemcpyAsync(prims, tmp.p, dn * 6 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
-    HIP_OK(hipFree(tmp));
   }
   return 0;
 }
@@ -790,12 +806,15 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   const FlowArgs keep = a;
   // the passes overwrite q and qsave: keep a copy and put it back afterwards
   const size_t qbytes = (size_t)10 * a.fs * sizeof(double);
-  double* snap = nullptr;
-  HIP_OK(hipMalloc(&snap, qbytes));
+  DevBuf snapb;
+  HIP_OK(snapb.alloc(qbytes));
+  double* snap = snapb.p;
   HIP_OK(hipMemcpyAsync(snap, a.q, qbytes, hipMemcpyDeviceToDevice, c->stream));
-  hipEvent_t e0, e1;
-  HIP_OK(hipEventCreate(&e0));
-  HIP_OK(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    if (e0) (void)hipEventDestroy(e0);
+    return fail("hipEventCreate failed");
+  }
   a.stage = 0;
   a.rk_a = 1.0; a.rk_b = 0.0; a.rk_c = 1.0;
   a.dostats = 0;
@@ -837,7 +856,6 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   a = keep;
   hipError_t e = hipMemcpyAsync(a.q, snap, qbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-  (void)hipFree(snap);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (!rc && e != hipSuccess) rc = fail(std::string("autotune restore: ") + hipGetErrorString(e));
@@ -891,10 +909,11 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
   if (op == 0) deriv_tables(n, t2);
   else filter_tables(n, alfa_filter, beta_halo, t2);
   tab.insert(tab.end(), t2.begin(), t2.end());
-  double *dt = nullptr, *df = nullptr, *dout = nullptr;
-  HIP_OK(hipMalloc(&dt, tab.size() * sizeof(double)));
-  HIP_OK(hipMalloc(&df, (size_t)nlines * (n + 11) * sizeof(double)));
-  HIP_OK(hipMalloc(&dout, (size_t)nlines * (n + 1) * sizeof(double)));
+  DevBuf bt, bf, bo;
+  HIP_OK(bt.alloc(tab.size() * sizeof(double)));
+  HIP_OK(bf.alloc((size_t)nlines * (n + 11) * sizeof(double)));
+  HIP_OK(bo.alloc((size_t)nlines * (n + 1) * sizeof(double)));
+  double *dt = bt.p, *df = bf.p, *dout = bo.p;
   HIP_OK(hipMemcpy(dt, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(df, f, (size_t)nlines * (n + 11) * sizeof(double), hipMemcpyHostToDevice));
   const ShapeInfo& s = kShapes[sh];
@@ -915,9 +934,6 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
 #undef LB
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(out, dout, (size_t)nlines * (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
-  (void)hipFree(dt);
-  (void)hipFree(df);
-  (void)hipFree(dout);
   return 0;
 }
 
