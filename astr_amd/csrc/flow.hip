@@ -274,7 +274,7 @@ struct ShapeInfo {
 };
 const ShapeInfo kShapes[kNumShapes] = {
     {0, 256, 4, 64, 5, 5}, {1, 256, 4, 64, 9, 9}, {2, 256, 16, 16, 5, 5},
-    {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5},
+    {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5}, {5, 512, 8, 64, 5, 5},
 };
 
 bool shape_fits(const ShapeInfo& s, int n) {
@@ -285,7 +285,7 @@ bool shape_fits(const ShapeInfo& s, int n) {
 size_t pass_lds_bytes(int shape, int /*prog*/) {
   const ShapeInfo& s = kShapes[shape];
   const long long tl = (long long)s.TL * line_stride(s.P, s.R, s.TL);
-  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8;
+  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8 + kHaloSlot;
   return (size_t)one * sizeof(double);
 }
 
@@ -329,7 +329,8 @@ hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir
     case 1: return launch_pass_s1(prog, minb, a, dir, st);
     case 2: return launch_pass_s2(prog, minb, a, dir, st);
     case 3: return launch_pass_s3(prog, minb, a, dir, st);
-    default: return launch_pass_s4(prog, minb, a, dir, st);
+    case 4: return launch_pass_s4(prog, minb, a, dir, st);
+    default: return launch_pass_s5(prog, minb, a, dir, st);
   }
 }
 
@@ -929,7 +930,8 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
     case 1: LB(CfgB); break;
     case 2: LB(CfgC); break;
     case 3: LB(CfgD); break;
-    default: LB(CfgE); break;
+    case 4: LB(CfgE); break;
+    default: LB(CfgF); break;
   }
 #undef LB
   HIP_OK(hipGetLastError());
@@ -991,7 +993,8 @@ hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, in
     case 1: return launch_batch_s1(prog, minb, bl, dir, sa, nblocks, st);
     case 2: return launch_batch_s2(prog, minb, bl, dir, sa, nblocks, st);
     case 3: return launch_batch_s3(prog, minb, bl, dir, sa, nblocks, st);
-    default: return launch_batch_s4(prog, minb, bl, dir, sa, nblocks, st);
+    case 4: return launch_batch_s4(prog, minb, bl, dir, sa, nblocks, st);
+    default: return launch_batch_s5(prog, minb, bl, dir, sa, nblocks, st);
   }
 }
 

@@ -100,9 +100,29 @@ __device__ __forceinline__ void stg(double* base, unsigned off, double v) {
 // tile) or no halo copy points s/h at index TL*LT: its T writes land in the
 // dummy cell (T + kLineOff + TL*LT) and its X reads return T's zero padding
 // (X + TL*LT = row -8 of T's first line), so no slot needs a guard.
+// After them sit the filter's halo-row coefficients (3 rows x 11), staged
+// once per tile so that no filter solve waits on a global load.
+constexpr int kHaloCoefs = 33;
+constexpr int kHaloSlot = 40;
+template <class C>
+constexpr int lds_line_doubles() {
+  return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
+}
 template <class C>
 constexpr int lds_doubles() {
-  return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
+  return lds_line_doubles<C>() + kHaloSlot;
+}
+
+// Copy the filter's halo-row coefficients into the tile's LDS; the barrier
+// before the first filter solve publishes them.  Every read of the previous
+// tile's copy ended before that tile's last barrier.
+template <class C>
+__device__ __forceinline__ FilterCoef stage_halo_coefs(const FilterCoef& fc, double* smem) {
+  double* ch = smem + lds_line_doubles<C>();
+  if (threadIdx.x < kHaloCoefs) ch[threadIdx.x] = fc.ch[threadIdx.x];
+  FilterCoef f = fc;
+  f.ch = ch;
+  return f;
 }
 
 // Map the tile's points onto slots.  dir 0: lines are (j,k), rows i; dir 1:
@@ -461,6 +481,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     } else {
       ChunkCoef<C::R, true> CF;
       load_chunk_coef<C::R, true>(CF, a.fac[dir], n + 7, p * C::R);
+      const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
       // Loads run two solves ahead: variable c+1's qrhs, q, qsave are issued
       // before variable c's filter solve, the next stress column before
       // variable c's derivative solve.
@@ -481,7 +502,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         qs[t] = sa.stage == 0 ? 0.0 : ldg(a.qsave + fs, S.g[t]);
       }
       __syncthreads();
-      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
+      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < NPT; t++) stg(a.q, S.g[t], X[S.s[t]]);
@@ -517,7 +538,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           }
         }
         __syncthreads();
-        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
+        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
         __syncthreads();
         double* dq = a.q + (1 + c) * fs;
 #pragma unroll
@@ -527,6 +548,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
   } else {  // F_ONLY: filter of q along dir, in place (filterq, tgvsolver.F90:791-883)
     ChunkCoef<C::R, true> CF;
     load_chunk_coef<C::R, true>(CF, a.fac[dir], n + 7, p * C::R);
+    const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
     double nx[NPT];
 #pragma unroll
     for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q, S.g[t]);
@@ -540,7 +562,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q + (m + 1) * fs, S.g[t]);
       }
       __syncthreads();
-      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, a.fc);
+      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
       __syncthreads();
       double* dst = a.q + m * fs;
 #pragma unroll
@@ -630,7 +652,8 @@ using CfgB = LineCfg<256, 4, 9, 9>;    // P=64, n <= 569
 using CfgC = LineCfg<256, 16, 5, 5>;   // P=16, n <= 73
 using CfgD = LineCfg<256, 8, 9, 9>;    // P=32, n <= 281
 using CfgE = LineCfg<256, 8, 5, 5>;    // P=32, n <= 153
-constexpr int kNumShapes = 5;
+using CfgF = LineCfg<512, 8, 5, 5>;    // P=64, n <= 313: CfgA's lines, 64-byte row segments
+constexpr int kNumShapes = 6;
 constexpr int kMinbVariants = 2;       // MINB = 1, 2
 
 // one per shape, each in its own translation unit (passes.hip)
@@ -643,6 +666,7 @@ ASTR_DECLARE_SHAPE(1)
 ASTR_DECLARE_SHAPE(2)
 ASTR_DECLARE_SHAPE(3)
 ASTR_DECLARE_SHAPE(4)
+ASTR_DECLARE_SHAPE(5)
 #undef ASTR_DECLARE_SHAPE
 
 }  // namespace astr

@@ -1,10 +1,10 @@
 // passes.hip -- the pass kernels of one launch shape (single case and
-// batched), in register budgets MINB = 1 and 2.  Built once per shape with ASTR_SHAPE_ID = 0..4 so the
+// batched), in register budgets MINB = 1 and 2.  Built once per shape with ASTR_SHAPE_ID = 0..5 so the
 // shapes compile in parallel; flow.hip dispatches to launch_pass_s<ID>.
 #include "flow_dev.hpp"
 
 #ifndef ASTR_SHAPE_ID
-#error "build with -DASTR_SHAPE_ID=<0..4>"
+#error "build with -DASTR_SHAPE_ID=<0..5>"
 #endif
 
 namespace astr {
@@ -25,10 +25,14 @@ using ShapeCfg = CfgC;
 #define ASTR_LAUNCH launch_pass_s3
 #define ASTR_BATCH launch_batch_s3
 using ShapeCfg = CfgD;
-#else
+#elif ASTR_SHAPE_ID == 4
 #define ASTR_LAUNCH launch_pass_s4
 #define ASTR_BATCH launch_batch_s4
 using ShapeCfg = CfgE;
+#else
+#define ASTR_LAUNCH launch_pass_s5
+#define ASTR_BATCH launch_batch_s5
+using ShapeCfg = CfgF;
 #endif
 
 hipError_t ASTR_LAUNCH(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st) {

@@ -182,7 +182,7 @@ def test_every_launch_selection(dims):
     q_ref = s.q.copy()
     tried = 0
     for shape, (tl, p, r) in A.SHAPES.items():
-        if p * r < max(dims) + 7 or 256 * r < tl * (max(dims) + 1):
+        if p * r < max(dims) + 7 or tl * p * r < tl * (max(dims) + 1):
             continue
         for minb in (1, 2):
             f = A.Flow(A.tgv_config(*dims, deltat=1e-3))
