@@ -225,8 +225,10 @@ __device__ __forceinline__ void solve_line(const double* __restrict__ T, double*
 
   if constexpr (FILT) {
     // the six halo rows: lane q<6 of each line evaluates row q (-3,-2,-1,
-    // n+1,n+2,n+3) over f(-5..5) or f(n+5..n-5); the owners pick them up
-    // by readlane
+    // n+1,n+2,n+3) over f(-5..5) or f(n+5..n-5) and leaves it in X[n+1+q],
+    // a row of the output line that nothing reads before this solve writes
+    // X[0..n]; a line's lanes are one wavefront, whose LDS accesses complete
+    // in order, so the owners read the six values back after a wave barrier
     double ev = 0.0;
     if (p < 6) {
       const int hr = p < 3 ? p : 5 - p;  // coefh row
@@ -237,21 +239,14 @@ __device__ __forceinline__ void solve_line(const double* __restrict__ T, double*
         ev = ev + fc.ch[hr * 11 + j] * f;
       }
     }
-    constexpr int NSEG = 64 / P;
-    const int myseg = (threadIdx.x & 63) / P;
-    double evs[NSEG][6];
-#pragma unroll
-    for (int s = 0; s < NSEG; s++)
-#pragma unroll
-      for (int q = 0; q < 6; q++) evs[s][q] = readlane(ev, s * P + q);
+    double* hx = X + n + 1;
+    if (p < 6) hx[p] = ev;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     double mine[6];
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-      double v = evs[0][q];
-#pragma unroll
-      for (int s = 1; s < NSEG; s++) v = myseg == s ? evs[s][q] : v;
-      mine[q] = v;
-    }
+    for (int q = 0; q < 6; q++) mine[q] = hx[q];
 #pragma unroll
     for (int k = 0; k < R; k++) {
       const int r = u0 + k - OFF;
