@@ -285,6 +285,19 @@ def load_traffic(n):
     return None
 
 
+def apply_selection(f, passes):
+    """Force the launch selection recorded in a bench line's launch.passes."""
+    import astr_amd as A
+    ids = {tuple(v): k for k, v in A.SHAPES.items()}
+    for key, v in passes.items():
+        if key == "last_pass_split":
+            f.split_last = bool(v)
+            continue
+        name, d = key.rsplit("_", 1)
+        f.set_pass_config(A.PASS_NAMES.index(name), "xyz".index(d), ids[tuple(v["shape"])],
+                          int(v["minb"]))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -297,6 +310,10 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-form measurement")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch selection instead of timing the candidates")
+    ap.add_argument("--selection", metavar="JSON",
+                    help="apply the launch selection of an earlier bench line (a file holding "
+                         "that JSON line) instead of autotuning; counter runs use it so that "
+                         "they measure the kernels the timed run chose")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="also time a device copy of BYTES (counter calibration)")
     args = ap.parse_args()
@@ -312,7 +329,11 @@ def main():
     f = A.Flow(A.tgv_config(n, deltat=dt))
     f.init_tgv(1e-3, 2024)
     tune_s = None
-    if not args.no_autotune:
+    if args.selection:
+        with open(args.selection) as fh:
+            line = [ln for ln in fh if ln.lstrip().startswith("{")][-1]
+        apply_selection(f, json.loads(line)["launch"]["passes"])
+    elif not args.no_autotune:
         t0 = time.perf_counter()
         f.autotune(reps=3)
         tune_s = time.perf_counter() - t0
