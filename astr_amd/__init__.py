@@ -51,6 +51,7 @@ EXPORTS = [
     "astr_hip_get_pass_config", "astr_hip_set_pass_config", "astr_hip_init_vortex",
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
     "astr_hip_batch_rk3_steps", "astr_hip_batch_info", "astr_hip_set_last_split",
+    "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
     "astr_hip_get_last_split",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
@@ -99,6 +100,9 @@ def lib():
         L.astr_hip_batch_autotune.argtypes = [C.c_void_p, C.c_int]
         L.astr_hip_batch_rk3_steps.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp]
         L.astr_hip_batch_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.astr_hip_batch_rk3_begin.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.astr_hip_batch_query.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.astr_hip_batch_rk3_end.argtypes = [C.c_void_p, _dp, _dp]
         _lib = L
     return _lib
 
@@ -296,6 +300,26 @@ class Batch:
         ke = np.empty((n, nsteps)) if stats else None
         en = np.empty((n, nsteps)) if stats else None
         _chk(lib().astr_hip_batch_rk3_steps(self.h, nsteps, 1 if graph else 0, _p(ke), _p(en)))
+        return (ke, en) if stats else None
+
+    def rk3_begin(self, nsteps, graph=True):
+        """Enqueue nsteps RK3 steps of every case and return at once."""
+        _chk(lib().astr_hip_batch_rk3_begin(self.h, nsteps, 1 if graph else 0))
+        self._pending = nsteps
+
+    def done(self):
+        """True once the steps of the last rk3_begin have finished."""
+        v = C.c_int()
+        _chk(lib().astr_hip_batch_query(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def rk3_end(self, stats=True):
+        """Wait for the steps of the last rk3_begin; their (ke, ens) histories."""
+        n, k = len(self.flows), getattr(self, "_pending", 0)
+        ke = np.empty((n, k)) if stats else None
+        en = np.empty((n, k)) if stats else None
+        _chk(lib().astr_hip_batch_rk3_end(self.h, _p(ke), _p(en)))
+        self._pending = 0
         return (ke, en) if stats else None
 
     def info(self):

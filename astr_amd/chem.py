@@ -45,6 +45,9 @@ def _bind():
                                        C.c_double, C.c_double]
     L.astr_hip_reactor_advance.argtypes = [C.c_void_p, C.c_int]
     L.astr_hip_reactor_sync.argtypes = [C.c_void_p]
+    L.astr_hip_reactor_query.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+    L.astr_hip_reactor_set_lanes.argtypes = [C.c_void_p, C.c_int]
+    L.astr_hip_reactor_get_lanes.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.astr_hip_reactor_status.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
     L.astr_hip_reactor_get.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]
@@ -169,12 +172,25 @@ class ReactorPool:
 
     FREE, RUNNING, DONE, FAILED = 0, 1, 2, 3
 
-    def __init__(self, mech: Mechanism, capacity, rtol=1e-6, atol=1e-12):
+    def __init__(self, mech: Mechanism, capacity, rtol=1e-6, atol=1e-12, lanes=None):
         self.mech = mech
         self.capacity = capacity
         h = C.c_void_p()
         _chk(_bind().astr_hip_reactor_create(mech.h, capacity, rtol, atol, C.byref(h)))
         self.h = h
+        if lanes is not None:
+            self.lanes = lanes
+
+    @property
+    def lanes(self):
+        """Lanes per reactor: 1 (throughput) or 16 (a team, for latency)."""
+        v = C.c_int()
+        _chk(_bind().astr_hip_reactor_get_lanes(self.h, C.byref(v)))
+        return v.value
+
+    @lanes.setter
+    def lanes(self, n):
+        _chk(_bind().astr_hip_reactor_set_lanes(self.h, int(n)))
 
     def close(self):
         if getattr(self, "h", None) is not None and self.h.value:
@@ -198,6 +214,12 @@ class ReactorPool:
 
     def sync(self):
         _chk(_bind().astr_hip_reactor_sync(self.h))
+
+    def idle(self):
+        """True once every launched advance has finished."""
+        v = C.c_int()
+        _chk(_bind().astr_hip_reactor_query(self.h, C.byref(v)))
+        return bool(v.value)
 
     def status(self):
         st = np.zeros(self.capacity, dtype=np.int32)

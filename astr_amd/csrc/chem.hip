@@ -489,8 +489,9 @@ __device__ void reactor_rhs_n(const Mech* __restrict__ M, double p, const double
   f[0] = -dh / cp;
 }
 
-// solve with the LU factors of the per-thread LDS matrix (stride 64)
-template <int N>
+// solve with LU factors in LDS, element (r,c) at a[(r*N + c)*S]: S = 64 for
+// the one-thread kernel's interleaved matrices, 1 for a team's own matrix
+template <int N, int S = 64>
 __device__ __forceinline__ void lu_solve_n(const double* a, const int (&piv)[N], double (&b)[N]) {
 #pragma unroll
   for (int k = 0; k < N; k++) {
@@ -504,14 +505,14 @@ __device__ __forceinline__ void lu_solve_n(const double* a, const int (&piv)[N],
       if (piv[k] == r) b[r] = bk;
     b[k] = bp;
 #pragma unroll
-    for (int r = k + 1; r < N; r++) b[r] -= a[(r * N + k) * 64] * b[k];
+    for (int r = k + 1; r < N; r++) b[r] -= a[(r * N + k) * S] * b[k];
   }
 #pragma unroll
   for (int k = N - 1; k >= 0; k--) {
     double sum = b[k];
 #pragma unroll
-    for (int c = k + 1; c < N; c++) sum -= a[(k * N + c) * 64] * b[c];
-    b[k] = sum / a[(k * N + k) * 64];
+    for (int c = k + 1; c < N; c++) sum -= a[(k * N + c) * S] * b[c];
+    b[k] = sum / a[(k * N + k) * S];
   }
 }
 
@@ -622,6 +623,147 @@ __global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, Re
   P.nstep[i] = nstep;
   P.nrej[i] = nrej;
   P.status[i] = st;
+}
+
+
+// ---------------------------------------------------------------------------
+// The same reactor integrated by a team of 16 lanes (one DPP row), four
+// reactors per wavefront, for latency: the step's N+1 right-hand sides that
+// do not depend on each other (the unperturbed state and the N
+// forward-difference columns) are evaluated by separate lanes at once, and
+// the LU elimination runs one row per lane.  The pivot search, both
+// triangular solves, the second stage and the step control are done by every
+// lane of the team, in the one-thread kernel's order, so each matrix entry,
+// factor and solution component goes through the same operations as there.
+// A team's lanes hold identical scalars (t, h, err), so its control flow is
+// uniform; teams of one wavefront may diverge, which is why the kernel uses
+// no workgroup barrier.  The team's matrix sits in LDS; a team is part of one
+// wavefront, whose LDS accesses complete in order, so a wave barrier (for
+// the compiler) orders one lane's write before another lane's read.
+// ---------------------------------------------------------------------------
+constexpr int kTeam = 16;
+
+__device__ __forceinline__ void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int NS>
+__global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M, ReactorPool P,
+                                                     int max_tries, double rtol, double atol) {
+  constexpr int N = NS + 1;
+  constexpr int TPB = 64 / kTeam;
+  static_assert(N < kTeam, "a team needs a lane per column and one for the unperturbed state");
+  __shared__ double sA[TPB][N * N];
+  const int team = threadIdx.x / kTeam, lane = threadIdx.x % kTeam;
+  const int i = blockIdx.x * TPB + team;
+  if (i >= P.cap || P.status[i] != RX_RUN) return;  // the whole team leaves
+  double* a = sA[team];
+#define AT(r, c) a[(r) * N + (c)]
+  const double gam = 1.0 + 0.70710678118654752440;
+  double y[N], f0[N], f1[N], k1[N], k2[N], yt[N];
+  int piv[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) y[k] = P.y[(size_t)k * P.cap + i];
+  double t = P.t[i], h = P.h[i];
+  const double tend = P.tend[i], p = P.p[i];
+  int nstep = P.nstep[i], nrej = P.nrej[i], st = RX_RUN;
+  for (int it = 0; it < max_tries; it++) {
+    if (t >= tend) { st = RX_DONE; break; }
+    if (h > tend - t) h = tend - t;
+    if (!(h > 1e-18)) { st = RX_FAIL; break; }
+    // lane j < N: the state with component j perturbed; lane N (and the
+    // spare lanes): the state itself
+    const double d = 1.4901161193847656e-08 * fmax(fabs(pick<N>(y, lane)), lane == 0 ? 1.0 : 1e-8);
+#pragma unroll
+    for (int k = 0; k < N; k++) yt[k] = k == lane ? y[k] + d : y[k];
+    reactor_rhs_n<NS>(M, p, yt, f1);
+#pragma unroll
+    for (int k = 0; k < N; k++) f0[k] = __shfl(f1[k], N, kTeam);
+    if (lane < N) {
+#pragma unroll
+      for (int k = 0; k < N; k++) AT(k, lane) = -gam * h * (f1[k] - f0[k]) / d;
+      AT(lane, lane) += 1.0;
+    }
+    team_sync();
+    // LU with partial pivoting: rows swapped one column per lane, then
+    // eliminated one row per lane
+    bool ok = true;
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      int pk = k;
+      double m = fabs(AT(k, k));
+#pragma unroll
+      for (int r = k + 1; r < N; r++) {
+        const double v = fabs(AT(r, k));
+        if (v > m) { m = v; pk = r; }
+      }
+      piv[k] = pk;
+      ok = ok && (m > 0.0);
+      team_sync();
+      if (pk != k && lane < N) {
+        const double tmp = AT(k, lane);
+        AT(k, lane) = AT(pk, lane);
+        AT(pk, lane) = tmp;
+      }
+      team_sync();
+      const double inv = 1.0 / AT(k, k);
+      if (lane > k && lane < N) {
+        const double l = AT(lane, k) * inv;
+        AT(lane, k) = l;
+#pragma unroll
+        for (int c = k + 1; c < N; c++) AT(lane, c) -= l * AT(k, c);
+      }
+      team_sync();
+    }
+    double err = 2.0;
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < N; k++) k1[k] = f0[k];
+      lu_solve_n<N, 1>(a, piv, k1);
+#pragma unroll
+      for (int k = 0; k < N; k++) yt[k] = y[k] + h * k1[k];
+      reactor_rhs_n<NS>(M, p, yt, f1);
+#pragma unroll
+      for (int k = 0; k < N; k++) k2[k] = f1[k] - 2.0 * k1[k];
+      lu_solve_n<N, 1>(a, piv, k2);
+      err = 0.0;
+#pragma unroll
+      for (int k = 0; k < N; k++) {
+        const double yn = y[k] + 1.5 * h * k1[k] + 0.5 * h * k2[k];
+        const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
+        const double e = fabs(0.5 * h * (k1[k] + k2[k])) / sc;
+        if (!(e <= err)) err = e;
+        yt[k] = yn;
+      }
+    }
+    // the next step's matrix writes must follow every lane's solve reads
+    team_sync();
+    if (err <= 1.0) {
+      t += h;
+#pragma unroll
+      for (int k = 0; k < N; k++) y[k] = yt[k];
+      nstep++;
+      const double fac = 0.9 / sqrt(fmax(err, 1e-10));
+      h *= fmin(5.0, fmax(0.2, fac));
+    } else {
+      nrej++;
+      const double fac = err < 1e300 ? 0.9 / sqrt(err) : 0.2;
+      h *= fmax(0.2, fmin(0.9, fac));
+    }
+    if (t >= tend) { st = RX_DONE; break; }
+  }
+#undef AT
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < N; k++) P.y[(size_t)k * P.cap + i] = y[k];
+    P.t[i] = t;
+    P.h[i] = h;
+    P.nstep[i] = nstep;
+    P.nrej[i] = nrej;
+    P.status[i] = st;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -871,6 +1013,7 @@ struct astr_reactor_pool {
   void* base = nullptr;
   double rtol = 1e-6, atol = 1e-12;
   hipStream_t stream = nullptr;
+  int lanes = 1;  // lanes per reactor: 1, or kTeam for the team kernel
 };
 
 extern "C" {
@@ -1005,6 +1148,7 @@ int astr_hip_reactor_create(astr_chem* chem, int capacity, double rtol, double a
   r->P.nstep = iv + cap;
   r->P.nrej = iv + 2 * cap;
   r->P.cap = capacity;
+  r->lanes = chem->host.ns == 9 ? kTeam : 1;
   if (hipMemset(r->base, 0, bytes) != hipSuccess || hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking) != hipSuccess) {
     (void)hipFree(r->base);
     delete r;
@@ -1055,13 +1199,45 @@ int astr_hip_reactor_advance(astr_reactor_pool* r, int max_tries) {
   CHIP_OK(hipSetDevice(r->chem->device));
   const int bs = 64;
   const dim3 grid((r->P.cap + bs - 1) / bs);
-  if (r->chem->host.ns == 9)
+  if (r->lanes == kTeam)
+    hipLaunchKernelGGL(k_reactor_team<9>, dim3((r->P.cap + bs / kTeam - 1) / (bs / kTeam)), dim3(bs), 0,
+                       r->stream, r->chem->dev, r->P, max_tries, r->rtol, r->atol);
+  else if (r->chem->host.ns == 9)
     hipLaunchKernelGGL(k_reactor_n<9>, grid, dim3(bs), 0, r->stream, r->chem->dev, r->P, max_tries,
                        r->rtol, r->atol);
   else
     hipLaunchKernelGGL(k_reactor, grid, dim3(bs), 0, r->stream, r->chem->dev, r->P, max_tries,
                        r->rtol, r->atol);
   CHIP_OK(hipGetLastError());
+  return 0;
+}
+
+int astr_hip_reactor_set_lanes(astr_reactor_pool* r, int lanes) {
+  if (!r) return cfail("null pool");
+  if (lanes == kTeam && r->chem->host.ns != 9)
+    return cfail("the team kernel is built for mechanisms of 9 species");
+  if (lanes != 1 && lanes != kTeam) return cfail("lanes per reactor must be 1 or 16");
+  CHIP_OK(hipStreamSynchronize(r->stream));
+  r->lanes = lanes;
+  return 0;
+}
+
+int astr_hip_reactor_get_lanes(astr_reactor_pool* r, int* lanes) {
+  if (!r || !lanes) return cfail("null argument");
+  *lanes = r->lanes;
+  return 0;
+}
+
+int astr_hip_reactor_query(astr_reactor_pool* r, int* idle) {
+  if (!r || !idle) return cfail("null argument");
+  CHIP_OK(hipSetDevice(r->chem->device));
+  const hipError_t e = hipStreamQuery(r->stream);
+  if (e == hipErrorNotReady) {
+    *idle = 0;
+    return 0;
+  }
+  CHIP_OK(e);
+  *idle = 1;
   return 0;
 }
 

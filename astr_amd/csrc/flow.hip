@@ -1030,6 +1030,7 @@ struct astr_hip_batch {
   int stats_cap = 0;
   hipGraphExec_t graph = nullptr;
   bool graph_valid = false;
+  int pending = 0;  // steps of the last begin not yet collected by end
 };
 
 namespace {
@@ -1235,11 +1236,12 @@ int astr_hip_batch_destroy(astr_hip_batch* b) {
   return 0;
 }
 
-int astr_hip_batch_rk3_steps(astr_hip_batch* b, int nsteps, int use_graph, double* ke, double* ens) {
+int astr_hip_batch_rk3_begin(astr_hip_batch* b, int nsteps, int use_graph) {
   if (!b) return fail("null batch");
   if (nsteps < 0) return fail("negative step count");
   HIP_OK(hipSetDevice(b->device));
   if (batch_sync_cases(b)) return -1;
+  b->pending = nsteps;
   if (nsteps == 0) return 0;
   if (ensure_stats(b, nsteps) || upload_jobs(b) || refresh_args(b)) return -1;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, b->stream, b->d_step, 0);
@@ -1251,8 +1253,29 @@ int astr_hip_batch_rk3_steps(astr_hip_batch* b, int nsteps, int use_graph, doubl
     for (int it = 0; it < nsteps; it++)
       if (enqueue_step(b)) return -1;
   }
+  return 0;
+}
+
+int astr_hip_batch_query(astr_hip_batch* b, int* done) {
+  if (!b || !done) return fail("null argument");
+  HIP_OK(hipSetDevice(b->device));
+  const hipError_t e = hipStreamQuery(b->stream);
+  if (e == hipErrorNotReady) {
+    *done = 0;
+    return 0;
+  }
+  HIP_OK(e);
+  *done = 1;
+  return 0;
+}
+
+int astr_hip_batch_rk3_end(astr_hip_batch* b, double* ke, double* ens) {
+  if (!b) return fail("null batch");
+  HIP_OK(hipSetDevice(b->device));
+  const int nsteps = b->pending;
+  b->pending = 0;
   const int nc = (int)b->cases.size();
-  if (ke || ens) {
+  if (nsteps > 0 && (ke || ens)) {
     std::vector<double> h(2 * (size_t)nc * b->stats_cap);
     HIP_OK(hipMemcpyAsync(h.data(), b->d_stats, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
@@ -1264,6 +1287,11 @@ int astr_hip_batch_rk3_steps(astr_hip_batch* b, int nsteps, int use_graph, doubl
   }
   HIP_OK(hipStreamSynchronize(b->stream));
   return 0;
+}
+
+int astr_hip_batch_rk3_steps(astr_hip_batch* b, int nsteps, int use_graph, double* ke, double* ens) {
+  if (astr_hip_batch_rk3_begin(b, nsteps, use_graph)) return -1;
+  return astr_hip_batch_rk3_end(b, ke, ens);
 }
 
 int astr_hip_batch_autotune(astr_hip_batch* b, int reps) {

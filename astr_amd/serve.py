@@ -2,23 +2,23 @@
 
 Requests arrive over time and are of two kinds:
   - Ignition: a 0-D constant-pressure H2-air reactor integrated to t_end on
-    the GPU (astr_amd.chem.ReactorPool: one slot per request);
+    the GPU (astr_amd.chem.ReactorPool: one slot per request, a team of 16
+    lanes per reactor by default);
   - Vortex: a 2-D isentropic-vortex case advanced `steps` RK3 steps
     (astr_amd.Flow contexts advanced together by astr_amd.Batch).
 
-Each tick of the loop:
-  1. admits queued requests while slots and the HBM budget allow (a vortex
-     case costs its context's device memory; the reactor pool is allocated
-     once);
-  2. launches one reactor advance on the pool's stream;
-  3. while that runs, advances all admitted vortex cases by `steps_per_tick`
-     RK3 steps in one batch on the batch's stream (rebuilt only when the
-     membership changed);
-  4. retires what finished.
-
-Requests that arrive while others are on the GPU join at the next tick, so
-the device work of old and new requests overlaps.  A request's result does
-not depend on the mix:
+The two kinds run on their own streams and are tended independently by one
+host loop, which never waits on either:
+  - when the reactor pool's last advance has finished, the loop retires the
+    reactors that reached t_end, admits waiting ignition requests into free
+    slots and launches the next advance (`reactor_tries` step attempts);
+  - when the vortex batch's last `steps_per_tick` steps have finished, the
+    loop collects their KE histories, retires finished cases, rebuilds the
+    batch if its membership changed (new cases admitted under the HBM
+    budget) and enqueues the next steps.
+Requests that arrive while others are on the GPU join at the next launch of
+their kind, so the device work of old and new requests overlaps.  A
+request's result does not depend on the mix:
   - a reactor's steps depend only on its own state;
   - a vortex case runs the single-case pass body under its own launch
     selection.
@@ -86,9 +86,10 @@ def h2_air(names, W, phi):
 
 class Server:
     def __init__(self, mech_path, hbm_budget=200e9, reactor_slots=2048, rtol=1e-6, atol=1e-12,
-                 steps_per_tick=10, reactor_tries=100, h0=1e-10):
+                 steps_per_tick=10, reactor_tries=100, h0=1e-10, reactor_lanes=None):
         self.mech = AC.Mechanism(mech_path)
-        self.pool = AC.ReactorPool(self.mech, reactor_slots, rtol=rtol, atol=atol)
+        self.pool = AC.ReactorPool(self.mech, reactor_slots, rtol=rtol, atol=atol,
+                                   lanes=reactor_lanes)
         self.budget = float(hbm_budget)
         self.steps_per_tick = steps_per_tick
         self.reactor_tries = reactor_tries
@@ -99,11 +100,20 @@ class Server:
         self.vortices = []   # [request, flow, steps done, admitted, ke list]
         self.batch = None
         self.batch_members = None
-        self.ticks = 0
+        self.pool_busy = False
+        self.batch_busy = False
+        self.batch_k = 0
+        self.batch_cases = []
+        self.launches = {"reactor": 0, "vortex": 0}
         self.max_active = 0
+        self.max_vortex = 0
 
     def close(self):
+        if self.pool_busy:
+            self.pool.sync()
         if self.batch is not None:
+            if self.batch_busy:
+                self.batch.rk3_end(stats=False)
             self.batch.close()
         for v in self.vortices:
             v[1].close()
@@ -113,7 +123,8 @@ class Server:
     # ---- admission ---------------------------------------------------------
     def _admit(self, req, now):
         if isinstance(req, Ignition):
-            if not self.free_slots:
+            # setting a slot is ordered on the pool's stream: only between advances
+            if self.pool_busy or not self.free_slots:
                 return False
             slot = self.free_slots.pop()
             Y0 = h2_air(self.mech.names, self.mech.W, req.phi)
@@ -130,48 +141,69 @@ class Server:
         self.vortices.append([req, f, 0, now, []])
         return True
 
-    # ---- one tick ------------------------------------------------------------
-    def _tick(self, clock, results):
-        self.ticks += 1
-        if self.reactors:
-            self.pool.advance(self.reactor_tries)
-        if self.vortices:
-            members = tuple(id(v[1]) for v in self.vortices)
-            if members != self.batch_members:
-                if self.batch is not None:
-                    self.batch.close()
-                self.batch = A.Batch([v[1] for v in self.vortices])
-                self.batch_members = members
-            k = min(self.steps_per_tick, min(v[0].steps - v[2] for v in self.vortices))
-            ke, _ = self.batch.rk3(k, graph=True)
-            for i, v in enumerate(self.vortices):
-                v[2] += k
-                v[4].extend(ke[i].tolist())
-        done_vortex = [v for v in self.vortices if v[2] >= v[0].steps]
-        if self.reactors:
-            self.pool.sync()
-            st = self.pool.status()
+    # ---- reactors ----------------------------------------------------------
+    def _reactors_collect(self, clock, results):
+        """If the pool's advance finished: retire the reactors that are done."""
+        if not self.pool_busy or not self.pool.idle():
+            return False
+        self.pool_busy = False
+        st = self.pool.status()
         now = clock()
-        if self.reactors:
-            for slot in [s for s in self.reactors if st[s] in (AC.ReactorPool.DONE, AC.ReactorPool.FAILED)]:
-                req, adm = self.reactors.pop(slot)
-                t, T, Y, nstep, nrej = self.pool.get(slot)
-                self.pool.release(slot)
-                self.free_slots.append(slot)
-                results.append(Result(req.rid, "ignition", req.arrival, adm, now,
-                                      {"T": T, "Y": Y, "t": t, "steps": nstep, "rejected": nrej,
-                                       "ok": int(st[slot]) == AC.ReactorPool.DONE}))
-        for v in done_vortex:
-            req, f, nsteps, adm, ke = v
-            q = f.download()
+        for slot in [s for s in self.reactors if st[s] in (AC.ReactorPool.DONE, AC.ReactorPool.FAILED)]:
+            req, adm = self.reactors.pop(slot)
+            t, T, Y, nstep, nrej = self.pool.get(slot)
+            self.pool.release(slot)
+            self.free_slots.append(slot)
+            results.append(Result(req.rid, "ignition", req.arrival, adm, now,
+                                  {"T": T, "Y": Y, "t": t, "steps": nstep, "rejected": nrej,
+                                   "ok": int(st[slot]) == AC.ReactorPool.DONE}))
+        return True
+
+    def _reactors_launch(self):
+        if self.reactors and not self.pool_busy:
+            self.pool.advance(self.reactor_tries)
+            self.pool_busy = True
+            self.launches["reactor"] += 1
+
+    # ---- vortex cases -------------------------------------------------------
+    def _vortices_collect(self, clock, results):
+        """If the batch's steps finished: record them, retire finished cases."""
+        if not self.batch_busy or not self.batch.done():
+            return False
+        ke, _ = self.batch.rk3_end()
+        self.batch_busy = False
+        # cases admitted during these steps are not in the batch yet
+        for v, h in zip(self.batch_cases, ke):
+            v[2] += self.batch_k
+            v[4].extend(h.tolist())
+        now = clock()
+        for v in [v for v in self.vortices if v[2] >= v[0].steps]:
+            req, f, nsteps, adm, kh = v
             results.append(Result(req.rid, "vortex", req.arrival, adm, now,
-                                  {"q": q, "ke": np.array(ke), "steps": nsteps}))
+                                  {"q": f.download(), "ke": np.array(kh), "steps": nsteps}))
             self.vortices.remove(v)
             self.used_bytes -= case_bytes(f.cfg)
             f.close()
-        if done_vortex and not self.vortices and self.batch is not None:
+        if not self.vortices and self.batch is not None:
             self.batch.close()
             self.batch, self.batch_members = None, None
+        return True
+
+    def _vortices_launch(self):
+        if not self.vortices or self.batch_busy:
+            return
+        members = tuple(id(v[1]) for v in self.vortices)
+        if members != self.batch_members:
+            if self.batch is not None:
+                self.batch.close()
+            self.batch = A.Batch([v[1] for v in self.vortices])
+            self.batch_members = members
+        k = min(self.steps_per_tick, min(v[0].steps - v[2] for v in self.vortices))
+        self.batch_cases = list(self.vortices)
+        self.batch.rk3_begin(k, graph=True)
+        self.batch_k = k
+        self.batch_busy = True
+        self.launches["vortex"] += 1
 
     def run(self, requests):
         """Serve requests (each with .arrival seconds from the start, on the
@@ -187,32 +219,38 @@ class Server:
             return time.perf_counter() - t0
 
         i = 0
-        self.max_vortex = 0
         while i < len(pending) or queue or self.reactors or self.vortices:
             now = clock()
             while i < len(pending) and pending[i].arrival <= now:
                 queue.append(pending[i])
                 i += 1
+            moved = self._reactors_collect(clock, results)
+            moved = self._vortices_collect(clock, results) or moved
             waiting = []
             for req in queue:
                 if not self._admit(req, clock()):
                     waiting.append(req)
+            moved = moved or len(waiting) < len(queue)
             queue = waiting
+            self._reactors_launch()
+            self._vortices_launch()
             self.max_active = max(self.max_active, len(self.reactors) + len(self.vortices))
             self.max_vortex = max(self.max_vortex, len(self.vortices))
-            if self.reactors or self.vortices:
-                self._tick(clock, results)
-            elif i < len(pending):
-                time.sleep(max(0.0, min(pending[i].arrival - clock(), 0.01)))
+            if not moved:
+                # nothing finished and nothing admitted: wait a little for the
+                # GPU or the next arrival
+                nxt = pending[i].arrival - clock() if i < len(pending) else 0.01
+                time.sleep(min(max(nxt, 0.0), 2e-4 if (self.pool_busy or self.batch_busy) else 0.01))
         return results
 
 
-def run_alone(req, mech_path, rtol=1e-6, atol=1e-12, h0=1e-10, reactor_tries=100):
+def run_alone(req, mech_path, rtol=1e-6, atol=1e-12, h0=1e-10, reactor_tries=100,
+              reactor_lanes=None):
     """The plain sequential path for one request: its own reactor pool of one
     slot, or its own context stepped by astr_hip_rk3_steps."""
     if isinstance(req, Ignition):
         mech = AC.Mechanism(mech_path)
-        pool = AC.ReactorPool(mech, 1, rtol=rtol, atol=atol)
+        pool = AC.ReactorPool(mech, 1, rtol=rtol, atol=atol, lanes=reactor_lanes)
         pool.set(0, req.T0, req.p, h2_air(mech.names, mech.W, req.phi), req.t_end, h0)
         st = pool.run(reactor_tries)
         t, T, Y, nstep, nrej = pool.get(0)

@@ -143,13 +143,20 @@ int astr_hip_set_pass_config(astr_hip_ctx *ctx, int prog, int dir, int shape, in
  *   - astr_hip_batch_autotune picks one selection per class of cases (same
  *     program and set of fitting shapes) per pass, by timing, and writes it
  *     into the cases' own selections;
- *   - ke/ens (nullable) receive ncases x nsteps values, case-major.
+ *   - ke/ens (nullable) receive ncases x nsteps values, case-major;
+ *   - rk3_steps is rk3_begin (enqueue the steps on the batch's stream and
+ *     return) followed by rk3_end (wait, then copy the histories of those
+ *     steps); rk3_query sets *done to 1 once the enqueued steps finished, so
+ *     a host loop can tend other work meanwhile.
  * ------------------------------------------------------------------------ */
 typedef struct astr_hip_batch astr_hip_batch;
 int astr_hip_batch_create(astr_hip_ctx *const *cases, int ncases, astr_hip_batch **out);
 int astr_hip_batch_destroy(astr_hip_batch *batch);
 int astr_hip_batch_autotune(astr_hip_batch *batch, int reps);
 int astr_hip_batch_rk3_steps(astr_hip_batch *batch, int nsteps, int use_graph, double *ke, double *ens);
+int astr_hip_batch_rk3_begin(astr_hip_batch *batch, int nsteps, int use_graph);
+int astr_hip_batch_query(astr_hip_batch *batch, int *done);
+int astr_hip_batch_rk3_end(astr_hip_batch *batch, double *ke, double *ens);
 int astr_hip_batch_info(astr_hip_batch *batch, int *ngroups, int *launches_per_step);
 
 /* fds%central / compact_filter on a batch of lines: derivative.F90:171-198
@@ -202,13 +209,19 @@ int astr_hip_chem_temperature_device(astr_chem *chem, long long n, const double 
  *
  * The integrator: ROS2, the two-stage L-stable Rosenbrock method (Verwer et
  * al. 1999), with a finite-difference Jacobian and error control at rtol,
- * atol (mass fractions; temperature uses atol 1e-6 K).  One thread integrates
- * one reactor.
+ * atol (mass fractions; temperature uses atol 1e-6 K).
+ *
+ * Lanes per reactor (set_lanes / get_lanes): 1, one thread per reactor and 64
+ * reactors per wavefront, for throughput over many thousands of reactors; or
+ * 16, a team per reactor that evaluates the Jacobian's columns on separate
+ * lanes and eliminates one matrix row per lane, for latency when reactors are
+ * few (the default for 9-species mechanisms; other sizes use 1).
  *
  * A pool holds `capacity` slots:
  *   - set starts a reactor in a slot;
  *   - advance launches up to max_tries step attempts for every running slot
- *     (asynchronous, on the pool's own stream);
+ *     (asynchronous, on the pool's own stream); query sets *idle to 1 once
+ *     the pool's launches finished, sync waits for them;
  *   - status reads all slots: 0 free, 1 running, 2 done, 3 failed;
  *   - get reads one slot's t, (T, Y) and step counts;
  *   - release frees a slot.
@@ -222,6 +235,9 @@ int astr_hip_reactor_destroy(astr_reactor_pool *pool);
 int astr_hip_reactor_set(astr_reactor_pool *pool, int slot, double T0, double p, const double *Y0,
                          double t_end, double h0);
 int astr_hip_reactor_advance(astr_reactor_pool *pool, int max_tries);
+int astr_hip_reactor_set_lanes(astr_reactor_pool *pool, int lanes);
+int astr_hip_reactor_get_lanes(astr_reactor_pool *pool, int *lanes);
+int astr_hip_reactor_query(astr_reactor_pool *pool, int *idle);
 int astr_hip_reactor_sync(astr_reactor_pool *pool);
 int astr_hip_reactor_status(astr_reactor_pool *pool, int *status);
 int astr_hip_reactor_get(astr_reactor_pool *pool, int slot, double *t, double *TY, int *nstep, int *nrej);

@@ -216,13 +216,16 @@ def test_ignition_fixture_matches_oracle(mech):
 
 
 @pytest.mark.gpu
-def test_gpu_reactor_converges_to_radau(mech):
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_gpu_reactor_converges_to_radau(mech, lanes):
     """ROS2 on the GPU at rtol 1e-7 against scipy's Radau at 1e-10 (stored)
-    for lean, stoichiometric and rich mixtures at 1 and 5 atm, to t = 1 ms."""
+    for lean, stoichiometric and rich mixtures at 1 and 5 atm, to t = 1 ms,
+    with one thread and with a team of 16 lanes per reactor."""
     z = np.load(IGN)
     gm = AC.Mechanism(MECH)
     cases = [tuple(c) for c in z["cases"]]
-    pool = AC.ReactorPool(gm, len(cases), rtol=1e-7, atol=1e-13)
+    pool = AC.ReactorPool(gm, len(cases), rtol=1e-7, atol=1e-13, lanes=lanes)
+    assert pool.lanes == lanes
     for k, (T0, p, phi) in enumerate(cases):
         pool.set(k, T0, p, _mix(mech, phi), 1e-3)
     st = pool.run(max_tries=2000)
@@ -236,16 +239,40 @@ def test_gpu_reactor_converges_to_radau(mech):
 
 
 @pytest.mark.gpu
-def test_gpu_reactor_same_alone_and_in_a_mix(mech):
+def test_gpu_reactor_team_equals_one_thread(mech):
+    """The team kernel (16 lanes per reactor: Jacobian columns on separate
+    lanes, row-parallel elimination) takes the same steps and reaches
+    bit-identical states as the one-thread kernel; two slots stay free."""
+    gm = AC.Mechanism(MECH)
+    rng = np.random.default_rng(11)
+    cases = [(rng.uniform(1150.0, 1400.0), rng.uniform(1.0, 10.0) * 101325.0, rng.uniform(0.5, 2.0))
+             for _ in range(6)]
+    out = {}
+    for lanes in (1, 16):
+        pool = AC.ReactorPool(gm, 8, rtol=1e-6, atol=1e-12, lanes=lanes)
+        for k, (T0, p, phi) in enumerate(cases):
+            pool.set(k, T0, p, _mix(mech, phi), 2e-4)
+        st = pool.run(max_tries=500)
+        assert np.all(st[:6] == AC.ReactorPool.DONE) and np.all(st[6:] == AC.ReactorPool.FREE)
+        out[lanes] = [pool.get(k) for k in range(6)]
+        pool.close()
+    for a, b in zip(out[1], out[16]):
+        assert a[3] > 20
+        assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) and a[3:] == b[3:]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_gpu_reactor_same_alone_and_in_a_mix(mech, lanes):
     """A reactor's result is bit-identical alone, among 63 others, and split
     into launches of a different length."""
     gm = AC.Mechanism(MECH)
     rng = np.random.default_rng(4)
     target = (1100.0, 3 * 101325.0, _mix(mech, 0.8))
-    alone = AC.ReactorPool(gm, 1, rtol=1e-6, atol=1e-12)
+    alone = AC.ReactorPool(gm, 1, rtol=1e-6, atol=1e-12, lanes=lanes)
     alone.set(0, target[0], target[1], target[2], 1e-3)
     alone.run(max_tries=1000)
-    mixed = AC.ReactorPool(gm, 64, rtol=1e-6, atol=1e-12)
+    mixed = AC.ReactorPool(gm, 64, rtol=1e-6, atol=1e-12, lanes=lanes)
     for k in range(64):
         if k == 37:
             mixed.set(k, target[0], target[1], target[2], 1e-3)
