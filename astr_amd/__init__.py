@@ -46,7 +46,7 @@ EXPORTS = [
     "astr_hip_last_error", "astr_hip_create", "astr_hip_destroy", "astr_hip_set_stream",
     "astr_hip_set_deltat", "astr_hip_field_info", "astr_hip_init_tgv", "astr_hip_upload",
     "astr_hip_download", "astr_hip_filterq", "astr_hip_gradcal", "astr_hip_rhscal",
-    "astr_hip_rk3_steps", "astr_hip_sync", "astr_hip_line_batch",
+    "astr_hip_rk3_steps", "astr_hip_rk3_steps_stats", "astr_hip_sync", "astr_hip_line_batch",
     "astr_hip_profile_begin", "astr_hip_profile_end", "astr_hip_autotune",
     "astr_hip_get_pass_config", "astr_hip_set_pass_config", "astr_hip_init_vortex",
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
@@ -83,6 +83,7 @@ def lib():
         L.astr_hip_gradcal.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_rhscal.argtypes = [C.c_void_p, _dp]
         L.astr_hip_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.astr_hip_rk3_steps_stats.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
                                           C.c_double, C.c_double]
@@ -194,11 +195,16 @@ class Flow:
         _chk(lib().astr_hip_rhscal(self.h, _p(qrhs)))
         return qrhs
 
-    def rk3(self, nsteps, stats=True):
+    def rk3(self, nsteps, stats=True, dissipation=False):
+        """nsteps RK3 steps; (ke, ens) per step, or (ke, ens, dissipation)
+        with dissipation=True, or None with stats=False."""
         ke = np.empty(nsteps) if stats else None
         en = np.empty(nsteps) if stats else None
-        _chk(lib().astr_hip_rk3_steps(self.h, nsteps, _p(ke), _p(en)))
-        return (ke, en) if stats else None
+        di = np.empty(nsteps) if stats and dissipation else None
+        _chk(lib().astr_hip_rk3_steps_stats(self.h, nsteps, _p(ke), _p(en), _p(di)))
+        if not stats:
+            return None
+        return (ke, en, di) if dissipation else (ke, en)
 
     steploop = rk3
 

@@ -67,29 +67,106 @@ struct DevBuf {
       return fail(std::string(#x) + ": " + hipGetErrorString(e_));                 \
   } while (0)
 
-// sum the per-tile stats in a fixed order (one block of 256 threads)
-__global__ __launch_bounds__(256) void k_stats_reduce(const double* __restrict__ part, int nt,
-                                                      double* __restrict__ out, double scale) {
-  __shared__ double sa[256], sb[256];
-  double x = 0.0, y = 0.0;
-  for (int i = threadIdx.x; i < nt; i += 256) {
-    x += part[2 * i];
-    y += part[2 * i + 1];
-  }
-  sa[threadIdx.x] = x;
-  sb[threadIdx.x] = y;
+// Per-tile partial sums left by the last gradient pass: rho|u|^2, rho|w|^2.
+constexpr int kTileStats = 2;
+// Statistics per step of a single case: KE, enstrophy, dissipation rate.
+constexpr int kStepStats = 3;
+// Statistics per step of a batched case: KE, enstrophy.
+constexpr int kBatchStats = 2;
+
+// Sum the per-tile stats in a fixed order (one block of 256 threads) into
+// KE = sum rho|u|^2 / 2N and enstrophy = sum rho|w|^2 / 2N (stacal,
+// tgvsolver.F90:930-932), N = im*jm*km, scale = 1/N.
+__device__ __forceinline__ void stats_sum(const double* __restrict__ part, int nt,
+                                          double* __restrict__ out, double scale) {
+  __shared__ double sv[kTileStats][256];
+  double x[kTileStats] = {0.0, 0.0};
+  for (int i = threadIdx.x; i < nt; i += 256)
+#pragma unroll
+    for (int q = 0; q < kTileStats; q++) x[q] += part[kTileStats * i + q];
+#pragma unroll
+  for (int q = 0; q < kTileStats; q++) sv[q][threadIdx.x] = x[q];
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      sa[threadIdx.x] += sa[threadIdx.x + h];
-      sb[threadIdx.x] += sb[threadIdx.x + h];
-    }
+    if ((int)threadIdx.x < h)
+#pragma unroll
+      for (int q = 0; q < kTileStats; q++) sv[q][threadIdx.x] += sv[q][threadIdx.x + h];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    out[0] = 0.5 * sa[0] * scale;
-    out[1] = 0.5 * sb[0] * scale;
+    out[0] = 0.5 * sv[0][0] * scale;
+    out[1] = 0.5 * sv[1][0] * scale;
   }
+}
+
+// fixed-order sum of 256 per-thread values of a block into v[0]
+__device__ __forceinline__ double block_sum256(double x) {
+  __shared__ double sv[256];
+  sv[threadIdx.x] = x;
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sv[threadIdx.x] += sv[threadIdx.x + h];
+    __syncthreads();
+  }
+  return sv[0];
+}
+
+// Dissipation rate (diss_rate_cal, statistic.F90:994-1043): partial sums of
+// 2 mu (s11^2+s22^2+s33^2 + 2(s12^2+s13^2+s23^2) - div^2/3), mu =
+// miucal(T)/Re, over the points i,j,k = 1..im,jm,km (k = 0 in 2-D), from the
+// stored velocity gradients dvel(m,d) and the temperature of q.  Block b
+// sums the points b*256+t, b*256+t+256*nb, ... per thread, then in a fixed
+// tree; k_diss_sum adds the nb block sums in a fixed tree.
+constexpr int kDissBlocks = 1024;
+
+// one point's term, in the reference's operation order (statistic.F90:1015-1031)
+__device__ __forceinline__ double diss_point(const FlowArgs& a, double t, const double (&g)[3][3]) {
+#pragma clang fp contract(off)
+  const double miu = miucal(a, t) / a.reynolds;
+  const double s11 = g[0][0], s22 = g[1][1], s33 = g[2][2];
+  const double s12 = 0.5 * (g[0][1] + g[1][0]);
+  const double s13 = 0.5 * (g[0][2] + g[2][0]);
+  const double s23 = 0.5 * (g[1][2] + g[2][1]);
+  const double div = s11 + s22 + s33;
+  return 2.0 * miu *
+         (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23) -
+          (1.0 / 3.0) * (div * div));
+}
+
+__global__ __launch_bounds__(256) void k_diss_part(FlowArgs a, double* __restrict__ part) {
+  const int im = a.n[0], jm = a.n[1], km = a.n[2];
+  const long long npts = (long long)im * jm * (km > 0 ? km : 1);
+  const long long fs = a.fs;
+  double acc = 0.0;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < npts; e += (long long)gridDim.x * 256) {
+    const int i = 1 + (int)(e % im);
+    const long long r = e / im;
+    const int j = 1 + (int)(r % jm);
+    const int k = km > 0 ? 1 + (int)(r / jm) : 0;
+    const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+    const Prim P = prim_of(a, a.q[o], a.q[fs + o], a.q[2 * fs + o], a.q[3 * fs + o], a.q[4 * fs + o]);
+    double g[3][3];
+#pragma unroll
+    for (int m = 0; m < 3; m++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) g[m][d] = d < a.ndir ? a.dv[(m + 3 * d) * fs + o] : 0.0;
+    acc += diss_point(a, P.t, g);
+  }
+  const double b = block_sum256(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = b;
+}
+
+__global__ __launch_bounds__(256) void k_diss_sum(const double* __restrict__ part, int nb,
+                                                  double* __restrict__ out, double scale) {
+  double x = 0.0;
+  for (int i = threadIdx.x; i < nb; i += 256) x += part[i];
+  const double b = block_sum256(x);
+  if (threadIdx.x == 0) out[0] = b * scale;
+}
+
+__global__ __launch_bounds__(256) void k_stats_reduce(const double* __restrict__ part, int nt,
+                                                      double* __restrict__ out, double scale) {
+  stats_sum(part, nt, out, scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -357,6 +434,7 @@ struct astr_hip_ctx {
   double* base = nullptr;   // one allocation for all fields
   double* coef = nullptr;   // LU tables
   double* stats_part = nullptr;
+  double* diss_part = nullptr;   // inside stats_part's allocation
   double* stats_out = nullptr;  // [2*max steps] device
   int stats_cap = 0;
   hipStream_t stream = nullptr;
@@ -458,11 +536,15 @@ int pass(astr_hip_ctx* c, int prog, int d) {
   return 0;
 }
 
-int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad) {
+// stats: 0 none, 1 KE/enstrophy, 2 also the dissipation rate
+// diss_out (device, nullable): the dissipation rate of the current state,
+// evaluated after the gradient passes (which then store every gradient) and
+// before the viscous passes update q
+int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* diss_out = nullptr) {
   FlowArgs& a = c->a;
   const int nd = a.ndir;
   a.dostats = stats ? 1 : 0;
-  a.store_last_grad = store_last_grad ? 1 : 0;
+  a.store_last_grad = store_last_grad || diss_out ? 1 : 0;
   for (int d = 0; d < nd; d++) {
     if (d == nd - 1 && c->split_last) {
       if (pass(c, GL_GRAD, d) || pass(c, GL_CONV, d)) return -1;
@@ -470,6 +552,14 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad) {
     }
     const int prog = d == nd - 1 ? GC_LAST : (d == 0 ? GC_FIRST : GC_MID);
     if (pass(c, prog, d)) return -1;
+  }
+  if (diss_out) {
+    const long long npts = (long long)a.n[0] * a.n[1] * (a.n[2] > 0 ? a.n[2] : 1);
+    const int nb = (int)std::min<long long>(kDissBlocks, (npts + 255) / 256);
+    hipLaunchKernelGGL(k_diss_part, dim3(nb), dim3(256), 0, c->stream, a, c->diss_part);
+    HIP_OK(hipGetLastError());
+    hipLaunchKernelGGL(k_diss_sum, dim3(1), dim3(256), 0, c->stream, c->diss_part, nb, diss_out, c->ke_scale);
+    HIP_OK(hipGetLastError());
   }
   a.dostats = 0;
   for (int d = 0; d < nd; d++) {
@@ -571,11 +661,13 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
     a.fac[d] = c->coef + off[3 + d];
   }
   // per-tile stats slots for the smallest tile any shape uses
+  // (then the dissipation rate's block partials)
   const int ntl = tiles_of(a.n, a.ndir - 1, 4);
-  if (hipMalloc(&c->stats_part, 2 * (size_t)ntl * sizeof(double)) != hipSuccess) {
+  if (hipMalloc(&c->stats_part, (kTileStats * (size_t)ntl + kDissBlocks) * sizeof(double)) != hipSuccess) {
     (void)hipFree(c->base); (void)hipFree(c->coef); delete c; return fail("hipMalloc of stats failed");
   }
   a.stats = c->stats_part;
+  c->diss_part = c->stats_part + kTileStats * (size_t)ntl;
   c->stream = nullptr;
   *out = c;
   return 0;
@@ -695,14 +787,18 @@ int astr_hip_rhscal(astr_hip_ctx* c, double* qrhs) {
 }
 
 int astr_hip_rk3_steps(astr_hip_ctx* c, int nsteps, double* ke, double* ens) {
+  return astr_hip_rk3_steps_stats(c, nsteps, ke, ens, nullptr);
+}
+
+int astr_hip_rk3_steps_stats(astr_hip_ctx* c, int nsteps, double* ke, double* ens, double* diss) {
   if (check_ctx(c)) return -1;
   if (nsteps < 0) return fail("negative step count");
   static const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
-  const bool want = ke || ens;
+  const bool want = ke || ens || diss;
   if (want && c->stats_cap < nsteps) {
     if (c->stats_out) (void)hipFree(c->stats_out);
     c->stats_out = nullptr;
-    HIP_OK(hipMalloc(&c->stats_out, 2 * (size_t)nsteps * sizeof(double)));
+    HIP_OK(hipMalloc(&c->stats_out, kStepStats * (size_t)nsteps * sizeof(double)));
     c->stats_cap = nsteps;
   }
   FlowArgs& a = c->a;
@@ -712,10 +808,12 @@ int astr_hip_rk3_steps(astr_hip_ctx* c, int nsteps, double* ke, double* ens) {
     for (int s = 0; s < 3; s++) {
       a.stage = s;
       a.rk_a = rk[s][0]; a.rk_b = rk[s][1]; a.rk_c = rk[s][2];
-      if (run_rhs(c, true, want && s == 0, false)) return -1;
+      if (run_rhs(c, true, want && s == 0, false,
+                  diss && s == 0 ? c->stats_out + kStepStats * it + 2 : nullptr))
+        return -1;
       if (want && s == 0) {
         hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, c->stream, c->stats_part, ntl,
-                           c->stats_out + 2 * it, c->ke_scale);
+                           c->stats_out + kStepStats * it, c->ke_scale);
         HIP_OK(hipGetLastError());
       }
       const int nd = a.ndir;
@@ -724,12 +822,13 @@ int astr_hip_rk3_steps(astr_hip_ctx* c, int nsteps, double* ke, double* ens) {
     }
   }
   if (want) {
-    std::vector<double> h(2 * (size_t)nsteps);
+    std::vector<double> h(kStepStats * (size_t)nsteps);
     HIP_OK(hipMemcpyAsync(h.data(), c->stats_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIP_OK(hipStreamSynchronize(c->stream));
     for (int it = 0; it < nsteps; it++) {
-      if (ke) ke[it] = h[2 * it];
-      if (ens) ens[it] = h[2 * it + 1];
+      if (ke) ke[it] = h[kStepStats * it];
+      if (ens) ens[it] = h[kStepStats * it + 1];
+      if (diss) diss[it] = h[kStepStats * it + 2];
     }
   }
   return 0;
@@ -953,34 +1052,14 @@ struct StatsJob {
   const double* part;
   int ntiles;
   double scale;
-  double* out;  // 2 doubles per step
+  double* out;  // kBatchStats doubles per step
 };
 
-// per-case KE / enstrophy of the current step (index *step), one block per case
+// per-case statistics of the current step (index *step), one block per case
 __global__ __launch_bounds__(256) void k_stats_batch(const StatsJob* __restrict__ jobs,
                                                      const int* __restrict__ step) {
-  __shared__ double sa[256], sb[256];
   const StatsJob J = jobs[blockIdx.x];
-  double x = 0.0, y = 0.0;
-  for (int i = threadIdx.x; i < J.ntiles; i += 256) {
-    x += J.part[2 * i];
-    y += J.part[2 * i + 1];
-  }
-  sa[threadIdx.x] = x;
-  sb[threadIdx.x] = y;
-  __syncthreads();
-  for (int h = 128; h > 0; h >>= 1) {
-    if ((int)threadIdx.x < h) {
-      sa[threadIdx.x] += sa[threadIdx.x + h];
-      sb[threadIdx.x] += sb[threadIdx.x + h];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) {
-    const int s = *step;
-    J.out[2 * s] = 0.5 * sa[0] * J.scale;
-    J.out[2 * s + 1] = 0.5 * sb[0] * J.scale;
-  }
+  stats_sum(J.part, J.ntiles, J.out + kBatchStats * *step, J.scale);
 }
 
 __global__ void k_set_int(int* p, int v) { *p = v; }
@@ -1121,7 +1200,7 @@ int ensure_stats(astr_hip_batch* b, int cap) {
   b->d_stats = nullptr;
   b->d_jobs = nullptr;
   const int nc = (int)b->cases.size();
-  HIP_OK(hipMalloc(&b->d_stats, 2 * (size_t)nc * cap * sizeof(double)));
+  HIP_OK(hipMalloc(&b->d_stats, kBatchStats * (size_t)nc * cap * sizeof(double)));
   HIP_OK(hipMalloc(&b->d_jobs, nc * sizeof(StatsJob)));
   b->stats_cap = cap;
   return 0;
@@ -1136,7 +1215,7 @@ int upload_jobs(astr_hip_batch* b) {
     jobs[i].part = c->stats_part;
     jobs[i].ntiles = tiles_of(c->a.n, nd - 1, kShapes[c->shape[GC_LAST][nd - 1]].TL);
     jobs[i].scale = c->ke_scale;
-    jobs[i].out = b->d_stats + 2 * (size_t)i * b->stats_cap;
+    jobs[i].out = b->d_stats + kBatchStats * (size_t)i * b->stats_cap;
   }
   HIP_OK(hipMemcpy(b->d_jobs, jobs.data(), nc * sizeof(StatsJob), hipMemcpyHostToDevice));
   return 0;
@@ -1276,13 +1355,14 @@ int astr_hip_batch_rk3_end(astr_hip_batch* b, double* ke, double* ens) {
   b->pending = 0;
   const int nc = (int)b->cases.size();
   if (nsteps > 0 && (ke || ens)) {
-    std::vector<double> h(2 * (size_t)nc * b->stats_cap);
+    std::vector<double> h(kBatchStats * (size_t)nc * b->stats_cap);
     HIP_OK(hipMemcpyAsync(h.data(), b->d_stats, h.size() * sizeof(double), hipMemcpyDeviceToHost, b->stream));
     HIP_OK(hipStreamSynchronize(b->stream));
     for (int i = 0; i < nc; i++)
       for (int it = 0; it < nsteps; it++) {
-        if (ke) ke[(size_t)i * nsteps + it] = h[2 * ((size_t)i * b->stats_cap + it)];
-        if (ens) ens[(size_t)i * nsteps + it] = h[2 * ((size_t)i * b->stats_cap + it) + 1];
+        const double* v = &h[kBatchStats * ((size_t)i * b->stats_cap + it)];
+        if (ke) ke[(size_t)i * nsteps + it] = v[0];
+        if (ens) ens[(size_t)i * nsteps + it] = v[1];
       }
   }
   HIP_OK(hipStreamSynchronize(b->stream));

@@ -27,7 +27,7 @@ struct FlowArgs {
   int trash;            // a point index inside every field's padding
   double alfa, bhalo;   // filter alpha and halo-row beta (LHS of the filter rows)
   int stage;            // 0,1,2
-  int dostats;          // stage-1 KE/enstrophy partial sums
+  int dostats;          // stage-1 KE/enstrophy partial sums (1) or none (0)
   int store_last_grad;  // last GC pass also stores its gradients
   double* q;            // 5 fields
   double* qsave;        // 5 fields
@@ -240,7 +240,7 @@ __device__ __forceinline__ double miucal(const FlowArgs& a, double t) {
 // what changes from stage to stage (the same for every case of a batch)
 struct StageArgs {
   int stage;            // 0,1,2
-  int dostats;          // stage-1 KE/enstrophy partial sums
+  int dostats;          // stage-1 KE/enstrophy partial sums (1) or none (0)
   int store_last_grad;  // last GC pass also stores its gradients
   double rk_a, rk_b, rk_c;
 };

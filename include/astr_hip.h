@@ -98,6 +98,11 @@ int astr_hip_rhscal(astr_hip_ctx *ctx, double *qrhs);
 /* steploop/rk3: tgvsolver.F90:944-960, :994-1082.  ke/ens (nullable) receive
  * stacal's kinetic energy and enstrophy (:893-942) at the start of each step. */
 int astr_hip_rk3_steps(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens);
+/* rk3_steps with the main path's statcal statistics of each step's first
+ * stage (statistic.F90:593-595): ke and ens as above, and diss[it] =
+ * sum 2 mu (s:s - div^2/3) / (im*jm*km) over i,j,k = 1..im,jm,km
+ * (diss_rate_cal, statistic.F90:994-1043).  Any of the three may be NULL. */
+int astr_hip_rk3_steps_stats(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens, double *diss);
 int astr_hip_sync(astr_hip_ctx *ctx);
 
 /* Per-pass timing with HIP events on the context's stream (no reference

@@ -627,8 +627,48 @@ void or_stacal(or_state *s, double *ke, double *ens) {
   *ens = (double)(0.5L * enst / cnt);
 }
 
-/* rk3, tgvsolver.F90:994-1082 */
-void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens) {
+/* diss_rate_cal, statistic.F90:994-1043: sum over i,j,k=1..n of
+ * 2 miu (s11^2+s22^2+s33^2 + 2(s12^2+s13^2+s23^2) - div^2/3), miu =
+ * miucal(tmp)/reynolds (the nondimensional branch, :1015-1016), divided by
+ * im*jm*km (:1037).  Summed like or_stacal. */
+void or_diss_rate(or_state *s, double *diss) {
+  long N = s->N;
+  int im = s->n[0], jm = s->n[1], km = s->n[2];
+  int kmax = km > 0 ? km : 0, kmin = km > 0 ? 1 : 0;
+  long double tot = 0.0L;
+  long double *pd = calloc(kmax + 1, sizeof(long double));
+#pragma omp parallel for
+  for (int k = kmin; k <= kmax; k++) {
+    long double a = 0.0L;
+    for (int j = 1; j <= jm; j++)
+      for (int i = 1; i <= im; i++) {
+        long e = i + s->st[1] * j + s->st[2] * k;
+        const double *dv = s->dvel;
+#define DV(m, n_) dv[((m) + 3 * (n_)) * N + e]
+        double miu = miucal(s, s->prims[5 * N + e]) / s->c.reynolds;
+        double s11 = DV(0, 0);
+        double s12 = 0.5 * (DV(0, 1) + DV(1, 0));
+        double s13 = 0.5 * (DV(0, 2) + DV(2, 0));
+        double s22 = DV(1, 1);
+        double s23 = 0.5 * (DV(1, 2) + DV(2, 1));
+        double s33 = DV(2, 2);
+#undef DV
+        double div = s11 + s22 + s33;
+        a += 2.0 * miu * (s11 * s11 + s22 * s22 + s33 * s33 + 2.0 * (s12 * s12 + s13 * s13 + s23 * s23) -
+                          num1d3 * (div * div));
+      }
+    pd[k] = a;
+  }
+  for (int k = kmin; k <= kmax; k++) tot += pd[k];
+  free(pd);
+  double cnt = (double)im * (double)jm * (double)(km > 0 ? km : 1);
+  *diss = (double)(tot / cnt);
+}
+
+/* rk3, tgvsolver.F90:994-1082, with statcal's three statistics of each
+ * step's first stage (KE and enstrophy as stacal, the dissipation rate as
+ * diss_rate_cal) */
+void or_rk3_steps3(or_state *s, int nsteps, double *ke, double *ens, double *diss) {
   static const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
   long N = s->N;
   for (int it = 0; it < nsteps; it++) {
@@ -639,6 +679,7 @@ void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens) {
         or_stacal(s, &a, &b);
         if (ke) ke[it] = a;
         if (ens) ens[it] = b;
+        if (diss) or_diss_rate(s, &diss[it]);
         memcpy(s->qsave, s->q, sizeof(double) * 5 * N);
       }
       const double c0 = rk[st][0], c1 = rk[st][1], c2 = rk[st][2], dt = s->c.dt;
@@ -650,4 +691,9 @@ void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens) {
     }
     s->nstep++;
   }
+}
+
+/* rk3, tgvsolver.F90:994-1082 */
+void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens) {
+  or_rk3_steps3(s, nsteps, ke, ens, NULL);
 }

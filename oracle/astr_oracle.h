@@ -68,6 +68,8 @@ void or_rhscal(or_state *s);
 void or_filterq(or_state *s);
 void or_stacal(or_state *s, double *ke, double *ens);
 void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens);
+void or_diss_rate(or_state *s, double *diss);
+void or_rk3_steps3(or_state *s, int nsteps, double *ke, double *ens, double *diss);
 
 /* counter-based noise shared with the device path (not a reference routine) */
 double or_gauss(uint64_t seed, uint32_t i, uint32_t j, uint32_t k, uint32_t c);

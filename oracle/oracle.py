@@ -61,6 +61,8 @@ def lib():
         L.or_filterq.argtypes = [C.c_void_p]
         L.or_stacal.argtypes = [C.c_void_p, _dp, _dp]
         L.or_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
+        L.or_rk3_steps3.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.or_diss_rate.argtypes = [C.c_void_p, _dp]
         L.or_df_compact.argtypes = [C.c_int, _dp, _dp]
         L.or_compact_filter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
         L.or_gauss.restype = C.c_double
@@ -76,7 +78,7 @@ def lib():
 
 
 def _p(a):
-    return a.ctypes.data_as(_dp)
+    return a.ctypes.data_as(_dp) if a is not None else None
 
 
 def tgv_cfg(im, jm=None, km=None, dt=1e-3, reynolds=1600.0, mach=0.1, suth_dim=110.4,
@@ -147,11 +149,19 @@ class OracleState:
         lib().or_stacal(self.h, _p(a), _p(b))
         return a[0], b[0]
 
-    def rk3(self, nsteps):
+    def diss_rate(self):
+        """diss_rate_cal (statistic.F90:994-1043) from the current dvel and
+        temperature (run rhscal first)."""
+        d = np.zeros(1)
+        lib().or_diss_rate(self.h, _p(d))
+        return d[0]
+
+    def rk3(self, nsteps, dissipation=False):
         ke = np.zeros(nsteps)
         en = np.zeros(nsteps)
-        lib().or_rk3_steps(self.h, nsteps, _p(ke), _p(en))
-        return ke, en
+        di = np.zeros(nsteps) if dissipation else None
+        lib().or_rk3_steps3(self.h, nsteps, _p(ke), _p(en), _p(di))
+        return (ke, en, di) if dissipation else (ke, en)
 
 
 def df_compact(f):

@@ -128,6 +128,38 @@ def test_rk3_steps(dims):
     assert max(err) < 1e-12, err
 
 
+@pytest.mark.parametrize("dims", [(32, 32, 32), (36, 28, 24), (40, 36, 0)])
+def test_dissipation_history_matches_oracle(dims):
+    """astr_hip_rk3_steps_stats: the main path's statcal statistics (KE,
+    enstrophy, dissipation rate of diss_rate_cal) of each step, seeded noisy
+    field, against the oracle."""
+    f, s = make_pair(*dims, dt=2e-3)
+    ke, en, di = f.rk3(5, dissipation=True)
+    ke0, en0, di0 = s.rk3(5, dissipation=True)
+    assert relmax(ke, ke0) < HIST_TOL and relmax(en, en0) < HIST_TOL
+    assert relmax(di, di0) < HIST_TOL
+    assert np.all(di > 0)
+
+
+def test_dissipation_tgv_closed_form():
+    """Taylor-Green velocity of amplitude A = 0.1 with rho = 1 and p = p_inf
+    (so T = 1 and mu = 1/Re everywhere): the first step's dissipation rate is
+    3A^2/4 mu, to the 643c derivative's error at 32^3 (5e-8)."""
+    n, amp = 32, 0.1
+    cfg = A.tgv_config(n, deltat=1e-3)
+    f = A.Flow(cfg)
+    x = np.arange(n + 1) * (2 * np.pi / n)
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u = amp * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -amp * np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = 0.0 * X
+    p0 = 1.0 / (cfg.gamma * cfg.mach ** 2)
+    f.upload(np.stack([np.ones_like(X), u, v, w, p0 / (cfg.gamma - 1.0) + 0.5 * (u * u + v * v)]))
+    ke, en, di = f.rk3(1, dissipation=True)
+    assert abs(di[0] * cfg.reynolds / (0.75 * amp ** 2) - 1.0) < 1e-7
+    assert abs(ke[0] / (0.125 * amp ** 2) - 1.0) < 1e-13
+
+
 def test_state_ref_128():
     """The reference-held history miniapps/tgv_solver_3d/state.ref_128:
     128^3, dt=1e-3, 100 steps, KE and enstrophy at the start of each step."""

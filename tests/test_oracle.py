@@ -215,6 +215,47 @@ def test_tgv_initial_kinetic_energy():
     assert abs(en - 0.375) < 1e-3   # -> 3/8 as the grid is refined
 
 
+def _diss_for(n, field, amp=0.1):
+    """diss_rate_cal of the oracle on an n^3 periodic lattice with rho = 1,
+    p = p_inf (so T = 1 and mu = miucal(1)/Re = 1/Re) and velocity
+    amp * field(x, y, z); returns dissipation * Re."""
+    cfg = O.tgv_cfg(n)
+    s = O.OracleState(cfg)
+    x = np.arange(n + 1) * cfg.dx
+    Z, Y, X = np.meshgrid(x, x, x, indexing="ij")
+    u, v, w = (amp * c for c in field(X, Y, Z))
+    g, ma = cfg.gamma, cfg.mach
+    p0 = 1.0 / (g * ma * ma)
+    q = np.stack([np.ones_like(X), u, v, w, p0 / (g - 1.0) + 0.5 * (u * u + v * v + w * w)])
+    s.set_q(q)
+    s.rhscal()
+    assert np.abs(s.prims[5] - 1.0).max() < 1e-12
+    return s.diss_rate() * cfg.reynolds
+
+
+def test_dissipation_closed_forms():
+    """diss_rate_cal (statistic.F90:994-1043) on fields with closed forms
+    (<.> over the periodic box, mu = 1/Re):
+      - Taylor-Green u = A sin x cos y cos z, v = -A cos x sin y cos z:
+        s11 = -s22, s13, s23 nonzero, div = 0, <s:s> = 3A^2/8 -> 3A^2/4 mu;
+      - u = A sin x (pure compression): s11 = div, s11^2 - div^2/3 ->
+        2A^2/3 mu (pins the divergence term);
+      - u = A sin y (pure shear): 2 s12^2 = A^2 cos^2 y / 2 -> A^2/2 mu (pins the
+        factor 2 and the 1/2 of the off-diagonal rates).
+    The compact derivative is 6th order, so the error drops 64x per halving
+    of h."""
+    tgv = lambda X, Y, Z: (np.sin(X) * np.cos(Y) * np.cos(Z), -np.cos(X) * np.sin(Y) * np.cos(Z),
+                           0.0 * X)
+    comp = lambda X, Y, Z: (np.sin(X), 0.0 * X, 0.0 * X)
+    shear = lambda X, Y, Z: (np.sin(Y), 0.0 * X, 0.0 * X)
+    A2 = 0.1 ** 2
+    for field, exact in ((tgv, 0.75 * A2), (comp, 2.0 / 3.0 * A2), (shear, 0.5 * A2)):
+        e16 = abs(_diss_for(16, field) / exact - 1.0)
+        e32 = abs(_diss_for(32, field) / exact - 1.0)
+        assert e32 < 1e-7, (exact, e16, e32)
+        assert e32 < e16 / 40.0 or e32 < 1e-13, (exact, e16, e32)
+
+
 def test_state_ref_128_first_steps():
     """First 6 lines of the reference-held history state.ref_128 (128^3)."""
     ref = np.loadtxt(os.path.join(GOLD, "state_ref_128.txt"))
