@@ -102,6 +102,16 @@ module astr_hip_capi
       type(c_ptr), value :: ke, ens
       integer(c_int) :: astr_hip_rk3_steps
     end function
+    ! rk3_steps with the main path's statcal statistics (kenergycal,
+    ! enstophycal, diss_rate_cal, statistic.F90:593-595) of each step
+    function astr_hip_rk3_steps_stats(ctx, nsteps, ke, ens, diss) &
+        bind(C, name='astr_hip_rk3_steps_stats')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nsteps
+      type(c_ptr), value :: ke, ens, diss
+      integer(c_int) :: astr_hip_rk3_steps_stats
+    end function
     function astr_hip_sync(ctx) bind(C, name='astr_hip_sync')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
