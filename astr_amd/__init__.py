@@ -52,6 +52,7 @@ EXPORTS = [
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
     "astr_hip_batch_rk3_steps", "astr_hip_batch_info", "astr_hip_set_last_split",
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
+    "astr_hip_gridgeom", "astr_hip_set_metrics",
     "astr_hip_get_last_split",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
@@ -84,6 +85,8 @@ def lib():
         L.astr_hip_rhscal.argtypes = [C.c_void_p, _dp]
         L.astr_hip_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.astr_hip_rk3_steps_stats.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.astr_hip_gridgeom.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
                                           C.c_double, C.c_double]
@@ -207,6 +210,26 @@ class Flow:
         return (ke, en, di) if dissipation else (ke, en)
 
     steploop = rk3
+
+    # -- grid metrics (gridgeom, geom.F90:99-692) ------------------------------
+    def gridgeom(self, x):
+        """Metrics of the coordinates x (3, km+1, jm+1, im+1), computed on the
+        device: (jacob (km+1, jm+1, im+1), dxi (3, 3, km+1, jm+1, im+1)),
+        dxi[a, b] = d xi_a / d x_b."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        if x.shape != (3,) + self.shape:
+            raise AstrError(f"x must have shape {(3,) + self.shape}")
+        jac = np.empty(self.shape)
+        buf = np.empty((3, 3) + self.shape)   # [b, a]: the C layout (a-1) + 3(b-1)
+        _chk(lib().astr_hip_gridgeom(self.h, _p(x), _p(jac), _p(buf)))
+        return jac, buf.transpose((1, 0, 2, 3, 4))
+
+    def set_metrics(self, jac, dxi):
+        """Use the metrics (as gridgeom returns them); only the uniform
+        Cartesian metrics the passes support are accepted."""
+        jac = np.ascontiguousarray(jac, dtype=np.float64)
+        buf = np.ascontiguousarray(np.asarray(dxi, dtype=np.float64).transpose((1, 0, 2, 3, 4)))
+        _chk(lib().astr_hip_set_metrics(self.h, _p(jac), _p(buf)))
 
     def profile_begin(self, max_launches):
         _chk(lib().astr_hip_profile_begin(self.h, max_launches))

@@ -571,6 +571,150 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* 
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// grid metrics (gridgeom, geom.F90:99-692) from the grid coordinates, on the
+// device with the line engine: dx(m,d) = d x_m / d xi_d by the compact
+// derivative (:128-161), the Jacobian (:350-381), the symmetric conservative
+// form of J d xi_r / d x_m (:395-540) and its division by J (:664-668).
+// Fields are dense (im+1)(jm+1)(km+1), i fastest; dx and dxi field (a-1) +
+// 3(b-1) holds the reference's (a,b).
+// ---------------------------------------------------------------------------
+namespace {
+
+struct GeomGrid {
+  int n[3];
+  long long st[3];
+  long long fs;
+};
+
+// coordinate m at index t (-5..n+5) along dir from the point base: the
+// single-rank periodic extension of gridsendrecv (parallel.F90:2859-2864),
+// x(n+h) = x(n) + (x(h) - x(0)), x(-h) = x(0) - (x(n) - x(n-h))
+__device__ __forceinline__ double geom_x(const double* __restrict__ X, const GeomGrid& G, int m, int dir,
+                                         long long base, int t) {
+  const double* f = X + m * G.fs + base;
+  const int n = G.n[dir];
+  const long long s = G.st[dir];
+  if (t < 0) return f[0] - (f[n * s] - f[(n + t) * s]);
+  if (t > n) return f[n * s] + (f[(t - n) * s] - f[0]);
+  return f[t * s];
+}
+
+// dx(a,e) at index t along dir: the periodic halo copy of dataswap
+__device__ __forceinline__ double geom_dx(const double* __restrict__ D, const GeomGrid& G, int a, int e,
+                                          int dir, long long base, int t) {
+  const int n = G.n[dir];
+  const int tt = t < 0 ? n + t : (t > n ? t - n : t);
+  return D[(a + 3 * e) * G.fs + base + tt * G.st[dir]];
+}
+
+// what a line holds: kind 0 the coordinate m; kind 1 the product
+// phi = 0.5 (dx(a,e) x_b - dx(b,e) x_a) (0-based indices)
+struct GeomLine {
+  int kind, m, a, e, b;
+};
+
+__device__ __forceinline__ long long geom_line_base(const GeomGrid& G, int dir, long long l) {
+  const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  return (l % (G.n[d1] + 1)) * G.st[d1] + (l / (G.n[d1] + 1)) * G.st[d2];
+}
+
+__global__ void k_geom_gather(const double* __restrict__ X, const double* __restrict__ D, GeomGrid G,
+                              int dir, GeomLine L, long long nlines, double* __restrict__ lines) {
+#pragma clang fp contract(off)
+  const int n = G.n[dir], W = n + 11;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nlines * W) return;
+  const long long l = idx / W;
+  const int t = (int)(idx - l * W) - 5;
+  const long long base = geom_line_base(G, dir, l);
+  double v;
+  if (L.kind == 0) {
+    v = geom_x(X, G, L.m, dir, base, t);
+  } else {
+    v = 0.5 * (geom_dx(D, G, L.a, L.e, dir, base, t) * geom_x(X, G, L.b, dir, base, t) -
+               geom_dx(D, G, L.b, L.e, dir, base, t) * geom_x(X, G, L.a, dir, base, t));
+  }
+  lines[idx] = v;
+}
+
+__global__ void k_geom_scatter(const double* __restrict__ out, GeomGrid G, int dir, long long nlines,
+                               double* __restrict__ dst, int accumulate) {
+  const int n = G.n[dir];
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nlines * (n + 1)) return;
+  const long long l = idx / (n + 1);
+  const int t = (int)(idx - l * (n + 1));
+  const long long o = geom_line_base(G, dir, l) + t * G.st[dir];
+  dst[o] = accumulate ? dst[o] + out[idx] : out[idx];
+}
+
+// Jacobian, the 2-D metrics, and dxi / J
+__global__ void k_geom_final(const double* __restrict__ D, double* __restrict__ XI, double* __restrict__ J,
+                             GeomGrid G, int ndims) {
+#pragma clang fp contract(off)
+  const long long o = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (o >= G.fs) return;
+  const long long fs = G.fs;
+#define DX(a, b) D[((a) - 1 + 3 * ((b) - 1)) * fs + o]
+#define XI_(a, b) XI[((a) - 1 + 3 * ((b) - 1)) * fs + o]
+  double jac;
+  if (ndims == 3) {
+    jac = DX(1, 1) * DX(2, 2) * DX(3, 3) + DX(1, 2) * DX(2, 3) * DX(3, 1) + DX(1, 3) * DX(2, 1) * DX(3, 2) -
+          DX(1, 3) * DX(2, 2) * DX(3, 1) - DX(1, 2) * DX(2, 1) * DX(3, 3) - DX(1, 1) * DX(2, 3) * DX(3, 2);
+  } else {
+    jac = DX(1, 1) * DX(2, 2) - DX(1, 2) * DX(2, 1);
+    const double x11 = DX(1, 1), x12 = DX(1, 2), x21 = DX(2, 1), x22 = DX(2, 2);
+    for (int b = 1; b <= 3; b++)
+      for (int a = 1; a <= 3; a++) XI_(a, b) = 0.0;
+    XI_(1, 1) = x22;
+    XI_(1, 2) = -x12;
+    XI_(2, 1) = -x21;
+    XI_(2, 2) = x11;
+  }
+  J[o] = jac;
+  for (int b = 1; b <= 3; b++)
+    for (int a = 1; a <= 3; a++) XI_(a, b) = XI_(a, b) / jac;
+#undef DX
+#undef XI_
+}
+
+// the 18 conservative terms: J dxi_r/dx_m += D_d [0.5 (dx(a,e) x_b - dx(b,e) x_a)],
+// as (d, r, m, a, e, b), 1-based, in the reference's order (geom.F90:395-515)
+const int kGeomTerms[18][6] = {
+    {1, 2, 1, 2, 3, 3}, {1, 2, 2, 3, 3, 1}, {1, 2, 3, 1, 3, 2},
+    {1, 3, 1, 3, 2, 2}, {1, 3, 2, 1, 2, 3}, {1, 3, 3, 2, 2, 1},
+    {2, 1, 1, 3, 3, 2}, {2, 1, 2, 1, 3, 3}, {2, 1, 3, 2, 3, 1},
+    {2, 3, 1, 2, 1, 3}, {2, 3, 2, 3, 1, 1}, {2, 3, 3, 1, 1, 2},
+    {3, 1, 1, 2, 2, 3}, {3, 1, 2, 3, 2, 1}, {3, 1, 3, 1, 2, 2},
+    {3, 2, 1, 3, 1, 2}, {3, 2, 2, 1, 1, 3}, {3, 2, 3, 2, 1, 1},
+};
+
+// derivative lines of length n already in device memory (f: n+11 per line,
+// out: n+1 per line) on stream st, with the LU table lu of n
+hipError_t deriv_lines_device(int n, long long nlines, const double* f, double* out, const double* lu,
+                              hipStream_t st) {
+  const int sh = pick_shape(n);
+  if (sh < 0) return hipErrorInvalidValue;
+  const ShapeInfo& s = kShapes[sh];
+  const long long nb = (nlines + s.TL - 1) / s.TL;
+  const size_t lds = (size_t)2 * s.TL * line_stride(s.P, s.R, s.TL) * sizeof(double);
+  FilterCoef fc{};
+#define LB(Cf) hipLaunchKernelGGL((k_lines<Cf, false>), dim3(nb), dim3(Cf::NT), lds, st, f, out, n, nlines, lu, fc, 0.0, 0.0)
+  switch (sh) {
+    case 0: LB(CfgA); break;
+    case 1: LB(CfgB); break;
+    case 2: LB(CfgC); break;
+    case 3: LB(CfgD); break;
+    case 4: LB(CfgE); break;
+    default: LB(CfgF); break;
+  }
+#undef LB
+  return hipGetLastError();
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* astr_hip_last_error(void) { return g_err.c_str(); }
@@ -1035,6 +1179,108 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
 #undef LB
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(out, dout, (size_t)nlines * (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* dxi) {
+  if (check_ctx(c)) return -1;
+  if (!x) return fail("null coordinates");
+  HIP_OK(hipSetDevice(c->device));
+  const int nd = c->a.ndir;
+  GeomGrid G;
+  for (int d = 0; d < 3; d++) G.n[d] = c->a.n[d];
+  G.st[0] = 1;
+  G.st[1] = G.n[0] + 1;
+  G.st[2] = (long long)(G.n[0] + 1) * (G.n[1] + 1);
+  G.fs = G.st[2] * (G.n[2] + 1);
+  long long maxl = 0, maxo = 0;
+  for (int d = 0; d < nd; d++) {
+    const long long nl = G.fs / (G.n[d] + 1);
+    maxl = std::max(maxl, nl * (G.n[d] + 11));
+    maxo = std::max(maxo, nl * (G.n[d] + 1));
+  }
+  DevBuf bx, bdx, bxi, bj, bl, bo, bt[3];
+  HIP_OK(bx.alloc(3 * G.fs * sizeof(double)));
+  HIP_OK(bdx.alloc(9 * G.fs * sizeof(double)));
+  HIP_OK(bxi.alloc(9 * G.fs * sizeof(double)));
+  HIP_OK(bj.alloc(G.fs * sizeof(double)));
+  HIP_OK(bl.alloc(maxl * sizeof(double)));
+  HIP_OK(bo.alloc(maxo * sizeof(double)));
+  for (int d = 0; d < nd; d++) {
+    std::vector<double> t;
+    deriv_tables(G.n[d], t);
+    HIP_OK(bt[d].alloc(t.size() * sizeof(double)));
+    HIP_OK(hipMemcpy(bt[d].p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  hipStream_t st = c->stream;
+  HIP_OK(hipMemcpyAsync(bx.p, x, 3 * G.fs * sizeof(double), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemsetAsync(bdx.p, 0, 9 * G.fs * sizeof(double), st));
+  HIP_OK(hipMemsetAsync(bxi.p, 0, 9 * G.fs * sizeof(double), st));
+  auto run = [&](int dir, GeomLine L, double* dst, int acc) -> int {
+    const long long nl = G.fs / (G.n[dir] + 1);
+    const long long ng = nl * (G.n[dir] + 11), ns = nl * (G.n[dir] + 1);
+    hipLaunchKernelGGL(k_geom_gather, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, bx.p, bdx.p, G,
+                       dir, L, nl, bl.p);
+    HIP_OK(hipGetLastError());
+    HIP_OK(deriv_lines_device(G.n[dir], nl, bl.p, bo.p, bt[dir].p, st));
+    hipLaunchKernelGGL(k_geom_scatter, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, bo.p, G, dir,
+                       nl, dst, acc);
+    HIP_OK(hipGetLastError());
+    return 0;
+  };
+  // dx(m,d) = d x_m / d xi_d (geom.F90:128-161)
+  for (int d = 0; d < nd; d++)
+    for (int m = 0; m < 3; m++)
+      if (run(d, GeomLine{0, m, 0, 0, 0}, bdx.p + (m + 3 * d) * G.fs, 0)) return -1;
+  // J dxi (geom.F90:395-515), 3-D only; the 2-D form is pointwise
+  if (nd == 3)
+    for (const auto& T : kGeomTerms) {
+      double* dst = bxi.p + ((T[1] - 1) + 3 * (T[2] - 1)) * G.fs;
+      if (run(T[0] - 1, GeomLine{1, 0, T[3] - 1, T[4] - 1, T[5] - 1}, dst, 1)) return -1;
+    }
+  hipLaunchKernelGGL(k_geom_final, dim3((unsigned)((G.fs + 255) / 256)), dim3(256), 0, st, bdx.p, bxi.p, bj.p,
+                     G, nd);
+  HIP_OK(hipGetLastError());
+  if (jacob) HIP_OK(hipMemcpyAsync(jacob, bj.p, G.fs * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (dxi) HIP_OK(hipMemcpyAsync(dxi, bxi.p, 9 * G.fs * sizeof(double), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int astr_hip_set_metrics(astr_hip_ctx* c, const double* jacob, const double* dxi) {
+  if (check_ctx(c)) return -1;
+  if (!jacob || !dxi) return fail("null metrics");
+  const FlowArgs& a = c->a;
+  const long long fs = (long long)(a.n[0] + 1) * (a.n[1] + 1) * (a.n[2] + 1);
+  const int nd = a.ndir;
+  double mean[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < nd; d++) {
+    const double* f = dxi + (d + 3 * d) * fs;
+    // long double: a plain running sum of ~1e4 equal values drifts by
+    // ~1e-14 relative, which the RHS would see as a changed grid spacing
+    long double s = 0.0L;
+    for (long long o = 0; o < fs; o++) s += f[o];
+    mean[d] = (double)(s / (long double)fs);
+    if (!(mean[d] > 0.0)) return fail("metrics: d xi/d x must be positive");
+  }
+  // this build's passes take 1/dx per direction: accept only metrics that
+  // are diagonal and constant to rounding (a uniform Cartesian grid)
+  const double tol = 1e-12;
+  for (int q = 0; q < 3; q++)
+    for (int p = 0; p < 3; p++) {
+      const double* f = dxi + (p + 3 * q) * fs;
+      const bool diag = p == q && p < nd;
+      for (long long o = 0; o < fs; o++) {
+        const double ref = diag ? mean[p] : 0.0;
+        if (!(std::fabs(f[o] - ref) <= tol * mean[p < nd ? p : 0]))
+          return fail("metrics are not those of a uniform Cartesian grid; this build supports only those");
+      }
+    }
+  double jm = 1.0;
+  for (int d = 0; d < nd; d++) jm *= 1.0 / mean[d];
+  for (long long o = 0; o < fs; o++)
+    if (!(std::fabs(jacob[o] - jm) <= tol * jm)) return fail("metrics: Jacobian inconsistent with d xi/d x");
+  for (int d = 0; d < nd; d++) c->a.inv_d[d] = mean[d];
   return 0;
 }
 

@@ -105,6 +105,19 @@ int astr_hip_rk3_steps(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens);
 int astr_hip_rk3_steps_stats(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens, double *diss);
 int astr_hip_sync(astr_hip_ctx *ctx);
 
+/* gridgeom: geom.F90:99-692 (single rank, periodic).  x(0:im,0:jm,0:km,1:3)
+ * are the grid coordinates; on the device: dx(m,d) = d x_m / d xi_d by the
+ * compact derivative with gridsendrecv's single-rank halo
+ * (parallel.F90:2859-2864), jacob = det(dx) (:350-381), the symmetric
+ * conservative form of jacob * dxi (:395-540), divided by jacob (:664-668).
+ * jacob(0:im,0:jm,0:km) and dxi(0:im,0:jm,0:km,1:3,1:3) are nullable. */
+int astr_hip_gridgeom(astr_hip_ctx *ctx, const double *x, double *jacob, double *dxi);
+/* Metrics for the passes, in gridgeom's layout.  This build's passes take one
+ * 1/dx per direction, so only the metrics of a uniform Cartesian grid
+ * (dxi diagonal and constant, jacob = 1/prod(dxi), both to 1e-12) are
+ * accepted; 1/dx becomes the mean of dxi(d,d).  Others return an error. */
+int astr_hip_set_metrics(astr_hip_ctx *ctx, const double *jacob, const double *dxi);
+
 /* Per-pass timing with HIP events on the context's stream (no reference
  * counterpart; the reference's timerept accumulators, src/utility.F90:27-297,
  * are no-ops).  Slot = pass*3 + direction, pass 0..8 = convective+gradient
