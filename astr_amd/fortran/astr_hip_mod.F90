@@ -112,6 +112,17 @@ module astr_hip_capi
       type(c_ptr), value :: ke, ens, diss
       integer(c_int) :: astr_hip_rk3_steps_stats
     end function
+    ! gridgeom (geom.F90:99-692): x(0:im,0:jm,0:km,1:3) -> jacob, dxi
+    function astr_hip_gridgeom(ctx, x, jacob, dxi) bind(C, name='astr_hip_gridgeom')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, x, jacob, dxi
+      integer(c_int) :: astr_hip_gridgeom
+    end function
+    function astr_hip_set_metrics(ctx, jacob, dxi) bind(C, name='astr_hip_set_metrics')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, jacob, dxi
+      integer(c_int) :: astr_hip_set_metrics
+    end function
     function astr_hip_sync(ctx) bind(C, name='astr_hip_sync')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
