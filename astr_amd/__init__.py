@@ -52,7 +52,7 @@ EXPORTS = [
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
     "astr_hip_batch_rk3_steps", "astr_hip_batch_info", "astr_hip_set_last_split",
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
-    "astr_hip_gridgeom", "astr_hip_set_metrics",
+    "astr_hip_gridgeom", "astr_hip_set_metrics", "astr_hip_qswap",
     "astr_hip_get_last_split",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
@@ -86,6 +86,7 @@ def lib():
         L.astr_hip_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.astr_hip_rk3_steps_stats.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.astr_hip_gridgeom.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.astr_hip_qswap.argtypes = [C.c_void_p]
         L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
@@ -210,6 +211,11 @@ class Flow:
         return (ke, en, di) if dissipation else (ke, en)
 
     steploop = rk3
+
+    def qswap(self):
+        """The main path's single-rank qswap: duplicate-plane averages along
+        i, then j, then k (parallel.F90:4876-4884, 5018-5030, 5159-5170)."""
+        _chk(lib().astr_hip_qswap(self.h))
 
     # -- grid metrics (gridgeom, geom.F90:99-692) ------------------------------
     def gridgeom(self, x):

@@ -572,6 +572,30 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* 
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// qswap's single-rank periodic branch (parallel.F90:4876-4884, 5018-5030,
+// 5159-5170).  Its halo copies are the passes' index wrap; what it adds is
+// the duplicate-plane average q(0) = q(n) = 0.5 (q(0) + q(n)) of direction
+// dir, over all points of the two other directions (i, then j, then k).
+// ---------------------------------------------------------------------------
+namespace {
+__global__ void k_qswap_dir(FlowArgs a, int dir) {
+  const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  const long long n1 = a.n[d1] + 1, n2 = d2 < 3 ? (long long)a.n[d2] + 1 : 1;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 5 * n1 * n2) return;
+  const int m = (int)(idx / (n1 * n2));
+  const long long r = idx - m * n1 * n2;
+  const long long st[3] = {1, a.pitch, a.plane};
+  const long long o0 = (r % n1) * st[d1] + (r / n1) * st[d2];
+  const long long on = o0 + (long long)a.n[dir] * st[dir];
+  double* q = a.q + m * a.fs;
+  const double v = 0.5 * (q[o0] + q[on]);
+  q[o0] = v;
+  q[on] = v;
+}
+}  // namespace
+
+// ---------------------------------------------------------------------------
 // grid metrics (gridgeom, geom.F90:99-692) from the grid coordinates, on the
 // device with the line engine: dx(m,d) = d x_m / d xi_d by the compact
 // derivative (:128-161), the Jacobian (:350-381), the symmetric conservative
@@ -1244,6 +1268,19 @@ int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* d
   if (jacob) HIP_OK(hipMemcpyAsync(jacob, bj.p, G.fs * sizeof(double), hipMemcpyDeviceToHost, st));
   if (dxi) HIP_OK(hipMemcpyAsync(dxi, bxi.p, 9 * G.fs * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
+  return 0;
+}
+
+int astr_hip_qswap(astr_hip_ctx* c) {
+  if (check_ctx(c)) return -1;
+  HIP_OK(hipSetDevice(c->device));
+  const FlowArgs& a = c->a;
+  for (int d = 0; d < a.ndir; d++) {
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    const long long cnt = 5LL * (a.n[d1] + 1) * (a.n[d2] + 1);
+    hipLaunchKernelGGL(k_qswap_dir, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, a, d);
+    HIP_OK(hipGetLastError());
+  }
   return 0;
 }
 

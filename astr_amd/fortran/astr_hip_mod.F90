@@ -112,6 +112,12 @@ module astr_hip_capi
       type(c_ptr), value :: ke, ens, diss
       integer(c_int) :: astr_hip_rk3_steps_stats
     end function
+    ! qswap, single rank (parallel.F90:4876-4884, 5018-5030, 5159-5170)
+    function astr_hip_qswap(ctx) bind(C, name='astr_hip_qswap')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int) :: astr_hip_qswap
+    end function
     ! gridgeom (geom.F90:99-692): x(0:im,0:jm,0:km,1:3) -> jacob, dxi
     function astr_hip_gridgeom(ctx, x, jacob, dxi) bind(C, name='astr_hip_gridgeom')
       import :: c_int, c_ptr

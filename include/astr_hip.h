@@ -105,6 +105,13 @@ int astr_hip_rk3_steps(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens);
 int astr_hip_rk3_steps_stats(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens, double *diss);
 int astr_hip_sync(astr_hip_ctx *ctx);
 
+/* qswap, single rank, periodic (parallel.F90:4876-4884, 5018-5030,
+ * 5159-5170): the main path's duplicate-plane average q(0) = q(n) =
+ * 0.5 (q(0) + q(n)) along i, then j, then k, on the device; the halo copies
+ * it also makes are the passes' own index wrap.  The miniapp's stage order
+ * (astr_hip_rk3_steps) does not call it. */
+int astr_hip_qswap(astr_hip_ctx *ctx);
+
 /* gridgeom: geom.F90:99-692 (single rank, periodic).  x(0:im,0:jm,0:km,1:3)
  * are the grid coordinates; on the device: dx(m,d) = d x_m / d xi_d by the
  * compact derivative with gridsendrecv's single-rank halo

@@ -164,6 +164,23 @@ class OracleState:
         return (ke, en, di) if dissipation else (ke, en)
 
 
+def qswap(q):
+    """qswap's single-rank periodic branch (parallel.F90:4876-4884,
+    5018-5030, 5159-5170) on a (5, km+1, jm+1, im+1) array (the lattice
+    0..n with its duplicate planes): q(0) = 0.5 (q(0) + q(n)), q(n) = q(0),
+    along i over all j, k, then along j over i = 0..im, then along k.  The
+    halo copies it also makes are not part of this array."""
+    q = np.array(q, dtype=np.float64, copy=True)
+    q[..., 0] = 0.5 * (q[..., 0] + q[..., -1])
+    q[..., -1] = q[..., 0]
+    q[:, :, 0, :] = 0.5 * (q[:, :, 0, :] + q[:, :, -1, :])
+    q[:, :, -1, :] = q[:, :, 0, :]
+    if q.shape[1] > 1:
+        q[:, 0] = 0.5 * (q[:, 0] + q[:, -1])
+        q[:, -1] = q[:, 0]
+    return q
+
+
 def df_compact(f):
     """Reference-layout line f(-hm:n+hm) -> df(0:n) by the C restatement."""
     f = np.ascontiguousarray(f, dtype=np.float64)

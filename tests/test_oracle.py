@@ -277,3 +277,24 @@ def test_philox_known_answers():
 def test_gauss_moments():
     z = np.array([O.gauss(7, i, j, 0, 0) for i in range(64) for j in range(64)])
     assert abs(z.mean()) < 0.05 and abs(z.std() - 1) < 0.05
+
+
+def test_qswap_restatement_properties():
+    """qswap's duplicate-plane average (parallel.F90:4882-4883, 5027-5028,
+    5168-5169): afterwards every duplicate plane equals its plane 0; along i
+    (the first sweep) each pair keeps its rounded sum exactly (2 v = fl(a+b));
+    the interior is untouched; a second application changes nothing."""
+    rng = np.random.default_rng(3)
+    for shape in ((5, 7, 6, 9), (5, 1, 6, 9)):
+        q = rng.standard_normal(shape)
+        s = O.qswap(q)
+        assert np.array_equal(s[..., 0], s[..., -1])
+        assert np.array_equal(s[:, :, 0], s[:, :, -1])
+        if shape[1] > 1:
+            assert np.array_equal(s[:, 0], s[:, -1])
+        inner = (slice(None), slice(1, -1) if shape[1] > 1 else slice(None), slice(1, -1), slice(1, -1))
+        assert np.array_equal(s[inner], q[inner])
+        # the i sweep alone: pair sums kept, on rows the j/k sweeps do not touch
+        rows = (slice(None), slice(1, -1) if shape[1] > 1 else slice(None), slice(1, -1))
+        assert np.array_equal(2.0 * s[rows + (0,)], q[rows + (0,)] + q[rows + (-1,)])
+        assert np.array_equal(O.qswap(s), s)
