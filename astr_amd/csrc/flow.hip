@@ -352,6 +352,7 @@ struct ShapeInfo {
 const ShapeInfo kShapes[kNumShapes] = {
     {0, 256, 4, 64, 5, 5}, {1, 256, 4, 64, 9, 9}, {2, 256, 16, 16, 5, 5},
     {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5}, {5, 512, 8, 64, 5, 5},
+    {6, 128, 4, 32, 9, 9}, {7, 128, 2, 64, 5, 5},
 };
 
 bool shape_fits(const ShapeInfo& s, int n) {
@@ -407,7 +408,9 @@ hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir
     case 2: return launch_pass_s2(prog, minb, a, dir, st);
     case 3: return launch_pass_s3(prog, minb, a, dir, st);
     case 4: return launch_pass_s4(prog, minb, a, dir, st);
-    default: return launch_pass_s5(prog, minb, a, dir, st);
+    case 5: return launch_pass_s5(prog, minb, a, dir, st);
+    case 6: return launch_pass_s6(prog, minb, a, dir, st);
+    default: return launch_pass_s7(prog, minb, a, dir, st);
   }
 }
 
@@ -731,7 +734,9 @@ hipError_t deriv_lines_device(int n, long long nlines, const double* f, double* 
     case 2: LB(CfgC); break;
     case 3: LB(CfgD); break;
     case 4: LB(CfgE); break;
-    default: LB(CfgF); break;
+    case 5: LB(CfgF); break;
+    case 6: LB(CfgG); break;
+    default: LB(CfgH); break;
   }
 #undef LB
   return hipGetLastError();
@@ -830,7 +835,9 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   }
   // per-tile stats slots for the smallest tile any shape uses
   // (then the dissipation rate's block partials)
-  const int ntl = tiles_of(a.n, a.ndir - 1, 4);
+  int tl_min = 32;
+  for (const ShapeInfo& s : kShapes) tl_min = std::min(tl_min, s.TL);
+  const int ntl = tiles_of(a.n, a.ndir - 1, tl_min);
   if (hipMalloc(&c->stats_part, (kTileStats * (size_t)ntl + kDissBlocks) * sizeof(double)) != hipSuccess) {
     (void)hipFree(c->base); (void)hipFree(c->coef); delete c; return fail("hipMalloc of stats failed");
   }
@@ -1198,7 +1205,9 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
     case 2: LB(CfgC); break;
     case 3: LB(CfgD); break;
     case 4: LB(CfgE); break;
-    default: LB(CfgF); break;
+    case 5: LB(CfgF); break;
+    case 6: LB(CfgG); break;
+    default: LB(CfgH); break;
   }
 #undef LB
   HIP_OK(hipGetLastError());
@@ -1356,7 +1365,9 @@ hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, in
     case 2: return launch_batch_s2(prog, minb, bl, dir, sa, nblocks, st);
     case 3: return launch_batch_s3(prog, minb, bl, dir, sa, nblocks, st);
     case 4: return launch_batch_s4(prog, minb, bl, dir, sa, nblocks, st);
-    default: return launch_batch_s5(prog, minb, bl, dir, sa, nblocks, st);
+    case 5: return launch_batch_s5(prog, minb, bl, dir, sa, nblocks, st);
+    case 6: return launch_batch_s6(prog, minb, bl, dir, sa, nblocks, st);
+    default: return launch_batch_s7(prog, minb, bl, dir, sa, nblocks, st);
   }
 }
 
@@ -1455,7 +1466,8 @@ int build_groups(astr_hip_batch* b) {
     g.bl.total = ts[nm];
     // persistent grid: a multiple of 8 blocks, at most as many blocks as the
     // budget keeps resident (minb per CU), never more than the tiles
-    const int cap = b->ncu * g.minb;
+    // (a 128-thread workgroup is half a 256-thread one: twice as many fit)
+    const int cap = b->ncu * g.minb * (kShapes[g.shape].NT < 256 ? 256 / kShapes[g.shape].NT : 1);
     int nb = g.bl.total < cap ? g.bl.total : cap;
     g.nblocks = (nb + 7) / 8 * 8;
   }
@@ -1719,7 +1731,7 @@ int astr_hip_batch_autotune(astr_hip_batch* b, int reps) {
         }
         BatchLaunch bl{dargs, dts, nm, ts[nm]};
         for (int mb = 1; mb <= kMinbVariants && !rc; mb++) {
-          const int cap = b->ncu * mb;
+          const int cap = b->ncu * mb * (kShapes[sh].NT < 256 ? 256 / kShapes[sh].NT : 1);
           const int nb = ((bl.total < cap ? bl.total : cap) + 7) / 8 * 8;
           hipError_t e = launch_batch(sh, prog, mb, bl, d, sa, nb, b->stream);
           if (e == hipSuccess) e = hipEventRecord(e0, b->stream);

@@ -653,7 +653,9 @@ using CfgC = LineCfg<256, 16, 5, 5>;   // P=16, n <= 73
 using CfgD = LineCfg<256, 8, 9, 9>;    // P=32, n <= 281
 using CfgE = LineCfg<256, 8, 5, 5>;    // P=32, n <= 153
 using CfgF = LineCfg<512, 8, 5, 5>;    // P=64, n <= 313: CfgA's lines, 64-byte row segments
-constexpr int kNumShapes = 6;
+using CfgG = LineCfg<128, 4, 9, 9>;    // P=32, n <= 281: CfgD's lines, two workgroups per CU at 1 wave/SIMD
+using CfgH = LineCfg<128, 2, 5, 5>;    // P=64, n <= 313: CfgA's lines in half-size workgroups
+constexpr int kNumShapes = 8;
 constexpr int kMinbVariants = 2;       // MINB = 1, 2
 
 // one per shape, each in its own translation unit (passes.hip)
@@ -667,6 +669,8 @@ ASTR_DECLARE_SHAPE(2)
 ASTR_DECLARE_SHAPE(3)
 ASTR_DECLARE_SHAPE(4)
 ASTR_DECLARE_SHAPE(5)
+ASTR_DECLARE_SHAPE(6)
+ASTR_DECLARE_SHAPE(7)
 #undef ASTR_DECLARE_SHAPE
 
 }  // namespace astr

@@ -1,10 +1,10 @@
 // passes.hip -- the pass kernels of one launch shape (single case and
-// batched), in register budgets MINB = 1 and 2.  Built once per shape with ASTR_SHAPE_ID = 0..5 so the
+// batched), in register budgets MINB = 1 and 2.  Built once per shape with ASTR_SHAPE_ID = 0..7 so the
 // shapes compile in parallel; flow.hip dispatches to launch_pass_s<ID>.
 #include "flow_dev.hpp"
 
 #ifndef ASTR_SHAPE_ID
-#error "build with -DASTR_SHAPE_ID=<0..5>"
+#error "build with -DASTR_SHAPE_ID=<0..7>"
 #endif
 
 namespace astr {
@@ -29,10 +29,18 @@ using ShapeCfg = CfgD;
 #define ASTR_LAUNCH launch_pass_s4
 #define ASTR_BATCH launch_batch_s4
 using ShapeCfg = CfgE;
-#else
+#elif ASTR_SHAPE_ID == 5
 #define ASTR_LAUNCH launch_pass_s5
 #define ASTR_BATCH launch_batch_s5
 using ShapeCfg = CfgF;
+#elif ASTR_SHAPE_ID == 6
+#define ASTR_LAUNCH launch_pass_s6
+#define ASTR_BATCH launch_batch_s6
+using ShapeCfg = CfgG;
+#else
+#define ASTR_LAUNCH launch_pass_s7
+#define ASTR_BATCH launch_batch_s7
+using ShapeCfg = CfgH;
 #endif
 
 hipError_t ASTR_LAUNCH(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st) {
