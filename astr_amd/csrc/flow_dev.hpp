@@ -655,7 +655,9 @@ using CfgE = LineCfg<256, 8, 5, 5>;    // P=32, n <= 153
 using CfgF = LineCfg<512, 8, 5, 5>;    // P=64, n <= 313: CfgA's lines, 64-byte row segments
 using CfgG = LineCfg<128, 4, 9, 9>;    // P=32, n <= 281: CfgD's lines, two workgroups per CU at 1 wave/SIMD
 using CfgH = LineCfg<128, 2, 5, 5>;    // P=64, n <= 313: CfgA's lines in half-size workgroups
-constexpr int kNumShapes = 8;
+using CfgI = LineCfg<64, 1, 5, 5>;     // P=64, n <= 313: one line per single-wavefront workgroup
+using CfgJ = LineCfg<64, 2, 9, 9>;     // P=32, n <= 281: two lines per single-wavefront workgroup
+constexpr int kNumShapes = 10;
 constexpr int kMinbVariants = 2;       // MINB = 1, 2
 
 // one per shape, each in its own translation unit (passes.hip)
@@ -671,6 +673,8 @@ ASTR_DECLARE_SHAPE(4)
 ASTR_DECLARE_SHAPE(5)
 ASTR_DECLARE_SHAPE(6)
 ASTR_DECLARE_SHAPE(7)
+ASTR_DECLARE_SHAPE(8)
+ASTR_DECLARE_SHAPE(9)
 #undef ASTR_DECLARE_SHAPE
 
 }  // namespace astr
