@@ -1623,7 +1623,7 @@ int astr_hip_batch_rk3_begin(astr_hip_batch* b, int nsteps, int use_graph) {
   if (nsteps < 0) return fail("negative step count");
   HIP_OK(hipSetDevice(b->device));
   if (batch_sync_cases(b)) return -1;
-  b->pending = nsteps;
+  b->pending = 0;  // set once every step is enqueued
   if (nsteps == 0) return 0;
   if (ensure_stats(b, nsteps) || upload_jobs(b) || refresh_args(b)) return -1;
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(1), 0, b->stream, b->d_step, 0);
@@ -1635,6 +1635,7 @@ int astr_hip_batch_rk3_begin(astr_hip_batch* b, int nsteps, int use_graph) {
     for (int it = 0; it < nsteps; it++)
       if (enqueue_step(b)) return -1;
   }
+  b->pending = nsteps;
   return 0;
 }
 
