@@ -257,6 +257,107 @@ contains
     write(*, '(3(A,I0))') '  ** dimension set: ', im, ' x ', jm, ' x ', km
   end subroutine program_init
 
+  ! readinput (src/readwrite.F90:466-716, the build without COMB): the
+  ! datin input file, read record by record as the reference reads it.  It
+  ! sets the grid, the filter coefficient and the reference values.  Options
+  ! outside what this build runs (a periodic box of flowtype tgv with the 643c
+  ! scheme, rk3, the filter, no species, no turbulence model, no grid file,
+  ! no restart) stop with a message, as the reference stops on bad input.
+  subroutine readinput(inputfile)
+    character(len=*), intent(in) :: inputfile
+    character(len=64) :: flowtype, gridfile
+    character(len=16) :: conschm, dschm, rkscheme, turbmode, iomode
+    character(len=255) :: lineread
+    logical :: lihomo, ljhomo, lkhomo, nondimen, diffterm, lfilter, lreadgrid, lfftk, &
+               limmbou, ltimrpt, lrestart, lchardecomp
+    integer :: fh, n, ia, ja, ka, kcutoff, recon_schem, num_species, ninit, bctype(6)
+    real(8) :: afil, rtem, re, ma, bfacmpld, shkcrt
+
+    open(newunit=fh, file=trim(inputfile), action='read', status='old')
+    read(fh, '(////)')
+    read(fh, *) flowtype
+    read(fh, '(/)')
+    read(fh, *) ia, ja, ka
+    read(fh, '(/)')
+    read(fh, *) lihomo, ljhomo, lkhomo
+    read(fh, '(/)')
+    read(fh, *) nondimen, diffterm, lfilter, lreadgrid, lfftk, limmbou, ltimrpt
+    read(fh, '(/)')
+    read(fh, *) lrestart
+    read(fh, '(/)')
+    read(fh, *) afil, kcutoff
+    if (.not. nondimen) call input_stop('dimensional input (nondimen = f)')
+    read(fh, '(/)')
+    read(fh, *) rtem, re, ma
+    read(fh, '(/)')
+    read(fh, *) conschm, dschm, rkscheme
+    read(fh, '(/)')
+    read(fh, *) recon_schem, lchardecomp, bfacmpld, shkcrt
+    read(fh, '(/)')
+    read(fh, *) num_species
+    read(fh, '(/)')
+    read(fh, *) turbmode, iomode
+    read(fh, '(/)')
+    do n = 1, 6
+      read(fh, *) bctype(n)
+    end do
+    read(fh, '(/)')
+    read(fh, *) ninit
+    read(fh, '(/)')
+    read(fh, '(A)') lineread
+    read(fh, '(/)')
+    read(fh, '(A)') gridfile
+    close(fh)
+
+    if (trim(flowtype) /= 'tgv') call input_stop('flowtype '//trim(flowtype))
+    if (.not. (lihomo .and. ljhomo .and. lkhomo)) call input_stop('non-homogeneous directions')
+    if (any(bctype /= 1)) call input_stop('boundary types other than periodic (1)')
+    if (.not. diffterm) call input_stop('diffterm = f')
+    if (.not. lfilter) call input_stop('lfilter = f')
+    if (lreadgrid .or. lfftk .or. limmbou .or. lrestart) &
+      call input_stop('lreadgrid, lfftz, limmbou or lrestart = t')
+    if (trim(conschm) /= '643c' .or. trim(dschm) /= '643c') &
+      call input_stop('schemes '//trim(conschm)//', '//trim(dschm))
+    if (trim(rkscheme) /= 'rk3') call input_stop('time scheme '//trim(rkscheme))
+    if (num_species /= 0) call input_stop('species (num_species > 0)')
+    if (trim(turbmode) /= 'none') call input_stop('turbmode '//trim(turbmode))
+    if (ninit /= 0) call input_stop('ninit /= 0')
+
+    im = ia; jm = ja; km = ka
+    numq = 5
+    alfa_filter = afil
+    ref_tem = rtem; reynolds = re; mach = ma
+    difschm = dschm(1:4)
+    write(*, '(3(A,I0))') '  ** dimension set: ', im, ' x ', jm, ' x ', km
+    write(*, '(A,4(1X,ES22.15))') '  ** ref_tem, reynolds, mach, alfa_filter:', ref_tem, reynolds, &
+      mach, alfa_filter
+    print *, ' >> ', trim(inputfile), ' ... done'
+  end subroutine readinput
+
+  ! readcont (src/readwrite.F90:729-769): datin/controller, the step count
+  ! and the time step
+  subroutine readcont(contfile)
+    character(len=*), intent(in) :: contfile
+    logical :: lwsequ, lwslic, lavg, lcracon
+    integer :: fh, feqchkpt, feqwsequ, feqslice, feqlist, feqavg
+    open(newunit=fh, file=trim(contfile), action='read', status='old')
+    read(fh, '(////)')
+    read(fh, *) lwsequ, lwslic, lavg, lcracon
+    read(fh, '(/)')
+    read(fh, *) maxstep, feqchkpt, feqwsequ, feqslice, feqlist, feqavg
+    read(fh, '(/)')
+    read(fh, *) deltat
+    close(fh)
+    write(*, '(A,I0,A,ES22.15)') '  ** maxstep ', maxstep, ', deltat', deltat
+    print *, ' >> ', trim(contfile), ' ... done'
+  end subroutine readcont
+
+  subroutine input_stop(what)
+    character(len=*), intent(in) :: what
+    print *, ' !! input not supported by the device solver: ', what
+    stop 1
+  end subroutine input_stop
+
   ! solver_init (tgvsolver.F90:236-261): scheme and filter coefficients,
   ! built on the device with the flow constants (flowfield_init :181-196)
   subroutine solver_init

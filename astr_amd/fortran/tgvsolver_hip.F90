@@ -5,7 +5,11 @@
 ! (nstep, time, kinetic energy, enstrophy per step, list-directed).
 !
 ! usage: tgvsolver_hip [im [jm [km [maxstep [deltat [noise_amp [seed]]]]]]]
+!        tgvsolver_hip --datin INPUTFILE CONTROLLER
 ! defaults are the reference's: 128^3, 100 steps, deltat 1e-3, no noise.
+! With --datin the grid, filter and reference values come from the
+! reference's datin input file and the step count and time step from its
+! controller file (readinput / readcont, src/readwrite.F90:466-769).
 !-----------------------------------------------------------------------
 program tgvsolver_hip
   use iso_c_binding, only: c_long_long
@@ -13,12 +17,31 @@ program tgvsolver_hip
   implicit none
   integer :: nargs, v(4), i
   character(len=64) :: arg
+  character(len=512) :: path
   real(8) :: tstart, tfinish
 
   call cpu_time(tstart)
 
-  v = [128, 128, 128, 100]
   nargs = command_argument_count()
+  if (nargs >= 3) then
+    call get_command_argument(1, arg)
+    if (trim(arg) == '--datin') then
+      call get_command_argument(2, path)
+      call readinput(trim(path))
+      call get_command_argument(3, path)
+      call readcont(trim(path))
+      call solver_init
+      call allocommarray
+      call flowfield_init
+      call steploop
+      call solver_end
+      call cpu_time(tfinish)
+      print *, ' ** steps: ', maxstep, ' wall (host cpu) time: ', tfinish - tstart
+      stop
+    end if
+  end if
+
+  v = [128, 128, 128, 100]
   do i = 1, min(nargs, 4)
     call get_command_argument(i, arg)
     read(arg, *) v(i)

@@ -92,3 +92,60 @@ def test_chemcells_hip_matches_oracle(tmp_path):
     assert np.abs(te / T - 1).max() < 1e-12
     assert np.abs(rgc / m.rgcmix(Y) - 1).max() < 1e-14
     assert np.abs(xm - m.convertxiyi(Y, "Y2X")).max() < 1e-15
+
+
+DATIN = os.path.join(GOLD, "datin_tgv")
+
+
+def _datin_run(tmp_path, text_edit=None):
+    """tgvsolver_hip --datin on the reference's Taylor-Green example
+    (examples/Taylor_Green_Vortex/datin, kept as a fixture), optionally
+    with one line of the input replaced."""
+    inp = os.path.join(DATIN, "input.tgv")
+    if text_edit:
+        lines = open(inp).read().split("\n")
+        old, new = text_edit
+        lines = [new if ln.strip() == old else ln for ln in lines]
+        inp = str(tmp_path / "input.edit")
+        open(inp, "w").write("\n".join(lines))
+    return subprocess.run([os.path.join(FDIR, "tgvsolver_hip"), "--datin", inp,
+                           os.path.join(DATIN, "controller")],
+                          cwd=tmp_path, capture_output=True, text=True, timeout=300)
+
+
+def test_datin_input_is_read_as_the_reference_reads_it(tmp_path):
+    """readinput / readcont (src/readwrite.F90:466-769) on the reference's TGV
+    example: grid 128^3, Re 1600, Ma 0.1, T_ref 273.15, alpha_f 0.49,
+    maxstep 10, deltat 1e-3 (what follows the parse needs a GPU)."""
+    r = _datin_run(tmp_path)
+    out = r.stdout
+    assert "dimension set: 128 x 128 x 128" in out, out[-2000:]
+    vals = out.split("ref_tem, reynolds, mach, alfa_filter:")[1].split("\n")[0].split()
+    assert [float(v) for v in vals] == [273.15, 1600.0, 0.1, 0.49]
+    assert "maxstep 10, deltat" in out
+    assert float(out.split("maxstep 10, deltat")[1].split("\n")[0]) == 1e-3
+
+
+@pytest.mark.parametrize("edit", [("tgv", "chl"), ("643c, 643c, rk3, rk3", "m7c, 643c, rk3, rk3"),
+                                  ("t,t,t,f,f,f,t,f", "t,t,t,t,f,f,t,f")])
+def test_datin_unsupported_options_stop(tmp_path, edit):
+    """Options the device solver does not run stop with a message and exit
+    status 1 before any device work."""
+    r = _datin_run(tmp_path, edit)
+    assert r.returncode == 1, r.stdout[-1000:] + r.stderr[-1000:]
+    assert "input not supported by the device solver" in r.stdout
+
+
+@pytest.mark.gpu
+def test_datin_run_reproduces_state_ref_128(tmp_path):
+    """The reference's own TGV input (128^3, Re 1600, Ma 0.1, dt 1e-3, 10 steps
+    from its controller) run through the Fortran host on the GPU reproduces the
+    first 10 lines of the reference-held state.ref_128 to 1e-9."""
+    r = _datin_run(tmp_path)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    got = np.loadtxt(tmp_path / "state.dat")
+    ref = np.loadtxt(os.path.join(GOLD, "state_ref_128.txt"))[:10]
+    assert got.shape == ref.shape
+    assert np.array_equal(got[:, 0], ref[:, 0])
+    for c in (2, 3):
+        assert np.abs(got[:, c] - ref[:, c]).max() / np.abs(ref[:, c]).max() < 1e-9
