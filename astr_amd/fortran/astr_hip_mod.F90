@@ -241,6 +241,10 @@ module astr_hip_tgv
   type(c_ptr) :: ctx = c_null_ptr
   integer :: statfile = 13
   logical :: statopen = .false.
+  ! flowstate.dat of the main path (statistic.F90:777-859): written by
+  ! steploop after readinput selected a tgv run
+  logical :: lflowstate = .false., fsopen = .false.
+  integer :: fsfile
 
 contains
 
@@ -325,6 +329,7 @@ contains
 
     im = ia; jm = ja; km = ka
     numq = 5
+    lflowstate = .true.
     alfa_filter = afil
     ref_tem = rtem; reynolds = re; mach = ma
     difschm = dschm(1:4)
@@ -456,25 +461,51 @@ contains
   ! steploop (tgvsolver.F90:944-960): the whole run in one device call
   ! (statistics of every step come back at the end), then the stacal lines
   subroutine steploop
-    real(8), allocatable, target :: ke(:), en(:)
+    real(8), allocatable, target :: ke(:), en(:), di(:)
     integer :: n, it
     n = maxstep - nstep
     if (n <= 0) return
-    allocate(ke(n), en(n))
-    call hipcheck(astr_hip_rk3_steps(ctx, int(n, c_int), c_loc(ke), c_loc(en)), 'steploop')
+    allocate(ke(n), en(n), di(n))
+    if (lflowstate) then
+      call hipcheck(astr_hip_rk3_steps_stats(ctx, int(n, c_int), c_loc(ke), c_loc(en), c_loc(di)), &
+                    'steploop')
+    else
+      call hipcheck(astr_hip_rk3_steps(ctx, int(n, c_int), c_loc(ke), c_loc(en)), 'steploop')
+    end if
     do it = 1, n
       call stacal(ke(it), en(it))
+      if (lflowstate) call statout(ke(it), en(it), di(it))
       nstep = nstep + 1
       time = time + deltat
     end do
     if (statopen) flush(statfile)
+    if (fsopen) flush(fsfile)
   end subroutine steploop
+
+  ! statout for flowtype tgv (statistic.F90:777-859) with listinit and
+  ! listwrite (utility.F90:310-434): a title line of A20 words, then per
+  ! step nstep, time/ref_tim (ref_tim = 1 for nondimensional input),
+  ! kenergy, enstophy, dissipation
+  subroutine statout(kenergy, enstophy, dissipation)
+    real(8), intent(in) :: kenergy, enstophy, dissipation
+    if (.not. fsopen) then
+      open(newunit=fsfile, file='flowstate.dat')
+      fsopen = .true.
+      write(fsfile, '(5(1X,A20))') 'nstep', 'time', 'kenergy', 'enstophy', 'dissipation'
+    end if
+    write(fsfile, '(1X,I20,4(1X,E20.13E2))') nstep, time, kenergy, enstophy, dissipation
+  end subroutine statout
 
   subroutine solver_end
     if (c_associated(ctx)) call hipcheck(astr_hip_destroy(ctx), 'solver_end')
     ctx = c_null_ptr
     if (statopen) close(statfile)
     statopen = .false.
+    if (fsopen) then
+      close(fsfile)
+      print *, ' << flowstate.dat'
+    end if
+    fsopen = .false.
   end subroutine solver_end
 
 end module astr_hip_tgv

@@ -149,3 +149,18 @@ def test_datin_run_reproduces_state_ref_128(tmp_path):
     assert np.array_equal(got[:, 0], ref[:, 0])
     for c in (2, 3):
         assert np.abs(got[:, c] - ref[:, c]).max() / np.abs(ref[:, c]).max() < 1e-9
+    # flowstate.dat in the main path's layout (statistic.F90:777-859): the same
+    # steps, the same KE and enstrophy to the 14 digits it prints, and the
+    # dissipation rate of the same run through the C ABI
+    lines = open(tmp_path / "flowstate.dat").read().split("\n")
+    assert lines[0].split() == ["nstep", "time", "kenergy", "enstophy", "dissipation"]
+    assert len(lines[0]) == 5 * 21
+    fs = np.loadtxt(tmp_path / "flowstate.dat", skiprows=1)
+    assert fs.shape == (10, 5)
+    assert np.array_equal(fs[:, 0], got[:, 0])
+    for c in (2, 3):
+        assert np.abs(fs[:, c] / got[:, c] - 1).max() < 1e-12
+    f = A.Flow(A.tgv_config(128, deltat=1e-3))
+    f.init_tgv(0.0, 0)
+    ke, en, di = f.rk3(10, dissipation=True)
+    assert np.abs(fs[:, 4] / di - 1).max() < 1e-12
