@@ -353,6 +353,7 @@ const ShapeInfo kShapes[kNumShapes] = {
     {0, 256, 4, 64, 5, 5}, {1, 256, 4, 64, 9, 9}, {2, 256, 16, 16, 5, 5},
     {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5}, {5, 512, 8, 64, 5, 5},
     {6, 128, 4, 32, 9, 9}, {7, 128, 2, 64, 5, 5}, {8, 64, 1, 64, 5, 5}, {9, 64, 2, 32, 9, 9},
+    {10, 256, 16, 16, 9, 9},
 };
 
 bool shape_fits(const ShapeInfo& s, int n) {
@@ -412,7 +413,8 @@ hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir
     case 6: return launch_pass_s6(prog, minb, a, dir, st);
     case 7: return launch_pass_s7(prog, minb, a, dir, st);
     case 8: return launch_pass_s8(prog, minb, a, dir, st);
-    default: return launch_pass_s9(prog, minb, a, dir, st);
+    case 9: return launch_pass_s9(prog, minb, a, dir, st);
+    default: return launch_pass_s10(prog, minb, a, dir, st);
   }
 }
 
@@ -740,7 +742,8 @@ hipError_t deriv_lines_device(int n, long long nlines, const double* f, double* 
     case 6: LB(CfgG); break;
     case 7: LB(CfgH); break;
     case 8: LB(CfgI); break;
-    default: LB(CfgJ); break;
+    case 9: LB(CfgJ); break;
+    default: LB(CfgK); break;
   }
 #undef LB
   return hipGetLastError();
@@ -1213,7 +1216,8 @@ int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double
     case 6: LB(CfgG); break;
     case 7: LB(CfgH); break;
     case 8: LB(CfgI); break;
-    default: LB(CfgJ); break;
+    case 9: LB(CfgJ); break;
+    default: LB(CfgK); break;
   }
 #undef LB
   HIP_OK(hipGetLastError());
@@ -1375,7 +1379,8 @@ hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, in
     case 6: return launch_batch_s6(prog, minb, bl, dir, sa, nblocks, st);
     case 7: return launch_batch_s7(prog, minb, bl, dir, sa, nblocks, st);
     case 8: return launch_batch_s8(prog, minb, bl, dir, sa, nblocks, st);
-    default: return launch_batch_s9(prog, minb, bl, dir, sa, nblocks, st);
+    case 9: return launch_batch_s9(prog, minb, bl, dir, sa, nblocks, st);
+    default: return launch_batch_s10(prog, minb, bl, dir, sa, nblocks, st);
   }
 }
 
