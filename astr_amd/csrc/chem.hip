@@ -67,8 +67,15 @@ struct Mech {
 // ---------------------------------------------------------------------------
 // device evaluation
 // ---------------------------------------------------------------------------
+// Reaction orders other than 1 and 2 go through pow, kept out of line: the
+// inlined pow's temporaries set the register peak of every kernel that
+// evaluates rates, although orders 1 and 2 never reach it.
+__device__ __attribute__((noinline)) double conc_pow_general(double c, double nu) {
+  return pow(c, nu);
+}
+
 __device__ __forceinline__ double conc_pow(double c, double nu) {
-  return nu == 1.0 ? c : (nu == 2.0 ? c * c : pow(c, nu));
+  return nu == 1.0 ? c : (nu == 2.0 ? c * c : conc_pow_general(c, nu));
 }
 
 // Thermodynamics and kinetics of one state: molar concentrations from rho and
@@ -123,6 +130,11 @@ __device__ __forceinline__ void kinetics(const Mech* __restrict__ M, double T, d
   }
 }
 
+// NSA: the per-cell arrays' size.  NSA = NSMAX serves any mechanism (loops
+// to M->ns); an instance with NSA = the mechanism's species count keeps four
+// arrays of that size in registers instead of NSMAX, with the same arithmetic
+// in the same order.
+template <int NSA>
 __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
                                               const double* __restrict__ Tin,
                                               const double* __restrict__ RPin, int input_is_p,
@@ -131,9 +143,9 @@ __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long l
                                               double* __restrict__ hk) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  const int ns = M->ns, nr = M->nr;
+  const int ns = NSA == NSMAX ? M->ns : NSA, nr = M->nr;
   const double T = Tin[i];
-  double y[NSMAX], c[NSMAX], g[NSMAX], w[NSMAX];
+  double y[NSA], c[NSA], g[NSA], w[NSA];
   double winv = 0.0;
   for (int k = 0; k < ns; k++) {
     y[k] = Y[k * n + i];
@@ -145,13 +157,24 @@ __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long l
   double cp = 0.0, h = 0.0;
   for (int k = 0; k < ns; k++) {
     c[k] = rho * y[k] / M->W[k];
-    const double* a = T <= M->tmid[k] ? M->lo[k] : M->hi[k];
-    // NASA 7-coefficient polynomials
-    const double cpr = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
-    const double hrt = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 +
-                       a[4] * T4 / 5.0 + a[5] * invT;
-    const double sr = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 +
-                      a[4] * T4 / 4.0 + a[6];
+    // NASA 7-coefficient polynomials, evaluated with both ranges' coefficients
+    // (uniform, so they stay in scalar registers) and the cell's range picked
+    double cpr, hrt, sr;
+    {
+      double cr[2], hr[2], srr[2];
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const double* a = q == 0 ? M->lo[k] : M->hi[k];
+        cr[q] = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
+        hr[q] = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 + a[4] * T4 / 5.0 +
+                a[5] * invT;
+        srr[q] = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 + a[4] * T4 / 4.0 + a[6];
+      }
+      const int q = T <= M->tmid[k] ? 0 : 1;
+      cpr = q == 0 ? cr[0] : cr[1];
+      hrt = q == 0 ? hr[0] : hr[1];
+      sr = q == 0 ? srr[0] : srr[1];
+    }
     g[k] = hrt - sr;
     cp += y[k] * cpr / M->W[k];
     const double hks = hrt * (kRref / M->W[k]) * T;
@@ -1078,9 +1101,13 @@ int astr_hip_chem_rates_device(astr_chem* c, long long n, const double* T, const
   if (!T || !rho_or_p || !Y || !wdot) return cfail("null buffer");
   CHIP_OK(hipSetDevice(c->device));
   const int bs = 256;
-  hipLaunchKernelGGL(k_chem, dim3((unsigned)((n + bs - 1) / bs)), dim3(bs), 0,
-                     (hipStream_t)stream, c->dev, n, T, rho_or_p, input_is_p, Y, wdot, h_mix,
-                     cp_mix, h_k);
+  const dim3 grid((unsigned)((n + bs - 1) / bs));
+  if (c->host.ns == 9)
+    hipLaunchKernelGGL(k_chem<9>, grid, dim3(bs), 0, (hipStream_t)stream, c->dev, n, T, rho_or_p,
+                       input_is_p, Y, wdot, h_mix, cp_mix, h_k);
+  else
+    hipLaunchKernelGGL(k_chem<NSMAX>, grid, dim3(bs), 0, (hipStream_t)stream, c->dev, n, T, rho_or_p,
+                       input_is_p, Y, wdot, h_mix, cp_mix, h_k);
   CHIP_OK(hipGetLastError());
   return 0;
 }
