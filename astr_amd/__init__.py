@@ -52,7 +52,8 @@ EXPORTS = [
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
     "astr_hip_batch_rk3_steps", "astr_hip_batch_info", "astr_hip_set_last_split",
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
-    "astr_hip_gridgeom", "astr_hip_set_metrics", "astr_hip_qswap",
+    "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt",
+    "astr_hip_set_metrics", "astr_hip_qswap",
     "astr_hip_get_last_split",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
@@ -87,11 +88,14 @@ def lib():
         L.astr_hip_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.astr_hip_rk3_steps_stats.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.astr_hip_gridgeom.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.astr_hip_gridgeom_nt.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, _dp, _dp]
         L.astr_hip_qswap.argtypes = [C.c_void_p]
         L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
                                           C.c_double, C.c_double]
+        L.astr_hip_line_batch_nt.argtypes = [C.c_int, C.c_int, C.c_int, C.c_longlong, _dp, _dp,
+                                             C.c_double, C.c_double, C.c_double]
         L.astr_hip_profile_begin.argtypes = [C.c_void_p, C.c_int]
         L.astr_hip_profile_end.argtypes = [C.c_void_p, _dp, C.POINTER(C.c_int)]
         L.astr_hip_autotune.argtypes = [C.c_void_p, C.c_int]
@@ -219,16 +223,21 @@ class Flow:
         _chk(lib().astr_hip_qswap(self.h))
 
     # -- grid metrics (gridgeom, geom.F90:99-692) ------------------------------
-    def gridgeom(self, x):
+    def gridgeom(self, x, npdc=None):
         """Metrics of the coordinates x (3, km+1, jm+1, im+1), computed on the
         device: (jacob (km+1, jm+1, im+1), dxi (3, 3, km+1, jm+1, im+1)),
-        dxi[a, b] = d xi_a / d x_b."""
+        dxi[a, b] = d xi_a / d x_b.  npdc: boundary type per direction i, j,
+        k (3 periodic, the default; 4 physical at both ends)."""
         x = np.ascontiguousarray(x, dtype=np.float64)
         if x.shape != (3,) + self.shape:
             raise AstrError(f"x must have shape {(3,) + self.shape}")
         jac = np.empty(self.shape)
         buf = np.empty((3, 3) + self.shape)   # [b, a]: the C layout (a-1) + 3(b-1)
-        _chk(lib().astr_hip_gridgeom(self.h, _p(x), _p(jac), _p(buf)))
+        if npdc is None:
+            _chk(lib().astr_hip_gridgeom(self.h, _p(x), _p(jac), _p(buf)))
+        else:
+            nt = (C.c_int * 3)(*[int(v) for v in npdc])
+            _chk(lib().astr_hip_gridgeom_nt(self.h, nt, _p(x), _p(jac), _p(buf)))
         return jac, buf.transpose((1, 0, 2, 3, 4))
 
     def set_metrics(self, jac, dxi):
@@ -370,4 +379,15 @@ def line_batch(op, f, alfa_filter=0.49, beta_halo=1.11):
     n = f.shape[1] - 11
     out = np.empty((f.shape[0], n + 1))
     _chk(lib().astr_hip_line_batch(op, n, f.shape[0], _p(f), _p(out), alfa_filter, beta_halo))
+    return out
+
+
+def line_batch_nt(op, ntype, f, alfa_filter=0.49, beta_halo=1.11, beta_bound=0.98):
+    """line_batch with the boundary type ntype of the lines' ends (1: physical
+    boundary at 0, 2: at n, 3: interfaces, 4: physical at both ends)."""
+    f = np.ascontiguousarray(np.atleast_2d(f), dtype=np.float64)
+    n = f.shape[1] - 11
+    out = np.empty((f.shape[0], n + 1))
+    _chk(lib().astr_hip_line_batch_nt(op, ntype, n, f.shape[0], _p(f), _p(out), alfa_filter,
+                                      beta_halo, beta_bound))
     return out

@@ -316,12 +316,13 @@ __global__ void k_prims(FlowArgs a, double* __restrict__ out, int hm) {
 
 // ---------------------------------------------------------------------------
 // standalone line batch: op 0 = df_compact, 1 = compact_filter, on lines with
-// explicit halos f(-5:n+5) (the reference's per-line interface)
+// explicit halos f(-5:n+5) (the reference's per-line interface); BND with
+// the boundary types bnd (bit 0: physical boundary at 0, bit 1: at n)
 // ---------------------------------------------------------------------------
-template <class C, bool FILT>
+template <class C, bool FILT, bool BND = false>
 __global__ __launch_bounds__(C::NT) void k_lines(const double* __restrict__ f, double* __restrict__ out,
                                                  int n, long long nlines, const double* __restrict__ ac,
-                                                 FilterCoef fc, double alfa, double bhalo) {
+                                                 FilterCoef fc, int bnd) {
   extern __shared__ double smem[];
   double* T = smem;
   double* X = smem + C::TL * C::LT;
@@ -335,7 +336,7 @@ __global__ __launch_bounds__(C::NT) void k_lines(const double* __restrict__ f, d
   ChunkCoef<C::R, FILT> CC;
   load_chunk_coef<C::R, FILT>(CC, ac, n + (FILT ? 7 : 3), p * C::R);
   __syncthreads();
-  solve_line<C::P, C::R, FILT>(T + l * C::LT, X + l * C::LT, n, p, CC, fc);
+  solve_line<C::P, C::R, FILT, BND>(T + l * C::LT, X + l * C::LT, n, p, CC, fc, bnd);
   __syncthreads();
   for (int e = threadIdx.x; e < C::TL * (n + 1); e += C::NT) {
     const int l2 = e / (n + 1), r = e - l2 * (n + 1);
@@ -492,6 +493,56 @@ void filter_tables(int n, double alfa, double bh, std::vector<double>& out) {
   thomas_prep(a, c, out);
 }
 
+// boundary bits of a reference ntype (1: physical boundary at 0, 2: at n,
+// 3: interfaces, 4: both physical); -1 for another value
+int ntype_bits(int ntype) {
+  switch (ntype) {
+    case 1: return 1;
+    case 2: return 2;
+    case 3: return 0;
+    case 4: return 3;
+    default: return -1;
+  }
+}
+
+// derivative.F90:106-136 with physical ends, in the periodic form's rows
+// -1..n+1: a physical end's row beyond the boundary is decoupled (a = c = 0)
+// and its boundary rows take a = c = 2 (row 0) and 1/4 (row 1)
+void deriv_tables_nt(int n, int bits, std::vector<double>& out) {
+  const int m = n + 3;
+  std::vector<double> a(m, 1.0 / 3.0), c(m, 1.0 / 3.0);
+  a[0] = c[0] = 0.0;
+  a[m - 1] = c[m - 1] = 0.0;
+  if (bits & 1) {
+    a[1] = c[1] = 2.0;
+    a[2] = c[2] = 0.25;
+  }
+  if (bits & 2) {
+    a[m - 2] = c[m - 2] = 2.0;
+    a[m - 3] = c[m - 3] = 0.25;
+  }
+  thomas_prep(a, c, out);
+}
+
+// filter.F90:31-100 with physical ends (row 0 / n take 0.98, :47-66), in the
+// periodic form's rows -3..n+3; the three rows beyond a physical end are
+// decoupled
+void filter_tables_nt(int n, int bits, double alfa, double bh, std::vector<double>& out) {
+  const int m = n + 7;
+  std::vector<double> a(m, alfa), c(m, alfa);
+  a[0] = c[0] = bh;
+  a[m - 1] = c[m - 1] = bh;
+  if (bits & 1) {
+    for (int u = 0; u < 3; u++) a[u] = c[u] = 0.0;
+    a[3] = c[3] = 0.98;
+  }
+  if (bits & 2) {
+    for (int u = m - 3; u < m; u++) a[u] = c[u] = 0.0;
+    a[m - 4] = c[m - 4] = 0.98;
+  }
+  thomas_prep(a, c, out);
+}
+
 // filter_coefficient_cal (filter.F90:299-432): the coefficients ntype=3 uses;
 // c10 into fc, the 33 halo-row coefficients coefh(0:2,0:10) into h33
 void filter_coefs(double alfa, double bh, FilterCoef& fc, double* h33) {
@@ -517,6 +568,59 @@ void filter_coefs(double alfa, double bh, FilterCoef& fc, double* h33) {
       (5.0 - 10.0 * alfa) / 512.0, (-1.0 + 2.0 * alfa) / 1024.0};
   std::memcpy(h33, h, sizeof(h));
   fc.ch = nullptr;
+}
+
+// filter_coefficient_cal (filter.F90:299-432): the rows a physical boundary
+// uses, coefb(0:2,0:6) row-major, coef6i(0:3), coef8i(0:4), into b30
+void filter_coefs_b(double alfa, double bb, double* b30) {
+  const double b[30] = {
+      (63.0 + 1.0 * bb) / 64.0, (3.0 + 29.0 * bb) / 32.0, (-15.0 + 15.0 * bb) / 64.0,
+      (5.0 - 5.0 * bb) / 16.0, (-15.0 + 15.0 * bb) / 64.0, (3.0 - 3.0 * bb) / 32.0,
+      (-1.0 + 1.0 * bb) / 64.0,
+      (1.0 + 62.0 * alfa) / 64.0, (29.0 + 6.0 * alfa) / 32.0, (15.0 + 34.0 * alfa) / 64.0,
+      (-5.0 + 10.0 * alfa) / 16.0, (15.0 - 30.0 * alfa) / 64.0, (-3.0 + 6.0 * alfa) / 32.0,
+      (1.0 - 2.0 * alfa) / 64.0,
+      (-1.0 + 2.0 * alfa) / 64.0, (3.0 + 26.0 * alfa) / 32.0, (49.0 + 30.0 * alfa) / 64.0,
+      (5.0 + 6.0 * alfa) / 16.0, (-15.0 + 30.0 * alfa) / 64.0, (3.0 - 6.0 * alfa) / 32.0,
+      (-1.0 + 2.0 * alfa) / 64.0,
+      (11.0 + 10.0 * alfa) / 32.0, (15.0 + 34.0 * alfa) / 64.0, (-3.0 + 6.0 * alfa) / 32.0,
+      (1.0 - 2.0 * alfa) / 64.0,
+      (93.0 + 70.0 * alfa) / 256.0, (7.0 + 18.0 * alfa) / 32.0, (-7.0 + 14.0 * alfa) / 64.0,
+      (1.0 - 2.0 * alfa) / 32.0, (-1.0 + 2.0 * alfa) / 256.0};
+  std::memcpy(b30, b, sizeof(b));
+}
+
+// k_lines on stream st with the launch shape sh: op 0 derivative, 1 filter;
+// bits the boundary bits (0: the periodic form's kernel)
+hipError_t launch_lines(int sh, int op, int bits, const double* f, double* out, int n, long long nlines,
+                        const double* lu, const FilterCoef& fc, hipStream_t st) {
+  const ShapeInfo& s = kShapes[sh];
+  const long long nb = (nlines + s.TL - 1) / s.TL;
+  const size_t lds = (size_t)2 * s.TL * line_stride(s.P, s.R, s.TL) * sizeof(double);
+#define LB(Cf)                                                                                          \
+  if (op == 0 && !bits)                                                                                 \
+    hipLaunchKernelGGL((k_lines<Cf, false>), dim3(nb), dim3(Cf::NT), lds, st, f, out, n, nlines, lu, fc, 0); \
+  else if (op == 0)                                                                                     \
+    hipLaunchKernelGGL((k_lines<Cf, false, true>), dim3(nb), dim3(Cf::NT), lds, st, f, out, n, nlines, lu, fc, bits); \
+  else if (!bits)                                                                                       \
+    hipLaunchKernelGGL((k_lines<Cf, true>), dim3(nb), dim3(Cf::NT), lds, st, f, out, n, nlines, lu, fc, 0); \
+  else                                                                                                  \
+    hipLaunchKernelGGL((k_lines<Cf, true, true>), dim3(nb), dim3(Cf::NT), lds, st, f, out, n, nlines, lu, fc, bits);
+  switch (sh) {
+    case 0: LB(CfgA); break;
+    case 1: LB(CfgB); break;
+    case 2: LB(CfgC); break;
+    case 3: LB(CfgD); break;
+    case 4: LB(CfgE); break;
+    case 5: LB(CfgF); break;
+    case 6: LB(CfgG); break;
+    case 7: LB(CfgH); break;
+    case 8: LB(CfgI); break;
+    case 9: LB(CfgJ); break;
+    default: LB(CfgK); break;
+  }
+#undef LB
+  return hipGetLastError();
 }
 
 int check_ctx(astr_hip_ctx* c) {
@@ -722,31 +826,14 @@ const int kGeomTerms[18][6] = {
 };
 
 // derivative lines of length n already in device memory (f: n+11 per line,
-// out: n+1 per line) on stream st, with the LU table lu of n
+// out: n+1 per line) on stream st, with the LU table lu of n and the
+// boundary bits
 hipError_t deriv_lines_device(int n, long long nlines, const double* f, double* out, const double* lu,
-                              hipStream_t st) {
+                              int bits, hipStream_t st) {
   const int sh = pick_shape(n);
   if (sh < 0) return hipErrorInvalidValue;
-  const ShapeInfo& s = kShapes[sh];
-  const long long nb = (nlines + s.TL - 1) / s.TL;
-  const size_t lds = (size_t)2 * s.TL * line_stride(s.P, s.R, s.TL) * sizeof(double);
   FilterCoef fc{};
-#define LB(Cf) hipLaunchKernelGGL((k_lines<Cf, false>), dim3(nb), dim3(Cf::NT), lds, st, f, out, n, nlines, lu, fc, 0.0, 0.0)
-  switch (sh) {
-    case 0: LB(CfgA); break;
-    case 1: LB(CfgB); break;
-    case 2: LB(CfgC); break;
-    case 3: LB(CfgD); break;
-    case 4: LB(CfgE); break;
-    case 5: LB(CfgF); break;
-    case 6: LB(CfgG); break;
-    case 7: LB(CfgH); break;
-    case 8: LB(CfgI); break;
-    case 9: LB(CfgJ); break;
-    default: LB(CfgK); break;
-  }
-#undef LB
-  return hipGetLastError();
+  return launch_lines(sh, 0, bits, f, out, n, nlines, lu, fc, st);
 }
 
 }  // namespace
@@ -1177,59 +1264,51 @@ int astr_hip_get_last_split(astr_hip_ctx* c, int* split) {
   return 0;
 }
 
-int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double* out,
-                        double alfa_filter, double beta_halo) {
-  if (n < 8 || nlines < 0 || (op != 0 && op != 1)) return fail("bad argument to astr_hip_line_batch");
+int astr_hip_line_batch_nt(int op, int ntype, int n, long long nlines, const double* f, double* out,
+                           double alfa_filter, double beta_halo, double beta_bound) {
+  const int bits = ntype_bits(ntype);
+  if (n < 8 || nlines < 0 || (op != 0 && op != 1) || bits < 0)
+    return fail("bad argument to astr_hip_line_batch_nt");
   if (nlines == 0) return 0;
   if (!f || !out) return fail("null buffer");
   const int sh = pick_shape(n);
   if (sh < 0) return fail("no launch shape fits this line length");
   FilterCoef fc;
-  double h33[33];
-  filter_coefs(alfa_filter, beta_halo, fc, h33);
-  std::vector<double> tab(h33, h33 + 33), t2;
-  if (op == 0) deriv_tables(n, t2);
-  else filter_tables(n, alfa_filter, beta_halo, t2);
+  double h63[kBndCoefs];
+  filter_coefs(alfa_filter, beta_halo, fc, h63);
+  filter_coefs_b(alfa_filter, beta_bound, h63 + 33);
+  std::vector<double> tab(h63, h63 + kBndCoefs), t2;
+  if (op == 0) deriv_tables_nt(n, bits, t2);
+  else filter_tables_nt(n, bits, alfa_filter, beta_halo, t2);
   tab.insert(tab.end(), t2.begin(), t2.end());
   DevBuf bt, bf, bo;
   HIP_OK(bt.alloc(tab.size() * sizeof(double)));
   HIP_OK(bf.alloc((size_t)nlines * (n + 11) * sizeof(double)));
   HIP_OK(bo.alloc((size_t)nlines * (n + 1) * sizeof(double)));
-  double *dt = bt.p, *df = bf.p, *dout = bo.p;
-  HIP_OK(hipMemcpy(dt, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
-  HIP_OK(hipMemcpy(df, f, (size_t)nlines * (n + 11) * sizeof(double), hipMemcpyHostToDevice));
-  const ShapeInfo& s = kShapes[sh];
-  const long long nb = (nlines + s.TL - 1) / s.TL;
-  const size_t lds = (size_t)2 * s.TL * line_stride(s.P, s.R, s.TL) * sizeof(double);
-  fc.ch = dt;
-  const double* lu = dt + 33;
-#define LB(Cf)                                                                            \
-  if (op == 0) hipLaunchKernelGGL((k_lines<Cf, false>), dim3(nb), dim3(Cf::NT), lds, 0, df, dout, n, nlines, lu, fc, alfa_filter, beta_halo); \
-  else hipLaunchKernelGGL((k_lines<Cf, true>), dim3(nb), dim3(Cf::NT), lds, 0, df, dout, n, nlines, lu, fc, alfa_filter, beta_halo);
-  switch (sh) {
-    case 0: LB(CfgA); break;
-    case 1: LB(CfgB); break;
-    case 2: LB(CfgC); break;
-    case 3: LB(CfgD); break;
-    case 4: LB(CfgE); break;
-    case 5: LB(CfgF); break;
-    case 6: LB(CfgG); break;
-    case 7: LB(CfgH); break;
-    case 8: LB(CfgI); break;
-    case 9: LB(CfgJ); break;
-    default: LB(CfgK); break;
-  }
-#undef LB
-  HIP_OK(hipGetLastError());
-  HIP_OK(hipMemcpy(out, dout, (size_t)nlines * (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(bt.p, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(bf.p, f, (size_t)nlines * (n + 11) * sizeof(double), hipMemcpyHostToDevice));
+  fc.ch = bt.p;
+  HIP_OK(launch_lines(sh, op, bits, bf.p, bo.p, n, nlines, bt.p + kBndCoefs, fc, 0));
+  HIP_OK(hipMemcpy(out, bo.p, (size_t)nlines * (n + 1) * sizeof(double), hipMemcpyDeviceToHost));
   return 0;
 }
 
-int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* dxi) {
+int astr_hip_line_batch(int op, int n, long long nlines, const double* f, double* out,
+                        double alfa_filter, double beta_halo) {
+  return astr_hip_line_batch_nt(op, 3, n, nlines, f, out, alfa_filter, beta_halo, 0.98);
+}
+
+int astr_hip_gridgeom_nt(astr_hip_ctx* c, const int* npdc, const double* x, double* jacob, double* dxi) {
   if (check_ctx(c)) return -1;
-  if (!x) return fail("null coordinates");
+  if (!x || !npdc) return fail("null coordinates or boundary types");
   HIP_OK(hipSetDevice(c->device));
   const int nd = c->a.ndir;
+  int bits[3] = {0, 0, 0};
+  for (int d = 0; d < nd; d++) {
+    if (npdc[d] != 3 && npdc[d] != 4)
+      return fail("gridgeom on one rank takes boundary type 3 (periodic) or 4 (physical at both ends)");
+    bits[d] = ntype_bits(npdc[d]);
+  }
   GeomGrid G;
   for (int d = 0; d < 3; d++) G.n[d] = c->a.n[d];
   G.st[0] = 1;
@@ -1251,7 +1330,7 @@ int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* d
   HIP_OK(bo.alloc(maxo * sizeof(double)));
   for (int d = 0; d < nd; d++) {
     std::vector<double> t;
-    deriv_tables(G.n[d], t);
+    deriv_tables_nt(G.n[d], bits[d], t);
     HIP_OK(bt[d].alloc(t.size() * sizeof(double)));
     HIP_OK(hipMemcpy(bt[d].p, t.data(), t.size() * sizeof(double), hipMemcpyHostToDevice));
   }
@@ -1265,7 +1344,7 @@ int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* d
     hipLaunchKernelGGL(k_geom_gather, dim3((unsigned)((ng + 255) / 256)), dim3(256), 0, st, bx.p, bdx.p, G,
                        dir, L, nl, bl.p);
     HIP_OK(hipGetLastError());
-    HIP_OK(deriv_lines_device(G.n[dir], nl, bl.p, bo.p, bt[dir].p, st));
+    HIP_OK(deriv_lines_device(G.n[dir], nl, bl.p, bo.p, bt[dir].p, bits[dir], st));
     hipLaunchKernelGGL(k_geom_scatter, dim3((unsigned)((ns + 255) / 256)), dim3(256), 0, st, bo.p, G, dir,
                        nl, dst, acc);
     HIP_OK(hipGetLastError());
@@ -1288,6 +1367,11 @@ int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* d
   if (dxi) HIP_OK(hipMemcpyAsync(dxi, bxi.p, 9 * G.fs * sizeof(double), hipMemcpyDeviceToHost, st));
   HIP_OK(hipStreamSynchronize(st));
   return 0;
+}
+
+int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* dxi) {
+  const int npdc[3] = {3, 3, 3};
+  return astr_hip_gridgeom_nt(c, npdc, x, jacob, dxi);
 }
 
 int astr_hip_qswap(astr_hip_ctx* c) {

@@ -177,6 +177,82 @@ __device__ __forceinline__ void load_chunk_coef(ChunkCoef<R, FILT>& C,
   }
 }
 
+// Right-hand-side rows of a line whose ends may be physical boundaries
+// (bnd bit 0: physical boundary at r=0, bit 1: at r=n; ntype 1, 2, 4 of the
+// reference), or v where row r is an interior row.  An end that is not
+// physical is an interface and takes the periodic form's rows.  The system
+// keeps the periodic form's unknowns -OFF..n+OFF; at a physical end the rows
+// beyond the boundary are decoupled (a = c = 0, right-hand side 0, so they
+// solve to 0) and the boundary row's sub/super-diagonal is the reference's
+// (a = c, the value the reference's solve reads), so that the
+// recurrences over rows 0..n are the reference's.
+//   derivative (compact_fd_rhs, derivative.F90:229-289): row 0 the
+//     3rd-order biased -5/2 f0 + 2 f1 + 1/2 f2, row 1 the 4th-order
+//     compact 3/4 (f2 - f0), mirrored at n;
+//   filter (compact_filter_rhs, filter.F90:175-262): rows 0..2 the one-sided
+//     coefb(k, 0..6), row 3 the 6th-order and row 4 the 8th-order central
+//     rows, mirrored at n; interface halo rows from coefh.
+// fc.ch holds coefh (33), then coefb rows 0..2 (21), coef6i (4), coef8i (5).
+constexpr int kBndCoefs = 63;
+template <bool FILT>
+__device__ double boundary_row(const double* __restrict__ Tl, int n, int r, int bnd,
+                               const FilterCoef& fc, double v) {
+  const bool bl = bnd & 1, br = bnd & 2;
+  if constexpr (!FILT) {
+    if (bl) {
+      if (r < 0) return 0.0;
+      if (r == 0) return -2.5 * Tl[0] + 2.0 * Tl[1] + 0.5 * Tl[2];
+      if (r == 1) return 0.75 * (Tl[2] - Tl[0]);
+    }
+    if (br) {
+      if (r > n) return 0.0;
+      if (r == n) return 2.5 * Tl[n] - 2.0 * Tl[n - 1] - 0.5 * Tl[n - 2];
+      if (r == n - 1) return 0.75 * (Tl[n] - Tl[n - 2]);
+    }
+    return v;
+  } else {
+    const double* __restrict__ ch = fc.ch;
+    const double* __restrict__ cb = ch + 33;
+    const double* __restrict__ c6 = ch + 54;
+    const double* __restrict__ c8 = ch + 58;
+    if (r < -3 || r > n + 3) return v;  // a chunk slot past the system's rows
+    int j = -1;  // a 6th/8th-order row centred at j
+    if (bl) {
+      if (r < 0) return 0.0;
+      if (r <= 2) {
+        double s = 0.0;
+        for (int t = 0; t <= 6; t++) s = s + cb[r * 7 + t] * Tl[t];
+        return s;
+      }
+      if (r == 3 || r == 4) j = r;
+    } else if (r < 0) {
+      double s = 0.0;
+      for (int t = 0; t <= 10; t++) s = s + ch[(r + 3) * 11 + t] * Tl[t - 5];
+      return s;
+    }
+    if (br) {
+      if (r > n) return 0.0;
+      if (r >= n - 2) {
+        double s = 0.0;
+        for (int t = 0; t <= 6; t++) s = s + cb[(n - r) * 7 + t] * Tl[n - t];
+        return s;
+      }
+      if (r == n - 3 || r == n - 4) j = r;
+    } else if (r > n) {
+      double s = 0.0;
+      for (int t = 0; t <= 10; t++) s = s + ch[(n + 3 - r) * 11 + t] * Tl[n + 5 - t];
+      return s;
+    }
+    if (j < 0) return v;
+    const bool six = (bl && j == 3) || (br && j == n - 3);
+    if (six)
+      return c6[0] * (Tl[j] + Tl[j]) + c6[1] * (Tl[j + 1] + Tl[j - 1]) + c6[2] * (Tl[j + 2] + Tl[j - 2]) +
+             c6[3] * (Tl[j + 3] + Tl[j - 3]);
+    return c8[0] * (Tl[j] + Tl[j]) + c8[1] * (Tl[j + 1] + Tl[j - 1]) + c8[2] * (Tl[j + 2] + Tl[j - 2]) +
+           c8[3] * (Tl[j + 3] + Tl[j - 3]) + c8[4] * (Tl[j + 4] + Tl[j - 4]);
+  }
+}
+
 // Solve one line held in LDS.
 //   T  : input line, row r at T[kLineOff + r]; rows -5..n+5 hold the line
 //        and its halo, the padding rows hold 0
@@ -185,10 +261,13 @@ __device__ __forceinline__ void load_chunk_coef(ChunkCoef<R, FILT>& C,
 //             rows at -1 and n+1 (derivative.F90:250-304).
 // FILT=true : 10th-order filter, unknowns -3..n+3 (M=n+7), halo rows
 //             -3..-1, n+1..n+3 from coefh (filter.F90:206-283).
-template <int P, int R, bool FILT>
+// BND=true: the ends carry the boundary types bnd (boundary_row); the filter
+//             then also resets a physical end to its input value
+//             (filter.F90:141-142).
+template <int P, int R, bool FILT, bool BND = false>
 __device__ __forceinline__ void solve_line(const double* __restrict__ T, double* __restrict__ X,
                                            int n, int p, const ChunkCoef<R, FILT>& C,
-                                           const FilterCoef& fc) {
+                                           const FilterCoef& fc, int bnd = 0) {
   constexpr int OFF = FILT ? 3 : 1;
   constexpr int H = FILT ? 5 : 3;
   constexpr int W = R + 2 * H;
@@ -223,7 +302,10 @@ __device__ __forceinline__ void solve_line(const double* __restrict__ T, double*
     }
   }
 
-  if constexpr (FILT) {
+  if constexpr (BND) {
+#pragma unroll
+    for (int k = 0; k < R; k++) y[k] = boundary_row<FILT>(T + kLineOff, n, u0 + k - OFF, bnd, fc, y[k]);
+  } else if constexpr (FILT) {
     // the six halo rows: lane q<6 of each line evaluates row q (-3,-2,-1,
     // n+1,n+2,n+3) over f(-5..5) or f(n+5..n-5) and leaves it in X[n+1+q],
     // a row of the output line that nothing reads before this solve writes
@@ -300,7 +382,11 @@ __device__ __forceinline__ void solve_line(const double* __restrict__ T, double*
 #pragma unroll
   for (int k = 0; k < R; k++) {
     const int r = u0 + k - OFF;
-    if (r >= 0 && r <= n) X[r] = y[k];
+    double v = y[k];
+    if constexpr (BND && FILT) {
+      if (((bnd & 1) && r == 0) || ((bnd & 2) && r == n)) v = T[kLineOff + r];
+    }
+    if (r >= 0 && r <= n) X[r] = v;
   }
 }
 

@@ -124,6 +124,25 @@ module astr_hip_capi
       type(c_ptr), value :: ctx, x, jacob, dxi
       integer(c_int) :: astr_hip_gridgeom
     end function
+    ! npdc(1:3) = npdci, npdcj, npdck: 3 periodic, 4 physical at both ends
+    function astr_hip_gridgeom_nt(ctx, npdc, x, jacob, dxi) bind(C, name='astr_hip_gridgeom_nt')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), intent(in) :: npdc(3)
+      type(c_ptr), value :: x, jacob, dxi
+      integer(c_int) :: astr_hip_gridgeom_nt
+    end function
+    ! fds%central (op 0) / compact_filter (op 1) on nlines lines f(-5:n+5)
+    ! with the boundary type ntype of their ends
+    function astr_hip_line_batch_nt(op, ntype, n, nlines, f, out, alfa_filter, beta_halo, &
+                                    beta_bound) bind(C, name='astr_hip_line_batch_nt')
+      import :: c_int, c_long_long, c_double, c_ptr
+      integer(c_int), value :: op, ntype, n
+      integer(c_long_long), value :: nlines
+      type(c_ptr), value :: f, out
+      real(c_double), value :: alfa_filter, beta_halo, beta_bound
+      integer(c_int) :: astr_hip_line_batch_nt
+    end function
     function astr_hip_set_metrics(ctx, jacob, dxi) bind(C, name='astr_hip_set_metrics')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, jacob, dxi

@@ -119,6 +119,13 @@ int astr_hip_qswap(astr_hip_ctx *ctx);
  * conservative form of jacob * dxi (:395-540), divided by jacob (:664-668).
  * jacob(0:im,0:jm,0:km) and dxi(0:im,0:jm,0:km,1:3,1:3) are nullable. */
 int astr_hip_gridgeom(astr_hip_ctx *ctx, const double *x, double *jacob, double *dxi);
+/* gridgeom with a boundary type per direction, npdc[d] = npdci/npdcj/npdck
+ * (commvar.F90; fds_compact_i/j/k of geom.F90:128-161): 3 periodic, 4
+ * physical at both ends, the two a single rank has.  A physical direction
+ * takes the compact derivative's boundary rows (derivative.F90:128-139,
+ * 229-289) and reads no halo. */
+int astr_hip_gridgeom_nt(astr_hip_ctx *ctx, const int *npdc, const double *x, double *jacob,
+                         double *dxi);
 /* Metrics for the passes, in gridgeom's layout.  This build's passes take one
  * 1/dx per direction, so only the metrics of a uniform Cartesian grid
  * (dxi diagonal and constant, jacob = 1/prod(dxi), both to 1e-12) are
@@ -189,6 +196,19 @@ int astr_hip_batch_info(astr_hip_batch *batch, int *ngroups, int *launches_per_s
  * f holds nlines lines of f(-5:n+5) (n+11 values each), out nlines x (n+1). */
 int astr_hip_line_batch(int op, int n, long long nlines, const double *f, double *out,
                         double alfa_filter, double beta_halo);
+/* The same with the boundary type ntype of a block's ends, as
+ * fd_scheme_initiate / compact_filter_initiate take it
+ * (derivative.F90:63-158, filter.F90:31-100): 1 physical boundary at i=0,
+ * 2 at i=n, 3 interfaces at both ends (astr_hip_line_batch), 4 physical at
+ * both ends.  A physical end takes the derivative's 3rd-order biased and
+ * 4th-order compact rows (derivative.F90:229-289) and the filter's
+ * one-sided coefb rows, 6th- and 8th-order rows and reset of the boundary
+ * value (filter.F90:141-142, 175-250); its halo values are not read.
+ * beta_bound is filter_coefficient_cal's beter_bouond (comsolver.F90:121
+ * passes 0.98); the filter's LHS takes beta_halo at an interface and 0.98
+ * at a physical end. */
+int astr_hip_line_batch_nt(int op, int ntype, int n, long long nlines, const double *f,
+                           double *out, double alfa_filter, double beta_halo, double beta_bound);
 
 /* ------------------------------------------------------------------------
  * Chemical source terms (per cell), replacing the reference's Cantera calls

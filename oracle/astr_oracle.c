@@ -19,6 +19,13 @@
  *   or_filter_*, or_compact_filter             -> reference Fortran
  *        compact_filter; dense solve; constant preservation; transfer
  *        function T(w)=sum c_m 2cos(mw)/(1+2a cos w) on interior rows.
+ *   or_df_compact_nt, or_compact_filter_nt     -> the reference's own
+ *        fd_scheme_initiate/df_compact and compact_filter_initiate/
+ *        compact_filter with ntype 1,2,4 (oracle/_ref), bit for bit;
+ *        exactness of the closed derivative system on cubics (its
+ *        3rd-order boundary row fails on quartics); the boundary
+ *        filter's preservation of polynomials of degree <= 5 with the
+ *        main path's beta_bound 0.98.
  *   or_rhscal, or_filterq, or_rk3_steps,
  *   or_stacal, or_tgv_init                      -> reference Fortran
  *        rhscal/filterq/rk3/stacal run through oracle/_ref on the same
@@ -213,6 +220,185 @@ void or_compact_filter(int n, double alfa, double bh, double bb, const double *f
   or_thomas_solve(m, ac, d, x);
   for (int j = 0; j <= n; j++) ff[j] = x[j + 3];
   free(a); free(c); free(ac); free(d); free(x);
+}
+
+/* ------------------------------------------------------------------ */
+/* Lines with physical boundaries: boundary type ntype (1: physical    */
+/* boundary at 0, 2: at n, 3: interfaces at both ends, 4: physical at  */
+/* both ends), as fd_scheme_initiate / compact_fd_rhs                  */
+/* (derivative.F90:63-158, :210-306) and compact_filter_initiate /     */
+/* compact_filter / compact_filter_rhs (filter.F90:31-285) set them up. */
+/* Unknowns x(i_0..i_m) stored at u = r - i_0.                         */
+/* ------------------------------------------------------------------ */
+static int left_phys(int ntype) { return ntype == 1 || ntype == 4; }
+static int right_phys(int ntype) { return ntype == 2 || ntype == 4; }
+
+int or_df_compact_nt(int n, int ntype, const double *f, double *df) {
+  if (ntype < 1 || ntype > 4) return -1;
+  const double *F = f + HM;
+  const int i0 = left_phys(ntype) ? 0 : -1;      /* derivative.F90:74-93 */
+  const int im_ = right_phys(ntype) ? n : n + 1;
+  const int m = im_ - i0 + 1;
+  double *a = malloc(sizeof(double) * m), *c = malloc(sizeof(double) * m);
+  double *ac = malloc(sizeof(double) * 3 * m), *d = malloc(sizeof(double) * m);
+  double *x = malloc(sizeof(double) * m);
+  /* LHS, derivative.F90:106-136 */
+  for (int u = 0; u < m; u++) { a[u] = num1d3; c[u] = num1d3; }
+  a[0] = c[0] = 0.0;
+  a[m - 1] = c[m - 1] = 0.0;
+  if (left_phys(ntype)) { a[0] = c[0] = 2.0; a[1] = c[1] = 0.25; }
+  if (right_phys(ntype)) { a[m - 1] = c[m - 1] = 2.0; a[m - 2] = c[m - 2] = 0.25; }
+  /* RHS, derivative.F90:229-304 */
+  int is, ie, j;
+  if (left_phys(ntype)) {
+    is = i0 + 2;
+    j = i0 + 1;
+    d[j - i0] = 0.75 * (F[j + 1] - F[j - 1]);
+    j = i0;
+    d[j - i0] = -2.5 * F[j] + 2.0 * F[j + 1] + 0.5 * F[j + 2];
+  } else {
+    is = i0 + 1;
+    j = i0;
+    double var1 = F[j + 1] - F[j - 1], var2 = F[j + 2] - F[j - 2], var3 = F[j + 3] - F[j - 3];
+    d[j - i0] = 0.75 * var1 - 0.15 * var2 + num1d60 * var3;
+  }
+  if (right_phys(ntype)) {
+    ie = im_ - 2;
+    j = im_ - 1;
+    d[j - i0] = 0.75 * (F[j + 1] - F[j - 1]);
+    j = im_;
+    d[j - i0] = 2.5 * F[j] - 2.0 * F[j - 1] - 0.5 * F[j - 2];
+  } else {
+    ie = im_ - 1;
+    j = im_;
+    double var1 = F[j + 1] - F[j - 1], var2 = F[j + 2] - F[j - 2], var3 = F[j + 3] - F[j - 3];
+    d[j - i0] = 0.75 * var1 - 0.15 * var2 + num1d60 * var3;
+  }
+  for (j = is; j <= ie; j++) {
+    double var1 = F[j + 1] - F[j - 1], var2 = F[j + 2] - F[j - 2];
+    d[j - i0] = num7d9 * var1 + num1d36 * var2;
+  }
+  or_thomas_prep(m, a, c, ac);
+  or_thomas_solve(m, ac, d, x);
+  for (j = 0; j <= n; j++) df[j] = x[j - i0];
+  free(a); free(c); free(ac); free(d); free(x);
+  return 0;
+}
+
+/* filter_coefficient_cal (filter.F90:299-432): the rows the physical
+ * boundary uses, coefb(0:2,0:6) row-major as coefb[k*7+j], coef6i(0:3),
+ * coef8i(0:4) */
+void or_filter_coefs_b(double alfa, double bb, double *coefb, double *coef6i, double *coef8i) {
+  coef6i[0] = (11.0 + 10.0 * alfa) / 32.0;
+  coef6i[1] = (15.0 + 34.0 * alfa) / 64.0;
+  coef6i[2] = (-3.0 + 6.0 * alfa) / 32.0;
+  coef6i[3] = (1.0 - 2.0 * alfa) / 64.0;
+  coef8i[0] = (93.0 + 70.0 * alfa) / 256.0;
+  coef8i[1] = (7.0 + 18.0 * alfa) / 32.0;
+  coef8i[2] = (-7.0 + 14.0 * alfa) / 64.0;
+  coef8i[3] = (1.0 - 2.0 * alfa) / 32.0;
+  coef8i[4] = (-1.0 + 2.0 * alfa) / 256.0;
+  double *b0 = coefb, *b1 = coefb + 7, *b2 = coefb + 14;
+  b2[0] = (-1.0 + 2.0 * alfa) / 64.0;
+  b2[1] = (3.0 + 26.0 * alfa) / 32.0;
+  b2[2] = (49.0 + 30.0 * alfa) / 64.0;
+  b2[3] = (5.0 + 6.0 * alfa) / 16.0;
+  b2[4] = (-15.0 + 30.0 * alfa) / 64.0;
+  b2[5] = (3.0 - 6.0 * alfa) / 32.0;
+  b2[6] = (-1.0 + 2.0 * alfa) / 64.0;
+  b1[0] = (1.0 + 62.0 * alfa) / 64.0;
+  b1[1] = (29.0 + 6.0 * alfa) / 32.0;
+  b1[2] = (15.0 + 34.0 * alfa) / 64.0;
+  b1[3] = (-5.0 + 10.0 * alfa) / 16.0;
+  b1[4] = (15.0 - 30.0 * alfa) / 64.0;
+  b1[5] = (-3.0 + 6.0 * alfa) / 32.0;
+  b1[6] = (1.0 - 2.0 * alfa) / 64.0;
+  b0[0] = (63.0 + 1.0 * bb) / 64.0;
+  b0[1] = (3.0 + 29.0 * bb) / 32.0;
+  b0[2] = (-15.0 + 15.0 * bb) / 64.0;
+  b0[3] = (5.0 - 5.0 * bb) / 16.0;
+  b0[4] = (-15.0 + 15.0 * bb) / 64.0;
+  b0[5] = (3.0 - 3.0 * bb) / 32.0;
+  b0[6] = (-1.0 + 1.0 * bb) / 64.0;
+}
+
+/* The LHS takes beta_halo at an interface end (filter.F90:90-94 with the
+ * main path's beter_halo) and 0.98 at a physical end (filter.F90:47-66). */
+int or_compact_filter_nt(int n, int ntype, double alfa, double bh, double bb, const double *f,
+                         double *ff) {
+  if (ntype < 1 || ntype > 4) return -1;
+  const double *F = f + HM;
+  const int i0 = left_phys(ntype) ? 0 : -3;
+  const int im_ = right_phys(ntype) ? n : n + 3;
+  const double b0 = left_phys(ntype) ? 0.98 : bh, bm = right_phys(ntype) ? 0.98 : bh;
+  const int m = im_ - i0 + 1;
+  double c10[6], ch[33], cb[21], c6[4], c8[5];
+  or_filter_coefs(alfa, bh, bb, c10, ch);
+  or_filter_coefs_b(alfa, bb, cb, c6, c8);
+  double *a = malloc(sizeof(double) * m), *c = malloc(sizeof(double) * m);
+  double *ac = malloc(sizeof(double) * 3 * m), *d = malloc(sizeof(double) * m);
+  double *x = malloc(sizeof(double) * m);
+  for (int u = 0; u < m; u++) { a[u] = alfa; c[u] = alfa; }
+  a[0] = c[0] = b0;
+  a[m - 1] = c[m - 1] = bm;
+  int is, ie, j, k;
+  if (left_phys(ntype)) {        /* filter.F90:175-205 */
+    is = i0 + 5;
+    for (k = i0; k <= i0 + 2; k++) {
+      double v = 0.0;
+      for (j = 0; j <= 6; j++) v = v + cb[(k - i0) * 7 + j] * F[i0 + j];
+      d[k - i0] = v;
+    }
+    j = i0 + 3;
+    d[j - i0] = c6[0] * (F[j] + F[j]) + c6[1] * (F[j + 1] + F[j - 1]) +
+                c6[2] * (F[j + 2] + F[j - 2]) + c6[3] * (F[j + 3] + F[j - 3]);
+    j = i0 + 4;
+    d[j - i0] = c8[0] * (F[j] + F[j]) + c8[1] * (F[j + 1] + F[j - 1]) +
+                c8[2] * (F[j + 2] + F[j - 2]) + c8[3] * (F[j + 3] + F[j - 3]) +
+                c8[4] * (F[j + 4] + F[j - 4]);
+  } else {                       /* filter.F90:207-219 */
+    is = i0 + 3;
+    for (k = i0; k <= i0 + 2; k++) {
+      double v = 0.0;
+      for (j = 0; j <= 10; j++) v = v + ch[(k - i0) * 11 + j] * F[i0 - 2 + j];
+      d[k - i0] = v;
+    }
+  }
+  if (right_phys(ntype)) {       /* filter.F90:221-250 */
+    ie = im_ - 5;
+    for (k = im_ - 2; k <= im_; k++) {
+      double v = 0.0;
+      for (j = 0; j <= 6; j++) v = v + cb[(im_ - k) * 7 + j] * F[im_ - j];
+      d[k - i0] = v;
+    }
+    j = im_ - 3;
+    d[j - i0] = c6[0] * (F[j] + F[j]) + c6[1] * (F[j + 1] + F[j - 1]) +
+                c6[2] * (F[j + 2] + F[j - 2]) + c6[3] * (F[j + 3] + F[j - 3]);
+    j = im_ - 4;
+    d[j - i0] = c8[0] * (F[j] + F[j]) + c8[1] * (F[j + 1] + F[j - 1]) +
+                c8[2] * (F[j + 2] + F[j - 2]) + c8[3] * (F[j + 3] + F[j - 3]) +
+                c8[4] * (F[j + 4] + F[j - 4]);
+  } else {                       /* filter.F90:252-262 */
+    ie = im_ - 3;
+    for (k = im_ - 2; k <= im_; k++) {
+      double v = 0.0;
+      for (j = 0; j <= 10; j++) v = v + ch[(im_ - k) * 11 + j] * F[im_ + 2 - j];
+      d[k - i0] = v;
+    }
+  }
+  for (j = is; j <= ie; j++) {   /* filter.F90:271-283 */
+    double var0 = F[j] + F[j], var1 = F[j + 1] + F[j - 1], var2 = F[j + 2] + F[j - 2];
+    double var3 = F[j + 3] + F[j - 3], var4 = F[j + 4] + F[j - 4], var5 = F[j + 5] + F[j - 5];
+    d[j - i0] = c10[0] * var0 + c10[1] * var1 + c10[2] * var2 + c10[3] * var3 + c10[4] * var4 +
+                c10[5] * var5;
+  }
+  or_thomas_prep(m, a, c, ac);
+  or_thomas_solve(m, ac, d, x);
+  for (j = 0; j <= n; j++) ff[j] = x[j - i0];
+  if (left_phys(ntype)) ff[0] = F[0];   /* filter.F90:141-142 */
+  if (right_phys(ntype)) ff[n] = F[n];
+  free(a); free(c); free(ac); free(d); free(x);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */

@@ -40,6 +40,13 @@ int  or_deriv_nrows(int n);
 void or_deriv_lhs(int n, double *a, double *c);
 void or_deriv_rhs(int n, const double *f, double *d);
 void or_df_compact(int n, const double *f, double *df);
+/* ntype 1: physical boundary at 0, 2: at n, 3: interfaces, 4: both physical;
+ * return -1 for another ntype */
+int or_df_compact_nt(int n, int ntype, const double *f, double *df);
+void or_filter_coefs_b(double alfa, double beta_bound, double *coefb, double *coef6i,
+                       double *coef8i);
+int or_compact_filter_nt(int n, int ntype, double alfa, double beta_halo, double beta_bound,
+                         const double *f, double *ff);
 int  or_filter_nrows(int n);
 void or_filter_coefs(double alfa, double beta_halo, double beta_bound,
                      double *coef10i, double *coefh);

@@ -1,7 +1,10 @@
 """Grid metrics (TEST INFRASTRUCTURE ONLY), restated from the reference's
-gridgeom (src/geom.F90:99-692) for a single-rank, fully periodic grid.  The
-derivative is the C restatement's df_compact (oracle/oracle.py), itself
-pinned against the reference's own lines.
+gridgeom (src/geom.F90:99-692) for a single rank, each direction periodic
+(npdc 3) or bounded by physical boundaries at both ends (npdc 4).  The
+derivative is the C restatement's df_compact / df_compact_nt
+(oracle/oracle.py), itself pinned against the reference's own lines; a
+physical direction reads no halo, so the halo extensions below only fill
+slots its boundary rows ignore.
 
 Arrays are numpy (k, j, i) views of the reference's x(0:im,0:jm,0:km,1:3):
 x has shape (3, km+1, jm+1, im+1); dx[m][d] and dxi[a][b] are 0-based
@@ -38,12 +41,13 @@ def _lines(arr, d):
     return a
 
 
-def deriv(ext_lines, d, shape):
+def deriv(ext_lines, d, shape, ntype=3):
     """fds%central along direction d of lines already extended to
-    f(-hm:n+hm); ext_lines has direction d last."""
+    f(-hm:n+hm); ext_lines has direction d last; ntype the direction's
+    boundary type."""
     out = np.empty(ext_lines.shape[:-1] + (ext_lines.shape[-1] - 2 * HM,))
     for idx in np.ndindex(*ext_lines.shape[:-1]):
-        out[idx] = O.df_compact(ext_lines[idx])
+        out[idx] = O.df_compact(ext_lines[idx]) if ntype == 3 else O.df_compact_nt(ext_lines[idx], ntype)
     return np.moveaxis(out, -1, AXIS[d])
 
 
@@ -67,9 +71,10 @@ TERMS = [
 ]
 
 
-def gridgeom(x):
-    """(jacob, dxi, dx) of the coordinates x (3, km+1, jm+1, im+1).
-    km = 0 selects the 2-D form (ndims = 2)."""
+def gridgeom(x, npdc=(3, 3, 3)):
+    """(jacob, dxi, dx) of the coordinates x (3, km+1, jm+1, im+1), npdc the
+    boundary type of directions i, j, k.  km = 0 selects the 2-D form
+    (ndims = 2)."""
     x = np.asarray(x, dtype=np.float64)
     shape = x.shape[1:]
     ndims = 3 if shape[0] > 1 else 2
@@ -77,7 +82,7 @@ def gridgeom(x):
     # dx(m,d) = d x_m / d xi_d (geom.F90:128-161)
     for d in range(ndims):
         for m in range(3):
-            dx[m, d] = deriv(_extend(x[m], d, _ext_offset), d, shape)
+            dx[m, d] = deriv(_extend(x[m], d, _ext_offset), d, shape, npdc[d])
     if ndims == 3:
         # geom.F90:353-371, summed left to right
         jac = (dx[0, 0] * dx[1, 1] * dx[2, 2] + dx[0, 1] * dx[1, 2] * dx[2, 0]
@@ -91,7 +96,7 @@ def gridgeom(x):
             xa = _extend(x[a], d, _ext_offset)
             xb = _extend(x[b], d, _ext_offset)
             phi = 0.5 * (dxa * xb - dxb * xa)
-            dxi[r, m] = dxi[r, m] + deriv(phi, d, shape)
+            dxi[r, m] = dxi[r, m] + deriv(phi, d, shape, npdc[d])
     else:
         # geom.F90:372-373, 522-530
         jac = dx[0, 0] * dx[1, 1] - dx[0, 1] * dx[1, 0]
@@ -105,13 +110,14 @@ def gridgeom(x):
     return jac, dxi, dx
 
 
-def metric_identity(jac, dxi):
+def metric_identity(jac, dxi, npdc=(3, 3, 3)):
     """The discrete metric identities sum_r D_r (J dxi(r,m)), m = 1..3
-    (the reference's `can`, geom.F90:549-580), with periodic halos."""
+    (the reference's `can`, geom.F90:549-580), with periodic halos or a
+    physical direction's boundary rows."""
     shape = jac.shape
     ndims = 3 if shape[0] > 1 else 2
     out = np.zeros((3,) + shape)
     for m in range(3):
         for r in range(ndims):
-            out[m] += deriv(_extend(jac * dxi[r, m], r, _ext_periodic), r, shape)
+            out[m] += deriv(_extend(jac * dxi[r, m], r, _ext_periodic), r, shape, npdc[r])
     return out

@@ -65,6 +65,10 @@ def lib():
         L.or_diss_rate.argtypes = [C.c_void_p, _dp]
         L.or_df_compact.argtypes = [C.c_int, _dp, _dp]
         L.or_compact_filter.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.or_df_compact_nt.argtypes = [C.c_int, C.c_int, _dp, _dp]
+        L.or_compact_filter_nt.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                           _dp, _dp]
+        L.or_filter_coefs_b.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
         L.or_gauss.restype = C.c_double
         L.or_gauss.argtypes = [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32]
         L.or_philox.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
@@ -195,6 +199,28 @@ def compact_filter(f, alfa=0.49, beta_halo=1.11, beta_bound=1.09):
     n = f.size - 2 * HM - 1
     out = np.zeros(n + 1)
     lib().or_compact_filter(n, alfa, beta_halo, beta_bound, _p(f), _p(out))
+    return out
+
+
+def df_compact_nt(f, ntype):
+    """df_compact of a line f(-hm:n+hm) with boundary type ntype (1: physical
+    boundary at 0, 2: at n, 3: interfaces, 4: both physical)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = f.size - 2 * HM - 1
+    out = np.zeros(n + 1)
+    if lib().or_df_compact_nt(n, ntype, _p(f), _p(out)):
+        raise ValueError(f"ntype {ntype}")
+    return out
+
+
+def compact_filter_nt(f, ntype, alfa=0.49, beta_halo=1.11, beta_bound=0.98):
+    """compact_filter of a line with boundary type ntype; the defaults are
+    the main path's coefficients (comsolver.F90:121)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    n = f.size - 2 * HM - 1
+    out = np.zeros(n + 1)
+    if lib().or_compact_filter_nt(n, ntype, alfa, beta_halo, beta_bound, _p(f), _p(out)):
+        raise ValueError(f"ntype {ntype}")
     return out
 
 

@@ -55,6 +55,42 @@ contains
     deallocate(s%a, s%c, s%ac)
   end subroutine ref_line_filter
 
+  ! one line of df_compact (derivative.F90:63-306) with boundary type ntype
+  ! (1: physical boundary at 0, 2: at n, 3: interfaces, 4: both physical)
+  subroutine ref_line_deriv_nt(n, ntype, f, df) bind(C, name="ref_line_deriv_nt")
+    use commtype,   only : compact_scheme
+    use derivative, only : fd_scheme_initiate, fds, compact_central
+    integer(c_int), value :: n, ntype
+    real(c_double), intent(in) :: f(-5:n+5)
+    real(c_double), intent(out) :: df(0:n)
+    type(compact_scheme) :: s
+    if (.not. allocated(fds)) allocate(compact_central :: fds)
+    call fd_scheme_initiate(asolver=s, scheme='643c', ntype=ntype, dim=n, dir=1)
+    df = fds%central(s, f=f, dim=n)
+    deallocate(s%a, s%c, s%ac)
+  end subroutine ref_line_deriv_nt
+
+  ! one line of compact_filter (filter.F90:31-285) with boundary type ntype;
+  ! the coefficients are set once per process from (alfa, beta_halo, beta_bound)
+  subroutine ref_line_filter_nt(n, ntype, alfa, bh, bb, f, ff) bind(C, name="ref_line_filter_nt")
+    use commtype, only : compact_scheme
+    use commvar,  only : alfa_filter
+    use filter,   only : compact_filter_initiate, filter_coefficient_cal,     &
+                         compact_filter, coef10i
+    integer(c_int), value :: n, ntype
+    real(c_double), value :: alfa, bh, bb
+    real(c_double), intent(in) :: f(-5:n+5)
+    real(c_double), intent(out) :: ff(0:n)
+    type(compact_scheme) :: s
+    alfa_filter = alfa
+    if (.not. allocated(coef10i)) then
+      call filter_coefficient_cal(alfa=alfa, beter_halo=bh, beter_bouond=bb)
+    endif
+    call compact_filter_initiate(afilter=s, ntype=ntype, dim=n)
+    ff = compact_filter(afilter=s, f=f, dim=n)
+    deallocate(s%a, s%c, s%ac)
+  end subroutine ref_line_filter_nt
+
   ! set up the miniapp for an im x jm x km TGV box (tgvsolver.F90:152-261)
   subroutine ref_field_init(im_, jm_, km_, dt) bind(C, name="ref_field_init")
     use commvar,   only : im, jm, km, numq, difschm, alfa_filter, deltat

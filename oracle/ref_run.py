@@ -34,10 +34,17 @@ def main():
         for r in range(lines.shape[0]):
             f = np.ascontiguousarray(lines[r])
             o = np.zeros(n + 1)
-            if task == "line_deriv":
+            nt = int(z.get("ntype", 3))
+            if task == "line_deriv" and "ntype" not in z:
                 L.ref_line_deriv(C.c_int(n), p(f), p(o))
-            else:
+            elif task == "line_deriv":
+                L.ref_line_deriv_nt(C.c_int(n), C.c_int(nt), p(f), p(o))
+            elif "ntype" not in z:
                 L.ref_line_filter(C.c_int(n), C.c_double(float(z.get("alfa", 0.49))), p(f), p(o))
+            else:
+                L.ref_line_filter_nt(C.c_int(n), C.c_int(nt), C.c_double(float(z["alfa"])),
+                                     C.c_double(float(z["beta_halo"])),
+                                     C.c_double(float(z["beta_bound"])), p(f), p(o))
             outs[r] = o
         res["out"] = outs
     else:
