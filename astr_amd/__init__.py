@@ -52,7 +52,7 @@ EXPORTS = [
     "astr_hip_batch_create", "astr_hip_batch_destroy", "astr_hip_batch_autotune",
     "astr_hip_batch_rk3_steps", "astr_hip_batch_info", "astr_hip_set_last_split",
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
-    "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt",
+    "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt", "astr_hip_updatefvar",
     "astr_hip_set_metrics", "astr_hip_qswap",
     "astr_hip_get_last_split",
 ]
@@ -90,6 +90,7 @@ def lib():
         L.astr_hip_gridgeom.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.astr_hip_gridgeom_nt.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, _dp, _dp]
         L.astr_hip_qswap.argtypes = [C.c_void_p]
+        L.astr_hip_updatefvar.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]
         L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
@@ -221,6 +222,16 @@ class Flow:
         """The main path's single-rank qswap: duplicate-plane averages along
         i, then j, then k (parallel.F90:4876-4884, 5018-5030, 5159-5170)."""
         _chk(lib().astr_hip_qswap(self.h))
+
+    def updatefvar(self, hm=0):
+        """updatefvar (fludyna.F90:191-242): (rho, vel, prs, tmp) of the
+        current q, numpy (k, j, i) views with halos of depth hm; vel has a
+        leading component axis of 3."""
+        shp = tuple(m + 2 * hm for m in self.shape)
+        rho, prs, tmp = np.empty(shp), np.empty(shp), np.empty(shp)
+        vel = np.empty((3,) + shp)
+        _chk(lib().astr_hip_updatefvar(self.h, _p(rho), _p(vel), _p(prs), _p(tmp), hm))
+        return rho, vel, prs, tmp
 
     # -- grid metrics (gridgeom, geom.F90:99-692) ------------------------------
     def gridgeom(self, x, npdc=None):

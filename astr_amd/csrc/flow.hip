@@ -1033,6 +1033,26 @@ int astr_hip_download(astr_hip_ctx* c, double* q, double* prims, int hm) {
   return 0;
 }
 
+int astr_hip_updatefvar(astr_hip_ctx* c, double* rho, double* vel, double* prs, double* tmp, int hm) {
+  if (check_ctx(c)) return -1;
+  if (bad_halo(c->a, hm)) return fail("bad argument to astr_hip_updatefvar");
+  const FlowArgs& a = c->a;
+  const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
+  DevBuf pb;
+  HIP_OK(pb.alloc(dn * 6 * sizeof(double)));
+  hipLaunchKernelGGL(k_prims, dense_grid(a, hm), dim3(64), 0, c->stream, a, pb.p, hm);
+  HIP_OK(hipGetLastError());
+  // k_prims writes rho, u, v, w, p, T as consecutive fields: vel(...,1:3) is
+  // fields 1..3 in the column-major layout
+  const size_t fb = dn * sizeof(double);
+  if (rho) HIP_OK(hipMemcpyAsync(rho, pb.p, fb, hipMemcpyDeviceToHost, c->stream));
+  if (vel) HIP_OK(hipMemcpyAsync(vel, pb.p + dn, 3 * fb, hipMemcpyDeviceToHost, c->stream));
+  if (prs) HIP_OK(hipMemcpyAsync(prs, pb.p + 4 * dn, fb, hipMemcpyDeviceToHost, c->stream));
+  if (tmp) HIP_OK(hipMemcpyAsync(tmp, pb.p + 5 * dn, fb, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
+}
+
 int astr_hip_filterq(astr_hip_ctx* c) {
   if (check_ctx(c)) return -1;
   for (int d = 0; d < c->a.ndir; d++)

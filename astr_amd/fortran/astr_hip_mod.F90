@@ -143,6 +143,12 @@ module astr_hip_capi
       real(c_double), value :: alfa_filter, beta_halo, beta_bound
       integer(c_int) :: astr_hip_line_batch_nt
     end function
+    function astr_hip_updatefvar(ctx, rho, vel, prs, tmp, hm) bind(C, name='astr_hip_updatefvar')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rho, vel, prs, tmp
+      integer(c_int), value :: hm
+      integer(c_int) :: astr_hip_updatefvar
+    end function
     function astr_hip_set_metrics(ctx, jacob, dxi) bind(C, name='astr_hip_set_metrics')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx, jacob, dxi

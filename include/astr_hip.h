@@ -86,6 +86,12 @@ int astr_hip_init_vortex(astr_hip_ctx *ctx, double beta, double xc, double yc, d
 int astr_hip_upload(astr_hip_ctx *ctx, const double *q, int hm);
 int astr_hip_download(astr_hip_ctx *ctx, double *q, double *prims, int hm);
 
+/* updatefvar: fludyna.F90:191-242 / q2fvar tgvsolver.F90:92-121 -- the
+ * primitives of the current q into the reference's separate arrays
+ * rho(-hm:im+hm,-hm:jm+hm,-hm:km+hm), vel(...,1:3), prs, tmp (each
+ * nullable), periodic halos of depth hm. */
+int astr_hip_updatefvar(astr_hip_ctx *ctx, double *rho, double *vel, double *prs, double *tmp, int hm);
+
 /* filterq: tgvsolver.F90:791-883 (x, y, z sweeps of the compact filter) */
 int astr_hip_filterq(astr_hip_ctx *ctx);
 

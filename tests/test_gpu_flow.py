@@ -80,6 +80,21 @@ def test_halo_download(dims):
     assert relmax(pr, s.prims[:, idx[0]][:, :, idx[1]][:, :, :, idx[2]]) < 1e-14
 
 
+@pytest.mark.parametrize("dims", [(20, 18, 16), (24, 22, 0)])
+def test_updatefvar_matches_oracle(dims):
+    """astr_hip_updatefvar fills rho, vel(1:3), prs, tmp as download's
+    primitives do (bit for bit, with and without halos) and as the oracle's
+    q2fvar does."""
+    f, s = make_pair(*dims)
+    for hm in (0, 3):
+        _, pr = f.download(prims=True, hm=hm)
+        rho, vel, prs, tmp = f.updatefvar(hm)
+        assert np.array_equal(rho, pr[0]) and np.array_equal(vel, pr[1:4])
+        assert np.array_equal(prs, pr[4]) and np.array_equal(tmp, pr[5])
+    rho, vel, prs, tmp = f.updatefvar()
+    assert relmax(np.concatenate([rho[None], vel, prs[None], tmp[None]]), s.prims) < 1e-14
+
+
 def test_init_tgv_matches_oracle():
     for amp in (0.0, 1e-3):
         f = A.Flow(A.tgv_config(24))
