@@ -108,9 +108,42 @@ template <class C>
 constexpr int lds_line_doubles() {
   return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
 }
+// After the halo slot: the derivative's LU factors ac1, ac2 of every chunk
+// row, staged once per tile, which the gradient/convective passes read back
+// before each solve instead of holding them in registers for the whole pass.
+template <class C>
+constexpr int lds_coef_offset() {
+  return lds_line_doubles<C>() + kHaloSlot;
+}
 template <class C>
 constexpr int lds_doubles() {
-  return lds_line_doubles<C>() + kHaloSlot;
+  return lds_coef_offset<C>() + 2 * C::P * C::R;
+}
+
+// Stage the derivative chunk factors of a line of M rows: c2 (ac2, 1 at row
+// 0) at [u], c1 (ac1, 0 at row M-1) at [P*R + u], 0 past the rows.
+template <class C>
+__device__ __forceinline__ double* stage_deriv_coefs(const double* __restrict__ ac, int M, double* smem) {
+  double* cs = smem + lds_coef_offset<C>();
+  for (int u = threadIdx.x; u < C::P * C::R; u += C::NT) {
+    const bool ok = u < M;
+    const int uu = ok ? u : 0;
+    double a1 = ac[uu], a2 = ac[M + uu];
+    if (u == 0) a2 = 1.0;
+    if (u == M - 1) a1 = 0.0;
+    cs[u] = ok ? a2 : 0.0;
+    cs[C::P * C::R + u] = ok ? a1 : 0.0;
+  }
+  return cs;
+}
+
+template <class C>
+__device__ __forceinline__ void lds_chunk_coef(ChunkCoef<C::R, false>& CD, const double* cs, int u0) {
+#pragma unroll
+  for (int k = 0; k < C::R; k++) {
+    CD.c2[k] = cs[u0 + k];
+    CD.c1[k] = cs[C::P * C::R + u0 + k];
+  }
 }
 
 // Copy the filter's halo-row coefficients into the tile's LDS; the barrier
@@ -308,8 +341,12 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     constexpr bool LAST = PROG == GC_LAST || PROG == GL_GRAD || PROG == GL_CONV;
     constexpr bool DO_GRAD = PROG != GL_CONV;
     constexpr bool DO_CONV = PROG != GL_GRAD;
-    ChunkCoef<C::R, false> CD;
-    load_chunk_coef<C::R, false>(CD, a.dac[dir], n + 3, p * C::R);
+    // the last pass, which holds the most registers, reads the chunk factors
+    // back from LDS before each solve; the others keep them in registers
+    ChunkCoef<C::R, false> CDr;
+    const double* cs = nullptr;
+    if constexpr (LAST) cs = stage_deriv_coefs<C>(a.dac[dir], n + 3, smem);
+    else load_chunk_coef<C::R, false>(CDr, a.dac[dir], n + 3, p * C::R);
     double qv[5][NPT];
 #pragma unroll
     for (int t = 0; t < NPT; t++)
@@ -337,7 +374,13 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         }
       }
       __syncthreads();
-      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      if constexpr (LAST) {
+        ChunkCoef<C::R, false> CD;
+        lds_chunk_coef<C>(CD, cs, p * C::R);
+        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      } else {
+        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CDr, a.fc);
+      }
       __syncthreads();
       const int fld = c < 3 ? c + 3 * dir : 9 + dir;
 #pragma unroll
@@ -427,7 +470,13 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         for (int t = 0; t < NPT; t++) qr[t] = ldg(dst, S.g[t]);
       }
       __syncthreads();
-      solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      if constexpr (LAST) {
+        ChunkCoef<C::R, false> CD;
+        lds_chunk_coef<C>(CD, cs, p * C::R);
+        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+      } else {
+        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CDr, a.fc);
+      }
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
