@@ -365,7 +365,7 @@ bool shape_fits(const ShapeInfo& s, int n) {
 size_t pass_lds_bytes(int shape, int /*prog*/) {
   const ShapeInfo& s = kShapes[shape];
   const long long tl = (long long)s.TL * line_stride(s.P, s.R, s.TL);
-  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8 + kHaloSlot + 2 * s.P * s.R;
+  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8 + kHaloSlot + 4 * s.P * s.R;
   return (size_t)one * sizeof(double);
 }
 

@@ -117,14 +117,15 @@ constexpr int lds_coef_offset() {
 }
 template <class C>
 constexpr int lds_doubles() {
-  return lds_coef_offset<C>() + 2 * C::P * C::R;
+  return lds_coef_offset<C>() + 4 * C::P * C::R;
 }
 
-// Stage the derivative chunk factors of a line of M rows: c2 (ac2, 1 at row
-// 0) at [u], c1 (ac1, 0 at row M-1) at [P*R + u], 0 past the rows.
-template <class C>
+// Stage the chunk factors of a line of M rows: c2 (ac2, 1 at row 0) at [u],
+// c1 (ac1, 0 at row M-1) at [P*R + u], 0 past the rows; the derivative's
+// at the coefficient offset, the filter's (FILT) after them.
+template <class C, bool FILT = false>
 __device__ __forceinline__ double* stage_deriv_coefs(const double* __restrict__ ac, int M, double* smem) {
-  double* cs = smem + lds_coef_offset<C>();
+  double* cs = smem + lds_coef_offset<C>() + (FILT ? 2 * C::P * C::R : 0);
   for (int u = threadIdx.x; u < C::P * C::R; u += C::NT) {
     const bool ok = u < M;
     const int uu = ok ? u : 0;
@@ -144,6 +145,20 @@ __device__ __forceinline__ void lds_chunk_coef(ChunkCoef<C::R, false>& CD, const
     CD.c2[k] = cs[u0 + k];
     CD.c1[k] = cs[C::P * C::R + u0 + k];
   }
+}
+
+// the filter's: also its last-row exception (load_chunk_coef in lines.hpp)
+template <class C>
+__device__ __forceinline__ void lds_chunk_coef_f(ChunkCoef<C::R, true>& CF, const double* cs, int u0,
+                                                 int M, double c3last) {
+#pragma unroll
+  for (int k = 0; k < C::R; k++) {
+    CF.c2[k] = cs[u0 + k];
+    CF.c1[k] = cs[C::P * C::R + u0 + k];
+  }
+  const int kl = M - 1 - u0;
+  CF.klast = kl >= 0 && kl < C::R ? kl : -1;
+  CF.c3last = c3last;
 }
 
 // Copy the filter's halo-row coefficients into the tile's LDS; the barrier
@@ -496,8 +511,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     // and its filter along dir follow its viscous sum at once, so one sum is
     // live at a time; density, which has no viscous term, goes first.
     constexpr bool RK = PROG == V_LAST_RK;
-    ChunkCoef<C::R, false> CD;
-    load_chunk_coef<C::R, false>(CD, a.dac[dir], n + 3, p * C::R);
+    // the chunk factors are read back from LDS before each solve, so they do
+    // not hold registers across the pass
+    const double* cs = stage_deriv_coefs<C>(a.dac[dir], n + 3, smem);
     const int c0 = dir == 0 ? 0 : (dir == 1 ? 1 : 2);
     const int c1 = dir == 0 ? 1 : (dir == 1 ? 3 : 4);
     const int c2 = dir == 0 ? 2 : (dir == 1 ? 4 : 5);
@@ -522,14 +538,18 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           for (int t = 0; t < NPT; t++) nx[t] = ldg(a.sg + col * fs, S.g[t]);
         }
         __syncthreads();
-        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        {
+          ChunkCoef<C::R, false> CD;
+          lds_chunk_coef<C>(CD, cs, p * C::R);
+          solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < NPT; t++) stg(dst, S.g[t], qr[t] + X[S.s[t]] * idn);
       }
     } else {
-      ChunkCoef<C::R, true> CF;
-      load_chunk_coef<C::R, true>(CF, a.fac[dir], n + 7, p * C::R);
+      const double* csf = stage_deriv_coefs<C, true>(a.fac[dir], n + 7, smem);
+      const double c3l = a.fac[dir][2 * (n + 7) + (n + 7) - 1];
       const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
       // Loads run two solves ahead: variable c+1's qrhs, q, qsave are issued
       // before variable c's filter solve, the next stress column before
@@ -551,7 +571,11 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         qs[t] = sa.stage == 0 ? 0.0 : ldg(a.qsave + fs, S.g[t]);
       }
       __syncthreads();
-      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+      {
+          ChunkCoef<C::R, true> CF;
+          lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
+          solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+        }
       __syncthreads();
 #pragma unroll
       for (int t = 0; t < NPT; t++) stg(a.q, S.g[t], X[S.s[t]]);
@@ -566,7 +590,11 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           for (int t = 0; t < NPT; t++) nx[t] = ldg(a.sg + col * fs, S.g[t]);
         }
         __syncthreads();
-        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        {
+          ChunkCoef<C::R, false> CD;
+          lds_chunk_coef<C>(CD, cs, p * C::R);
+          solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < NPT; t++) {
@@ -587,7 +615,11 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           }
         }
         __syncthreads();
-        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+        {
+          ChunkCoef<C::R, true> CF;
+          lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
+          solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+        }
         __syncthreads();
         double* dq = a.q + (1 + c) * fs;
 #pragma unroll
