@@ -362,10 +362,11 @@ bool shape_fits(const ShapeInfo& s, int n) {
 }
 
 // LDS bytes of a pass in a shape (lds_doubles in flow_dev.hpp)
-size_t pass_lds_bytes(int shape, int /*prog*/) {
+size_t pass_lds_bytes(int shape, int prog) {
   const ShapeInfo& s = kShapes[shape];
   const long long tl = (long long)s.TL * line_stride(s.P, s.R, s.TL);
-  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8 + kHaloSlot + 4 * s.P * s.R;
+  const long long one = (2 * tl > 2 * s.NT ? 2 * tl : 2 * s.NT) + kLineOff + 8 + kHaloSlot +
+                        coef_regions(prog) * 2 * s.P * s.R;
   return (size_t)one * sizeof(double);
 }
 

@@ -115,17 +115,24 @@ template <class C>
 constexpr int lds_coef_offset() {
   return lds_line_doubles<C>() + kHaloSlot;
 }
+// chunk-factor doubles a pass stages: the derivative's (2 P R) in the last
+// gradient pass and the viscous passes, the filter's (2 P R) in the filter
+// pass and the RK pass, none in the first two gradient passes
+constexpr int coef_regions(int prog) {
+  return prog == V_LAST_RK ? 2 : (prog == GC_FIRST || prog == GC_MID ? 0 : 1);
+}
 template <class C>
-constexpr int lds_doubles() {
-  return lds_coef_offset<C>() + 4 * C::P * C::R;
+constexpr int lds_doubles(int prog) {
+  return lds_coef_offset<C>() + coef_regions(prog) * 2 * C::P * C::R;
 }
 
-// Stage the chunk factors of a line of M rows: c2 (ac2, 1 at row 0) at [u],
-// c1 (ac1, 0 at row M-1) at [P*R + u], 0 past the rows; the derivative's
-// at the coefficient offset, the filter's (FILT) after them.
-template <class C, bool FILT = false>
-__device__ __forceinline__ double* stage_deriv_coefs(const double* __restrict__ ac, int M, double* smem) {
-  double* cs = smem + lds_coef_offset<C>() + (FILT ? 2 * C::P * C::R : 0);
+// Stage the chunk factors of a line of M rows into coefficient region `region`
+// (2 P R doubles each, after the halo slot): c2 (ac2, 1 at row 0) at [u],
+// c1 (ac1, 0 at row M-1) at [P*R + u], 0 past the rows.
+template <class C>
+__device__ __forceinline__ double* stage_chunk_coefs(const double* __restrict__ ac, int M, double* smem,
+                                                     int region) {
+  double* cs = smem + lds_coef_offset<C>() + region * 2 * C::P * C::R;
   for (int u = threadIdx.x; u < C::P * C::R; u += C::NT) {
     const bool ok = u < M;
     const int uu = ok ? u : 0;
@@ -360,7 +367,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     // back from LDS before each solve; the others keep them in registers
     ChunkCoef<C::R, false> CDr;
     const double* cs = nullptr;
-    if constexpr (LAST) cs = stage_deriv_coefs<C>(a.dac[dir], n + 3, smem);
+    if constexpr (LAST) cs = stage_chunk_coefs<C>(a.dac[dir], n + 3, smem, 0);
     else load_chunk_coef<C::R, false>(CDr, a.dac[dir], n + 3, p * C::R);
     double qv[5][NPT];
 #pragma unroll
@@ -513,7 +520,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     constexpr bool RK = PROG == V_LAST_RK;
     // the chunk factors are read back from LDS before each solve, so they do
     // not hold registers across the pass
-    const double* cs = stage_deriv_coefs<C>(a.dac[dir], n + 3, smem);
+    const double* cs = stage_chunk_coefs<C>(a.dac[dir], n + 3, smem, 0);
     const int c0 = dir == 0 ? 0 : (dir == 1 ? 1 : 2);
     const int c1 = dir == 0 ? 1 : (dir == 1 ? 3 : 4);
     const int c2 = dir == 0 ? 2 : (dir == 1 ? 4 : 5);
@@ -548,7 +555,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         for (int t = 0; t < NPT; t++) stg(dst, S.g[t], qr[t] + X[S.s[t]] * idn);
       }
     } else {
-      const double* csf = stage_deriv_coefs<C, true>(a.fac[dir], n + 7, smem);
+      const double* csf = stage_chunk_coefs<C>(a.fac[dir], n + 7, smem, 1);
       const double c3l = a.fac[dir][2 * (n + 7) + (n + 7) - 1];
       const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
       // Loads run two solves ahead: variable c+1's qrhs, q, qsave are issued
@@ -627,8 +634,8 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       }
     }
   } else {  // F_ONLY: filter of q along dir, in place (filterq, tgvsolver.F90:791-883)
-    ChunkCoef<C::R, true> CF;
-    load_chunk_coef<C::R, true>(CF, a.fac[dir], n + 7, p * C::R);
+    const double* csf = stage_chunk_coefs<C>(a.fac[dir], n + 7, smem, 0);
+    const double c3l = a.fac[dir][2 * (n + 7) + (n + 7) - 1];
     const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
     double nx[NPT];
 #pragma unroll
@@ -643,7 +650,11 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q + (m + 1) * fs, S.g[t]);
       }
       __syncthreads();
-      solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+      {
+        ChunkCoef<C::R, true> CF;
+        lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
+        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+      }
       __syncthreads();
       double* dst = a.q + m * fs;
 #pragma unroll
@@ -692,7 +703,7 @@ __global__ __launch_bounds__(C::NT, MINB) void k_pass_batch(BatchLaunch bl, int 
 template <class C, int MINB>
 hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageArgs& sa, int nblocks,
                           hipStream_t st) {
-  const size_t lds = (size_t)lds_doubles<C>() * sizeof(double);
+  const size_t lds = (size_t)lds_doubles<C>(prog) * sizeof(double);
   dim3 g(nblocks), b(C::NT);
   switch (prog) {
     case GC_FIRST: hipLaunchKernelGGL((k_pass_batch<C, GC_FIRST, MINB>), g, b, lds, st, bl, dir, sa); break;
@@ -711,7 +722,7 @@ hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageA
 template <class C, int MINB>
 hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
   const int nt = tiles_of(a.n, dir, C::TL);
-  const size_t lds = (size_t)lds_doubles<C>() * sizeof(double);
+  const size_t lds = (size_t)lds_doubles<C>(prog) * sizeof(double);
   dim3 g(nt), b(C::NT);
   switch (prog) {
     case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST, MINB>), g, b, lds, st, a, dir); break;
