@@ -637,18 +637,11 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     const double* csf = stage_chunk_coefs<C>(a.fac[dir], n + 7, smem, 0);
     const double c3l = a.fac[dir][2 * (n + 7) + (n + 7) - 1];
     const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
-    double nx[NPT];
-#pragma unroll
-    for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q, S.g[t]);
 #pragma unroll
     for (int m = 0; m < 5; m++) {
 #pragma unroll
       for (int t = 0; t < NPT; t++)
-        put<C>(T, S, t, nx[t]);
-      if (m < 4) {
-#pragma unroll
-        for (int t = 0; t < NPT; t++) nx[t] = ldg(a.q + (m + 1) * fs, S.g[t]);
-      }
+        put<C>(T, S, t, ldg(a.q + m * fs, S.g[t]));
       __syncthreads();
       {
         ChunkCoef<C::R, true> CF;
