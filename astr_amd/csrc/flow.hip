@@ -1240,8 +1240,26 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
     }
   }
   if (!rc) {
-    const int dl = a.ndir - 1;
-    c->split_last = c->tuned_ms[GL_GRAD][dl] + c->tuned_ms[GL_CONV][dl] < c->tuned_ms[GC_LAST][dl];
+    // the fused and the split last gradient pass, each timed inside whole
+    // right-hand sides with the selections above (in a step the split form's
+    // second kernel finds the first one's data in cache, which the per-pass
+    // timings do not see)
+    float form_ms[2] = {0.f, 0.f};
+    for (int sp = 0; sp < 2 && !rc; sp++) {
+      c->split_last = sp;
+      if (run_rhs(c, false, false, false)) {
+        rc = -1;
+        break;
+      }
+      hipError_t e = hipEventRecord(e0, c->stream);
+      for (int r = 0; r < reps && e == hipSuccess; r++)
+        if (run_rhs(c, false, false, false)) e = hipErrorUnknown;
+      if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      if (e == hipSuccess) e = hipEventElapsedTime(&form_ms[sp], e0, e1);
+      if (e != hipSuccess) rc = fail(std::string("autotune rhs: ") + hipGetErrorString(e));
+    }
+    if (!rc) c->split_last = form_ms[1] < form_ms[0];
   }
   a = keep;
   hipError_t e = hipMemcpyAsync(a.q, snap, qbytes, hipMemcpyDeviceToDevice, c->stream);
