@@ -354,7 +354,7 @@ const ShapeInfo kShapes[kNumShapes] = {
     {0, 256, 4, 64, 5, 5}, {1, 256, 4, 64, 9, 9}, {2, 256, 16, 16, 5, 5},
     {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5}, {5, 512, 8, 64, 5, 5},
     {6, 128, 4, 32, 9, 9}, {7, 128, 2, 64, 5, 5}, {8, 64, 1, 64, 5, 5}, {9, 64, 2, 32, 9, 9},
-    {10, 256, 16, 16, 9, 9},
+    {10, 256, 16, 16, 9, 9}, {11, 512, 8, 64, 9, 9},
 };
 
 bool shape_fits(const ShapeInfo& s, int n) {
@@ -416,7 +416,8 @@ hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir
     case 7: return launch_pass_s7(prog, minb, a, dir, st);
     case 8: return launch_pass_s8(prog, minb, a, dir, st);
     case 9: return launch_pass_s9(prog, minb, a, dir, st);
-    default: return launch_pass_s10(prog, minb, a, dir, st);
+    case 10: return launch_pass_s10(prog, minb, a, dir, st);
+    default: return launch_pass_s11(prog, minb, a, dir, st);
   }
 }
 
@@ -618,7 +619,8 @@ hipError_t launch_lines(int sh, int op, int bits, const double* f, double* out, 
     case 7: LB(CfgH); break;
     case 8: LB(CfgI); break;
     case 9: LB(CfgJ); break;
-    default: LB(CfgK); break;
+    case 10: LB(CfgK); break;
+    default: LB(CfgL); break;
   }
 #undef LB
   return hipGetLastError();
@@ -1503,7 +1505,8 @@ hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, in
     case 7: return launch_batch_s7(prog, minb, bl, dir, sa, nblocks, st);
     case 8: return launch_batch_s8(prog, minb, bl, dir, sa, nblocks, st);
     case 9: return launch_batch_s9(prog, minb, bl, dir, sa, nblocks, st);
-    default: return launch_batch_s10(prog, minb, bl, dir, sa, nblocks, st);
+    case 10: return launch_batch_s10(prog, minb, bl, dir, sa, nblocks, st);
+    default: return launch_batch_s11(prog, minb, bl, dir, sa, nblocks, st);
   }
 }
 
