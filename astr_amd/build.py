@@ -12,7 +12,7 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
 SOURCES = ["flow.hip", "chem.hip"]
 # passes.hip is compiled once per launch shape (-DASTR_SHAPE_ID=k)
-PASS_SHAPES = 12
+PASS_SHAPES = 13
 OUT = os.path.join(PKG, "libastr_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")

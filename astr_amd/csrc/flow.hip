@@ -355,6 +355,7 @@ const ShapeInfo kShapes[kNumShapes] = {
     {3, 256, 8, 32, 9, 9}, {4, 256, 8, 32, 5, 5}, {5, 512, 8, 64, 5, 5},
     {6, 128, 4, 32, 9, 9}, {7, 128, 2, 64, 5, 5}, {8, 64, 1, 64, 5, 5}, {9, 64, 2, 32, 9, 9},
     {10, 256, 16, 16, 9, 9}, {11, 512, 8, 64, 9, 9},
+    {12, 64, 1, 64, 9, 9},
 };
 
 bool shape_fits(const ShapeInfo& s, int n) {
@@ -417,7 +418,8 @@ hipError_t launch_pass(int shape, int prog, int minb, const FlowArgs& a, int dir
     case 8: return launch_pass_s8(prog, minb, a, dir, st);
     case 9: return launch_pass_s9(prog, minb, a, dir, st);
     case 10: return launch_pass_s10(prog, minb, a, dir, st);
-    default: return launch_pass_s11(prog, minb, a, dir, st);
+    case 11: return launch_pass_s11(prog, minb, a, dir, st);
+    default: return launch_pass_s12(prog, minb, a, dir, st);
   }
 }
 
@@ -620,7 +622,8 @@ hipError_t launch_lines(int sh, int op, int bits, const double* f, double* out, 
     case 8: LB(CfgI); break;
     case 9: LB(CfgJ); break;
     case 10: LB(CfgK); break;
-    default: LB(CfgL); break;
+    case 11: LB(CfgL); break;
+    default: LB(CfgM); break;
   }
 #undef LB
   return hipGetLastError();
@@ -1506,7 +1509,8 @@ hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, in
     case 8: return launch_batch_s8(prog, minb, bl, dir, sa, nblocks, st);
     case 9: return launch_batch_s9(prog, minb, bl, dir, sa, nblocks, st);
     case 10: return launch_batch_s10(prog, minb, bl, dir, sa, nblocks, st);
-    default: return launch_batch_s11(prog, minb, bl, dir, sa, nblocks, st);
+    case 11: return launch_batch_s11(prog, minb, bl, dir, sa, nblocks, st);
+    default: return launch_batch_s12(prog, minb, bl, dir, sa, nblocks, st);
   }
 }
 

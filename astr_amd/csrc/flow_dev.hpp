@@ -744,7 +744,8 @@ using CfgI = LineCfg<64, 1, 5, 5>;     // P=64, n <= 313: one line per single-wa
 using CfgJ = LineCfg<64, 2, 9, 9>;     // P=32, n <= 281: two lines per single-wavefront workgroup
 using CfgK = LineCfg<256, 16, 9, 9>;   // P=16, n <= 137: 128-byte row segments for mid-size lines
 using CfgL = LineCfg<512, 8, 9, 9>;    // P=64, n <= 569: CfgB's lines, 64-byte row segments
-constexpr int kNumShapes = 12;
+using CfgM = LineCfg<64, 1, 9, 9>;     // P=64, n <= 569: one long line per single-wavefront workgroup
+constexpr int kNumShapes = 13;
 constexpr int kMinbVariants = 2;       // MINB = 1, 2
 
 // one per shape, each in its own translation unit (passes.hip)
@@ -764,6 +765,7 @@ ASTR_DECLARE_SHAPE(8)
 ASTR_DECLARE_SHAPE(9)
 ASTR_DECLARE_SHAPE(10)
 ASTR_DECLARE_SHAPE(11)
+ASTR_DECLARE_SHAPE(12)
 #undef ASTR_DECLARE_SHAPE
 
 }  // namespace astr

@@ -1,10 +1,10 @@
 // passes.hip -- the pass kernels of one launch shape (single case and
-// batched), in register budgets MINB = 1 and 2.  Built once per shape with ASTR_SHAPE_ID = 0..11 so the
+// batched), in register budgets MINB = 1 and 2.  Built once per shape with ASTR_SHAPE_ID = 0..12 so the
 // shapes compile in parallel; flow.hip dispatches to launch_pass_s<ID>.
 #include "flow_dev.hpp"
 
 #ifndef ASTR_SHAPE_ID
-#error "build with -DASTR_SHAPE_ID=<0..11>"
+#error "build with -DASTR_SHAPE_ID=<0..12>"
 #endif
 
 namespace astr {
@@ -53,10 +53,14 @@ using ShapeCfg = CfgJ;
 #define ASTR_LAUNCH launch_pass_s10
 #define ASTR_BATCH launch_batch_s10
 using ShapeCfg = CfgK;
-#else
+#elif ASTR_SHAPE_ID == 11
 #define ASTR_LAUNCH launch_pass_s11
 #define ASTR_BATCH launch_batch_s11
 using ShapeCfg = CfgL;
+#else
+#define ASTR_LAUNCH launch_pass_s12
+#define ASTR_BATCH launch_batch_s12
+using ShapeCfg = CfgM;
 #endif
 
 hipError_t ASTR_LAUNCH(int prog, int minb, const FlowArgs& a, int dir, hipStream_t st) {
