@@ -108,16 +108,20 @@ template <class C>
 constexpr int lds_line_doubles() {
   return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
 }
-// After the halo slot: the derivative's LU factors ac1, ac2 of every chunk
-// row, staged once per tile, which the gradient/convective passes read back
-// before each solve instead of holding them in registers for the whole pass.
+// After the halo slot: the LU factors ac2, ac1 of every chunk row of the
+// pass's line systems, staged once per tile.  Each solve reads its chunk's
+// factors back just before it runs instead of holding them in registers for
+// the whole pass; the registers freed raise the waves per SIMD of the last
+// gradient pass, the viscous passes and the filter pass.
 template <class C>
 constexpr int lds_coef_offset() {
   return lds_line_doubles<C>() + kHaloSlot;
 }
-// chunk-factor doubles a pass stages: the derivative's (2 P R) in the last
-// gradient pass and the viscous passes, the filter's (2 P R) in the filter
-// pass and the RK pass, none in the first two gradient passes
+// Regions of 2 P R doubles a pass stages: one (the derivative's) in the last
+// gradient pass and the viscous passes, one (the filter's) in the filter
+// pass, two in the RK pass, none in the first two gradient passes, which
+// keep the factors in registers (there the freed registers would not reach
+// another wave per SIMD, and the reloads only cost time).
 constexpr int coef_regions(int prog) {
   return prog == V_LAST_RK ? 2 : (prog == GC_FIRST || prog == GC_MID ? 0 : 1);
 }
