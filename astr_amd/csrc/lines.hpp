@@ -68,6 +68,14 @@ __device__ __forceinline__ double dpp0(double v) {
   return mk64(l, h);
 }
 
+// DPP move with a row mask: rows outside ROWMASK receive `old`.
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dppm(double v, double old) {
+  const int l = __builtin_amdgcn_update_dpp(lo32(old), lo32(v), CTRL, ROWMASK, 0xf, false);
+  const int h = __builtin_amdgcn_update_dpp(hi32(old), hi32(v), CTRL, ROWMASK, 0xf, false);
+  return mk64(l, h);
+}
+
 __device__ __forceinline__ double readlane(double v, int lane) {
   return mk64(__builtin_amdgcn_readlane(lo32(v), lane), __builtin_amdgcn_readlane(hi32(v), lane));
 }
@@ -94,7 +102,13 @@ __device__ __forceinline__ double affine_scan(double& A, double& B) {
   ASTR_SCAN_STEP(4)
   ASTR_SCAN_STEP(8)
 #undef ASTR_SCAN_STEP
-  if constexpr (P >= 32) {
+  if constexpr (P >= 32 && FWD) {
+    // rows 1 and 3 compose the totals of lanes 15 and 47 (row_bcast:15)
+    const double Ac = dppm<0x142, 0xA>(A, 1.0);
+    const double Bc = dppm<0x142, 0xA>(B, 0.0);
+    B = fma(A, Bc, B);
+    A = A * Ac;
+  } else if constexpr (P >= 32) {
     // pair rows (0,1) and (2,3): the receiving row composes the other's total
     const double Aa = readlane(A, FWD ? 15 : 16), Ba = readlane(B, FWD ? 15 : 16);
     const double Ab = readlane(A, FWD ? 47 : 48), Bb = readlane(B, FWD ? 47 : 48);
@@ -105,29 +119,26 @@ __device__ __forceinline__ double affine_scan(double& A, double& B) {
     B = fma(A, Bc, B);
     A = A * Ac;
   }
-  if constexpr (P == 64) {
+  if constexpr (P == 64 && FWD) {
+    // rows 2 and 3 compose the total of lane 31 (row_bcast:31)
+    const double Aa = dppm<0x143, 0xC>(A, 1.0);
+    const double Ba = dppm<0x143, 0xC>(B, 0.0);
+    B = fma(A, Ba, B);
+    A = A * Aa;
+  } else if constexpr (P == 64) {
     const double Aa = readlane(A, FWD ? 31 : 32), Ba = readlane(B, FWD ? 31 : 32);
     const bool recv = FWD ? row >= 2 : row <= 1;
     B = fma(A, recv ? Ba : 0.0, B);
     A = A * (recv ? Aa : 1.0);
   }
-  // exclusive carry: B of the neighbour (inside the row by DPP, across rows
-  // by readlane of the neighbouring row's edge lane)
-  double c = dpp0<FWD ? 0x111 : 0x101>(B);
-  const double e0 = readlane(B, FWD ? 15 : 16);
-  const double e1 = readlane(B, FWD ? 31 : 32);
-  const double e2 = readlane(B, FWD ? 47 : 48);
-  const bool edge = FWD ? col == 0 : col == 15;
-  if (edge) {
-    // forward: row r reads lane 16r-1 (rows 1..3 -> e0,e1,e2); backward:
-    // row r reads lane 16(r+1) (rows 0..2 -> e0,e1,e2); only inside a segment
-    const int nrow = FWD ? row - 1 : row + 1;
-    const bool inseg = nrow >= 0 && nrow <= 3 && (nrow / (P / 16)) == (row / (P / 16));
-    const int idx = FWD ? nrow : row;
-    const double ev = idx == 0 ? e0 : (idx == 1 ? e1 : e2);
-    c = inseg ? ev : 0.0;
-  }
-  return c;
+  // exclusive carry: B of the neighbouring lane towards the segment start
+  // (wave_shr:1 forward, wave_shl:1 backward, across rows), 0 at the
+  // segment boundary
+  (void)row;
+  (void)col;
+  const double c = dpp0<FWD ? 0x138 : 0x130>(B);
+  const bool start = FWD ? (lane & (P - 1)) == 0 : (lane & (P - 1)) == P - 1;
+  return start ? 0.0 : c;
 }
 
 // LU factors of one chunk, held in registers, with encodings that make the
