@@ -14,7 +14,9 @@
 // (commfunc.F90:790-813) are first-order linear recurrences; each lane runs
 // them over its chunk with a zero carry, the carries across the P chunks are
 // composed by a Kogge-Stone scan over lanes (DPP row shifts inside each row
-// of 16 lanes, v_readlane across rows), and each lane then adds
+// of 16 lanes; across rows DPP row broadcasts for the forward scan and
+// v_readlane for the backward one, for which DPP has no downward broadcast;
+// the exclusive carries by DPP wavefront shifts), and each lane then adds
 // carry * (running product of the recurrence coefficients).  This is the
 // same linear map as the sequential sweep, re-associated.
 //
@@ -89,7 +91,7 @@ template <int P, bool FWD>
 __device__ __forceinline__ double affine_scan(double& A, double& B) {
   static_assert(P == 16 || P == 32 || P == 64, "segment must be 16, 32 or 64 lanes");
   const int lane = threadIdx.x & 63;
-  const int row = lane >> 4, col = lane & 15;
+  const int row = lane >> 4;
 #define ASTR_SCAN_STEP(S)                                                  \
   {                                                                        \
     const double A2 = dpp<(FWD ? 0x110 : 0x100) + S>(A, 1.0);              \
@@ -135,7 +137,6 @@ __device__ __forceinline__ double affine_scan(double& A, double& B) {
   // (wave_shr:1 forward, wave_shl:1 backward, across rows), 0 at the
   // segment boundary
   (void)row;
-  (void)col;
   const double c = dpp0<FWD ? 0x138 : 0x130>(B);
   const bool start = FWD ? (lane & (P - 1)) == 0 : (lane & (P - 1)) == P - 1;
   return start ? 0.0 : c;
