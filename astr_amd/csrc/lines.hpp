@@ -14,9 +14,10 @@
 // (commfunc.F90:790-813) are first-order linear recurrences; each lane runs
 // them over its chunk with a zero carry, the carries across the P chunks are
 // composed by a Kogge-Stone scan over lanes (DPP row shifts inside each row
-// of 16 lanes; across rows DPP row broadcasts for the forward scan and
-// v_readlane for the backward one, for which DPP has no downward broadcast;
-// the exclusive carries by DPP wavefront shifts), and each lane then adds
+// of 16 lanes; across rows DPP row broadcasts, with a wavefront shift first
+// for the backward scan's row pairs, and v_readlane only for the backward
+// scan's step between the 32-lane halves; the exclusive carries by DPP
+// wavefront shifts), and each lane then adds
 // carry * (running product of the recurrence coefficients).  This is the
 // same linear map as the sequential sweep, re-associated.
 //
@@ -111,13 +112,11 @@ __device__ __forceinline__ double affine_scan(double& A, double& B) {
     B = fma(A, Bc, B);
     A = A * Ac;
   } else if constexpr (P >= 32) {
-    // pair rows (0,1) and (2,3): the receiving row composes the other's total
-    const double Aa = readlane(A, FWD ? 15 : 16), Ba = readlane(B, FWD ? 15 : 16);
-    const double Ab = readlane(A, FWD ? 47 : 48), Bb = readlane(B, FWD ? 47 : 48);
-    const bool recv_a = FWD ? row == 1 : row == 0;
-    const bool recv_b = FWD ? row == 3 : row == 2;
-    const double Ac = recv_a ? Aa : (recv_b ? Ab : 1.0);
-    const double Bc = recv_a ? Ba : (recv_b ? Bb : 0.0);
+    // rows 0 and 2 compose the totals of lanes 16 and 48: wave_shl:1 brings
+    // them to lanes 15 and 47, row_newbcast:15 spreads those over rows 0, 2
+    const double As = dpp0<0x130>(A), Bs = dpp0<0x130>(B);
+    const double Ac = dppm<0x15F, 0x5>(As, 1.0);
+    const double Bc = dppm<0x15F, 0x5>(Bs, 0.0);
     B = fma(A, Bc, B);
     A = A * Ac;
   }
