@@ -60,6 +60,14 @@ struct DevBuf {
   hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes); }
 };
 
+// Failure injection for the tests of the allocation error paths: with the
+// environment variable ASTR_HIP_TEST_FAIL_ALLOC set to 1, the lazily grown
+// device buffers report a failed allocation instead of allocating.
+bool test_fail_alloc() {
+  const char* v = std::getenv("ASTR_HIP_TEST_FAIL_ALLOC");
+  return v && v[0] == '1';
+}
+
 #define HIP_OK(x)                                                                  \
   do {                                                                             \
     hipError_t e_ = (x);                                                           \
@@ -844,6 +852,69 @@ hipError_t deriv_lines_device(int n, long long nlines, const double* f, double* 
 
 }  // namespace
 
+namespace {
+
+// Device buffers of the per-step statistics: the per-tile partial sums the
+// last gradient pass leaves (2 per tile), the dissipation rate's block
+// partials (kDissBlocks) and the per-step results (kStepStats per step).
+struct StatsBufs {
+  double* part;
+  double* diss;
+  double* out;
+};
+
+// steploop/rk3 (tgvsolver.F90:944-960, :994-1082) with the statistics of
+// each step's first stage in the buffers sb; every statistics value is
+// written before it is read, so the buffers need no clearing
+int rk3_run(astr_hip_ctx* c, int nsteps, const StatsBufs& sb, double* ke, double* ens, double* diss) {
+  static const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
+  const bool want = ke || ens || diss;
+  FlowArgs& a = c->a;
+  double* const stats_keep = a.stats;
+  double* const diss_keep = c->diss_part;
+  a.stats = sb.part;
+  c->diss_part = sb.diss;
+  const int stats_prog = c->split_last ? GL_GRAD : GC_LAST;
+  const int ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[stats_prog][a.ndir - 1]].TL);
+  int rc = 0;
+  for (int it = 0; it < nsteps && !rc; it++) {
+    for (int s = 0; s < 3 && !rc; s++) {
+      a.stage = s;
+      a.rk_a = rk[s][0]; a.rk_b = rk[s][1]; a.rk_c = rk[s][2];
+      if (run_rhs(c, true, want && s == 0, false,
+                  diss && s == 0 ? sb.out + kStepStats * it + 2 : nullptr)) {
+        rc = -1;
+        break;
+      }
+      if (want && s == 0) {
+        hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, c->stream, sb.part, ntl,
+                           sb.out + kStepStats * it, c->ke_scale);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) rc = fail(std::string("k_stats_reduce: ") + hipGetErrorString(e));
+      }
+      const int nd = a.ndir;
+      for (int d = 0; d < nd - 1 && !rc; d++)
+        if (pass(c, F_ONLY, d)) rc = -1;
+    }
+  }
+  a.stats = stats_keep;
+  c->diss_part = diss_keep;
+  if (rc) return rc;
+  if (want && nsteps > 0) {
+    // strided copies of each statistic straight into the caller's arrays
+    const size_t sp = kStepStats * sizeof(double);
+    double* dst[3] = {ke, ens, diss};
+    for (int q = 0; q < 3; q++)
+      if (dst[q])
+        HIP_OK(hipMemcpy2DAsync(dst[q], sizeof(double), sb.out + q, sp, sizeof(double), (size_t)nsteps,
+                                hipMemcpyDeviceToHost, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+  }
+  return 0;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* astr_hip_last_error(void) { return g_err.c_str(); }
@@ -1088,45 +1159,20 @@ int astr_hip_rk3_steps(astr_hip_ctx* c, int nsteps, double* ke, double* ens) {
 int astr_hip_rk3_steps_stats(astr_hip_ctx* c, int nsteps, double* ke, double* ens, double* diss) {
   if (check_ctx(c)) return -1;
   if (nsteps < 0) return fail("negative step count");
-  static const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
   const bool want = ke || ens || diss;
-  if (want && c->stats_cap < nsteps) {
+  if (want && (c->stats_cap < nsteps || !c->stats_out)) {
+    // the capacity is recorded only once the buffer exists: a failed
+    // allocation leaves (nullptr, 0), and the next call allocates again
     if (c->stats_out) (void)hipFree(c->stats_out);
     c->stats_out = nullptr;
-    HIP_OK(hipMalloc(&c->stats_out, kStepStats * (size_t)nsteps * sizeof(double)));
+    c->stats_cap = 0;
+    double* p = nullptr;
+    if (test_fail_alloc()) return fail("hipMalloc of the step statistics failed (injected)");
+    HIP_OK(hipMalloc(&p, kStepStats * (size_t)std::max(nsteps, 1) * sizeof(double)));
+    c->stats_out = p;
     c->stats_cap = nsteps;
   }
-  FlowArgs& a = c->a;
-  const int stats_prog = c->split_last ? GL_GRAD : GC_LAST;
-  const int ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[stats_prog][a.ndir - 1]].TL);
-  for (int it = 0; it < nsteps; it++) {
-    for (int s = 0; s < 3; s++) {
-      a.stage = s;
-      a.rk_a = rk[s][0]; a.rk_b = rk[s][1]; a.rk_c = rk[s][2];
-      if (run_rhs(c, true, want && s == 0, false,
-                  diss && s == 0 ? c->stats_out + kStepStats * it + 2 : nullptr))
-        return -1;
-      if (want && s == 0) {
-        hipLaunchKernelGGL(k_stats_reduce, dim3(1), dim3(256), 0, c->stream, c->stats_part, ntl,
-                           c->stats_out + kStepStats * it, c->ke_scale);
-        HIP_OK(hipGetLastError());
-      }
-      const int nd = a.ndir;
-      for (int d = 0; d < nd - 1; d++)
-        if (pass(c, F_ONLY, d)) return -1;
-    }
-  }
-  if (want) {
-    std::vector<double> h(kStepStats * (size_t)nsteps);
-    HIP_OK(hipMemcpyAsync(h.data(), c->stats_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    for (int it = 0; it < nsteps; it++) {
-      if (ke) ke[it] = h[kStepStats * it];
-      if (ens) ens[it] = h[kStepStats * it + 1];
-      if (diss) diss[it] = h[kStepStats * it + 2];
-    }
-  }
-  return 0;
+  return rk3_run(c, nsteps, StatsBufs{c->stats_part, c->diss_part, c->stats_out}, ke, ens, diss);
 }
 
 int astr_hip_sync(astr_hip_ctx* c) {

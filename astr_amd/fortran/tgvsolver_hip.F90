@@ -30,6 +30,9 @@ program tgvsolver_hip
       call readinput(trim(path))
       call get_command_argument(3, path)
       call readcont(trim(path))
+      print *, ' ** note: this run uses the miniapp discretisation of'
+      print *, '    miniapps/tgv_solver_3d/tgvsolver.F90 (filter after the RK'
+      print *, '    update, Sutherland 110.4 K), not the stage order of astr run'
       call solver_init
       call allocommarray
       call flowfield_init

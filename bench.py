@@ -49,13 +49,36 @@ PASS_WORDS = {"gc_first": 20, "gc_mid": 25, "gc_last": 50, "visc_mid": 12,
 STAGE_BYTES = 1536
 
 
+def _host_cpus():
+    """The host's CPU counts: all CPUs, those in this process's affinity mask,
+    and the share OMP_NUM_THREADS states (None when unset)."""
+    share = os.environ.get("OMP_NUM_THREADS")
+    return {"cpu_count": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "cpu_share": int(share) if share and share.isdigit() else None}
+
+
+def _baseline_cores():
+    """Processes for the CPU baseline: every CPU of the affinity mask, at most
+    the stated CPU share (ASTR_BASELINE_CORES overrides both)."""
+    env = os.environ.get("ASTR_BASELINE_CORES")
+    if env:
+        return max(1, int(env))
+    h = _host_cpus()
+    n = h["affinity_cpus"]
+    if h["cpu_share"]:
+        n = min(n, h["cpu_share"])
+    return max(1, n)
+
+
 def cpu_baseline(seconds_target=12.0):
     """The reference's own Fortran (oracle/_ref) on a bounded 64^3 sample:
-    one serial run, then one run per host core at once (the MPI build of the
-    main path cannot be compiled here, so independent per-core runs of the
-    serial miniapp stand in for its ranks; with no halo exchange this bounds
-    an MPI run from above).  Falls back to the C restatement if the reference
-    build is absent."""
+    one serial run, then one run per usable host core at once (the MPI build
+    of the main path cannot be compiled here, so independent per-core runs of
+    the serial miniapp stand in for its ranks; with no halo exchange this
+    bounds an MPI run from above).  The process count is the size of this
+    process's CPU affinity mask, capped by the machine's CPU share for one
+    GPU when OMP_NUM_THREADS states it; both counts go into the line.  Falls
+    back to the C restatement if the reference build is absent."""
     from concurrent.futures import ThreadPoolExecutor
 
     from oracle import oracle as O
@@ -71,13 +94,13 @@ def cpu_baseline(seconds_target=12.0):
             r = O.run_ref("rk3", dims=[n, n, n], dt=5e-4, nsteps=steps)
             sec1 = float(r["seconds"])
             one = pts * 3 * steps / sec1 / 1e6
-            cores = max(1, min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16"))))
+            cores = _baseline_cores()
             with ThreadPoolExecutor(cores) as ex:
                 runs = list(ex.map(lambda _: O.run_ref("rk3", dims=[n, n, n], dt=5e-4, nsteps=steps),
                                    range(cores)))
             secs = [float(x["seconds"]) for x in runs]
             return {"value": cores * pts * 3 * steps / max(secs) / 1e6, "unit": "Mpt-updates/s",
-                    "cores": cores, "kind": "reference",
+                    "cores": cores, "kind": "reference", **_host_cpus(),
                     "single_core_value": one,
                     "sample": f"reference tgvsolver.F90 rk3 (flang -O3), TGV {n}^3, {steps} steps: "
                               f"{cores} serial processes at once, one per core, slowest "
@@ -94,9 +117,10 @@ def cpu_baseline(seconds_target=12.0):
     t0 = time.perf_counter()
     s.rk3(steps)
     sec = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    cores = _baseline_cores()
     return {"value": pts * 3 * steps / sec / 1e6, "unit": "Mpt-updates/s", "cores": cores,
-            "kind": "port", "sample": f"C restatement, TGV {n}^3, {steps} steps, {sec:.1f} s"}
+            "kind": "port", **_host_cpus(),
+            "sample": f"C restatement (OpenMP), TGV {n}^3, {steps} steps, {sec:.1f} s"}
 
 
 def chem_batch(ncells=1 << 20, reps=20):
