@@ -282,3 +282,85 @@ def test_gpu_reactor_same_alone_and_in_a_mix(mech, lanes):
     mixed.run(max_tries=37)
     a, m = alone.get(0), mixed.get(37)
     assert a[0] == m[0] and a[1] == m[1] and np.array_equal(a[2], m[2]) and a[3:] == m[3:]
+
+
+# Jachimowski (1988) H2-air mechanism, 19 reversible reactions, in the units
+# the mechanism is published in: A in cm^3, mol, s (cm^6/mol^2/s for the
+# termolecular three-body reactions), E in cal/mol; k_f = A T^b exp(-E/RT).
+# Transcribed by hand, reaction by reaction, from the equations and rate
+# parameters of chemMech/Jachimowski88_H2-9-19.xml (that file stores A in
+# m^3/kmol/s; the conversion here is the test's own).  The publication itself
+# (NASA TP-2791) is not available offline, so the table checks how the parser
+# reads the mechanism file, not the file against the paper.
+# (reactants, products, A, b, E, third-body efficiencies or None)
+JACHIMOWSKI_1988 = [
+    ("H2 O2", "HO2 H", 1.0e14, 0.0, 56000.0, None),
+    ("H O2", "OH O", 2.6e14, 0.0, 16800.0, None),
+    ("O H2", "OH H", 1.8e10, 1.0, 8900.0, None),
+    ("OH H2", "H2O H", 2.2e13, 0.0, 5150.0, None),
+    ("OH OH", "H2O O", 6.3e12, 0.0, 1090.0, None),
+    ("H OH", "H2O", 2.2e22, -2.0, 0.0, {"H2O": 6.0}),
+    ("H H", "H2", 6.4e17, -1.0, 0.0, {"H2": 2.0, "H2O": 6.0}),
+    ("H O", "OH", 6.0e16, -0.6, 0.0, {"H2O": 6.0}),
+    ("H O2", "HO2", 2.1e15, 0.0, -1000.0, {"H2": 2.0, "H2O": 6.0}),
+    ("HO2 H", "OH OH", 1.4e14, 0.0, 1080.0, None),
+    ("HO2 H", "H2O O", 1.0e13, 0.0, 1080.0, None),
+    ("HO2 O", "O2 OH", 1.5e13, 0.0, 950.0, None),
+    ("HO2 OH", "H2O O2", 8.0e12, 0.0, 0.0, None),
+    ("HO2 HO2", "H2O2 O2", 2.0e12, 0.0, 0.0, None),
+    ("H H2O2", "H2 HO2", 1.4e12, 0.0, 3600.0, None),
+    ("O H2O2", "OH HO2", 1.4e13, 0.0, 6400.0, None),
+    ("OH H2O2", "H2O HO2", 6.1e12, 0.0, 1430.0, None),
+    ("H2O2", "OH OH", 1.2e17, 0.0, 45500.0, {"H2O": 15.0}),
+    ("O O", "O2", 6.0e13, 0.0, -1800.0, {}),
+]
+
+
+def _side(mech, text):
+    out = {}
+    for s in text.split():
+        k = mech.names.index(s)
+        out[k] = out.get(k, 0.0) + 1.0
+    return out
+
+
+def test_mechanism_rate_table(mech):
+    """Every reaction's stoichiometry, reversibility, rate constant and
+    third-body efficiencies as the parser reads them, against the table above:
+    k_f(T) at 900, 1500 and 2500 K within 1e-12 relative (cm^3/mol = 1e-3
+    m^3/kmol per concentration factor beyond the first, [M] counted as one;
+    E in cal/mol with R = 8.31446261815324/4.184 cal/mol/K), efficiencies
+    exactly (default 1).  A planted slip in any one (A, b, E) triple, order
+    or efficiency breaks it."""
+    assert mech.nr == len(JACHIMOWSKI_1988)
+    rcal = 8.31446261815324 / 4.184
+    for j, (reac, prod, A, b, E, eff) in enumerate(JACHIMOWSKI_1988):
+        r = mech.rx[j]
+        assert r["rev"], j
+        assert dict(r["reac"]) == _side(mech, reac), j
+        assert dict(r["prod"]) == _side(mech, prod), j
+        assert r["tb"] == (eff is not None), j
+        order = sum(_side(mech, reac).values()) + (1 if eff is not None else 0)
+        for T in (900.0, 1500.0, 2500.0):
+            k_tab = A * 1.0e-3 ** (order - 1) * T ** b * np.exp(-E / (rcal * T))
+            k = r["A"] * T ** r["b"] * np.exp(-r["EaR"] / T)
+            assert abs(k / k_tab - 1.0) < 1e-12, (j, T, k, k_tab)
+        want = np.ones(mech.ns)
+        for s, v in (eff or {}).items():
+            want[mech.names.index(s)] = v
+        assert np.array_equal(r["eff"], want), j
+
+
+def test_mechanism_rate_table_catches_a_planted_slip(mech):
+    """The table check is sensitive: one changed efficiency or activation
+    energy in a copy of the parsed mechanism fails it."""
+    import copy
+
+    for field, j, bump in (("eff", 8, None), ("EaR", 15, 1.0 + 1e-9), ("A", 17, 1.001)):
+        m2 = copy.deepcopy(mech)
+        if field == "eff":
+            m2.rx[j]["eff"][m2.names.index("H2O")] = 16.0
+        else:
+            m2.rx[j][field] *= bump
+        with pytest.raises(AssertionError):
+            test_mechanism_rate_table(m2)

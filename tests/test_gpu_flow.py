@@ -322,3 +322,23 @@ def test_tgv256_long_run_properties():
     assert np.all(np.isfinite(q))
     m0 = q0[0, :-1, :-1, :-1].sum()
     assert abs(q[0, :-1, :-1, :-1].sum() / m0 - 1) < 1e-6
+
+
+def test_stats_allocation_failure_recovers():
+    """A failed allocation of the step-statistics buffer (forced by the
+    library's ASTR_HIP_TEST_FAIL_ALLOC hook) reports an error and leaves no
+    stale capacity: the next call allocates again and its statistics equal
+    a fresh context's."""
+    f, s = make_pair(20, 18, 16)
+    ref = A.Flow(A.tgv_config(20, 18, 16, deltat=1e-3))
+    ref.upload(s.q.copy())
+    ke0, en0 = ref.rk3(2)
+    os.environ["ASTR_HIP_TEST_FAIL_ALLOC"] = "1"
+    try:
+        with pytest.raises(A.AstrError, match="injected"):
+            f.rk3(4)
+    finally:
+        del os.environ["ASTR_HIP_TEST_FAIL_ALLOC"]
+    f.upload(s.q.copy())
+    ke, en = f.rk3(2)
+    assert np.array_equal(ke, ke0) and np.array_equal(en, en0)

@@ -298,3 +298,106 @@ def test_qswap_restatement_properties():
         rows = (slice(None), slice(1, -1) if shape[1] > 1 else slice(None), slice(1, -1))
         assert np.array_equal(2.0 * s[rows + (0,)], q[rows + (0,)] + q[rows + (-1,)])
         assert np.array_equal(O.qswap(s), s)
+
+
+def _z_uniform_pair(im=24, jm=20, km=12, seed=5):
+    """A 2-D state of the oracle (noisy TGV, km = 0) and the 3-D state that
+    repeats it on every k-plane of an im x jm x km box (w = 0)."""
+    s2 = O.OracleState(O.tgv_cfg(im, jm, 0, dt=2e-3))
+    s2.tgv_init(1e-2, seed)
+    q2 = s2.q.copy()
+    s2.set_q(q2)
+    s3 = O.OracleState(O.tgv_cfg(im, jm, km, dt=2e-3))
+    s3.set_q(np.repeat(q2, km + 1, axis=1))
+    return s2, s3
+
+
+def test_oracle_2d_equals_z_uniform_3d():
+    """The 2-D path (ndir = 2: no z sweeps, z gradients zero) against the
+    reference-pinned 3-D path on a field constant along k with w = 0, where
+    every z derivative is exactly zero (the 643c right-hand side of a constant
+    line vanishes): rhscal's rho, rho u, rho v and E right-hand sides equal
+    the 2-D ones bit for bit on every k-plane and the rho w one is zero.  The
+    z filter keeps a constant line only to rounding (its beta_h = 1.11 rows
+    and coefficient sums), about 20 ulp, so one filterq agrees within 5e-14
+    and three RK3 steps and their statistics within 1e-13."""
+    s2, s3 = _z_uniform_pair()
+    s2.rhscal()
+    s3.rhscal()
+    for m in (0, 1, 2, 4):
+        for k in range(s3.cfg.km + 1):
+            assert np.array_equal(s3.qrhs[m, k], s2.qrhs[m, 0]), (m, k)
+    assert np.all(s3.qrhs[3] == 0.0)
+    assert np.all(s3.dvel[6:] == 0.0) and np.all(s3.dtmp[2] == 0.0)
+    assert np.array_equal(s3.dtmp[:2, 5], s2.dtmp[:2, 0])
+    s2, s3 = _z_uniform_pair()
+    s2.filterq()
+    s3.filterq()
+    for m in (0, 1, 2, 4):
+        assert relmax(s3.q[m], np.repeat(s2.q[m], s3.cfg.km + 1, axis=0)) < 5e-14, m
+    s2, s3 = _z_uniform_pair()
+    ke2, en2, di2 = s2.rk3(3, dissipation=True)
+    ke3, en3, di3 = s3.rk3(3, dissipation=True)
+    assert relmax(ke3, ke2) < 1e-13 and relmax(en3, en2) < 1e-13 and relmax(di3, di2) < 1e-13
+    for m in (0, 1, 2, 4):
+        assert relmax(s3.q[m], np.repeat(s2.q[m], s3.cfg.km + 1, axis=0)) < 1e-13, m
+    assert np.abs(s3.q[3]).max() < 1e-13 * np.abs(s3.q[1]).max()
+
+
+def _diss2d_for(n, field, amp=0.1):
+    """diss_rate_cal of the 2-D oracle (km = 0) on an n^2 periodic lattice,
+    rho = 1, T = 1 (mu = 1/Re), velocity amp * field(x, y); returns
+    dissipation * Re."""
+    cfg = O.tgv_cfg(n, n, 0)
+    s = O.OracleState(cfg)
+    x = np.arange(n + 1) * cfg.dx
+    Y, X = np.meshgrid(x, x, indexing="ij")
+    u, v = (amp * c for c in field(X, Y))
+    g, ma = cfg.gamma, cfg.mach
+    p0 = 1.0 / (g * ma * ma)
+    q = np.stack([np.ones_like(X), u, v, 0.0 * X, p0 / (g - 1.0) + 0.5 * (u * u + v * v)])[:, None]
+    s.set_q(q)
+    s.rhscal()
+    return s.diss_rate() * cfg.reynolds
+
+
+def test_dissipation_closed_forms_2d():
+    """diss_rate_cal on the 2-D path: shear u = A sin y gives A^2/2 mu (the
+    factor 2 and the 1/2 of the off-diagonal rate), compression u = A sin x
+    gives 2A^2/3 mu (the -div^2/3 term), each converging at 6th order."""
+    A2 = 0.1 ** 2
+    shear = lambda X, Y: (np.sin(Y), 0.0 * X)
+    comp = lambda X, Y: (np.sin(X), 0.0 * X)
+    for field, exact in ((shear, 0.5 * A2), (comp, 2.0 / 3.0 * A2)):
+        e16 = abs(_diss2d_for(16, field) / exact - 1.0)
+        e32 = abs(_diss2d_for(32, field) / exact - 1.0)
+        assert e32 < 1e-7, (exact, e16, e32)
+        assert e32 < e16 / 40.0 or e32 < 1e-13, (exact, e16, e32)
+
+
+def test_qswap_corner_order():
+    """qswap averages along i, then j, then k (parallel.F90:4882-4883,
+    5027-5028, 5168-5169), so the value it leaves at the eight corners is
+    ((c000+cn00)/2 + (c0n0+cnn0)/2)/2 ... with the i pairs innermost.  On this
+    field the k-innermost order rounds differently; the restatement must give
+    the i-innermost value bit for bit."""
+    rng = np.random.default_rng(11)
+    n = 6
+    for trial in range(200):
+        q = rng.standard_normal((5, n + 1, n + 1, n + 1)) * np.exp(rng.uniform(-20, 20, (5, 1, 1, 1)))
+        c = q[:, [0, -1]][:, :, [0, -1]][:, :, :, [0, -1]]   # c[m, k, j, i]
+        avg = lambda a, b: 0.5 * (a + b)
+        ijk = avg(avg(avg(c[:, 0, 0, 0], c[:, 0, 0, 1]), avg(c[:, 0, 1, 0], c[:, 0, 1, 1])),
+                  avg(avg(c[:, 1, 0, 0], c[:, 1, 0, 1]), avg(c[:, 1, 1, 0], c[:, 1, 1, 1])))
+        kji = avg(avg(avg(c[:, 0, 0, 0], c[:, 1, 0, 0]), avg(c[:, 0, 1, 0], c[:, 1, 1, 0])),
+                  avg(avg(c[:, 0, 0, 1], c[:, 1, 0, 1]), avg(c[:, 0, 1, 1], c[:, 1, 1, 1])))
+        if not np.array_equal(ijk, kji):
+            break
+    else:
+        pytest.fail("no field with order-dependent corners found")
+    s = O.qswap(q)
+    for kk in (0, -1):
+        for jj in (0, -1):
+            for ii in (0, -1):
+                assert np.array_equal(s[:, kk, jj, ii], ijk)
+    assert not np.array_equal(s[:, 0, 0, 0], kji)
