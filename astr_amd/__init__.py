@@ -54,7 +54,7 @@ EXPORTS = [
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
     "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt", "astr_hip_updatefvar",
     "astr_hip_set_metrics", "astr_hip_qswap",
-    "astr_hip_get_last_split",
+    "astr_hip_get_last_split", "astr_hip_rk3_steps_stats_noalloc",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
           5: (8, 64, 5), 6: (4, 32, 9), 7: (2, 64, 5),
@@ -87,6 +87,7 @@ def lib():
         L.astr_hip_rhscal.argtypes = [C.c_void_p, _dp]
         L.astr_hip_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp]
         L.astr_hip_rk3_steps_stats.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.astr_hip_rk3_steps_stats_noalloc.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         L.astr_hip_gridgeom.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.astr_hip_gridgeom_nt.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, _dp, _dp]
         L.astr_hip_qswap.argtypes = [C.c_void_p]
@@ -217,6 +218,14 @@ class Flow:
         return (ke, en, di) if dissipation else (ke, en)
 
     steploop = rk3
+
+    def rk3_noalloc(self, nsteps):
+        """rk3(nsteps, dissipation=True) through the entry point that allocates
+        no device memory (its statistics buffers are the library's fixed
+        per-GPU array; calls overlapping on one GPU race)."""
+        ke, en, di = np.empty(nsteps), np.empty(nsteps), np.empty(nsteps)
+        _chk(lib().astr_hip_rk3_steps_stats_noalloc(self.h, nsteps, _p(ke), _p(en), _p(di)))
+        return ke, en, di
 
     def qswap(self):
         """The main path's single-rank qswap: duplicate-plane averages along

@@ -466,6 +466,10 @@ struct astr_hip_ctx {
   std::vector<int> ev_kind;
   int ev_used = 0;
   bool prof = false;
+  // device address of this device's copy of `scratch` (nullptr when the
+  // lookup failed; scratch_err then holds the reason)
+  double* scratch = nullptr;
+  std::string scratch_err;
 };
 
 namespace {
@@ -852,7 +856,38 @@ hipError_t deriv_lines_device(int n, long long nlines, const double* f, double* 
 
 }  // namespace
 
+// Statistics buffers of astr_hip_rk3_steps_stats_noalloc, for callers that
+// cannot allocate device memory: the per-tile partial sums, the dissipation
+// rate's block partials and the per-step results, packed in this order.
+// Each device holds its own copy (8 MiB) until the process ends.  Host code
+// never passes the array's name to a kernel: the name is a host-side stand-in,
+// and kernels take the device address that hipGetSymbolAddress returns.
+constexpr long long kScratchDoubles = 1ll << 20;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
 namespace {
+
+// Device address of `scratch` on the current device, looked up once per
+// device; nullptr with the HIP error in *err when the lookup fails.
+double* scratch_address(int device, std::string* err) {
+  static std::mutex mu;
+  static double* addr[64] = {nullptr};
+  if (device < 0 || device >= 64) {
+    *err = "device index out of range for the scratch table";
+    return nullptr;
+  }
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return nullptr;
+    }
+    addr[device] = static_cast<double*>(p);
+  }
+  return addr[device];
+}
 
 // Device buffers of the per-step statistics: the per-tile partial sums the
 // last gradient pass leaves (2 per tile), the dissipation rate's block
@@ -1015,6 +1050,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   a.stats = c->stats_part;
   c->diss_part = c->stats_part + kTileStats * (size_t)ntl;
   c->stream = nullptr;
+  // the no-allocation entry point's buffers: this device's copy of scratch
+  c->scratch = scratch_address(c->device, &c->scratch_err);
   *out = c;
   return 0;
 }
@@ -1173,6 +1210,23 @@ int astr_hip_rk3_steps_stats(astr_hip_ctx* c, int nsteps, double* ke, double* en
     c->stats_cap = nsteps;
   }
   return rk3_run(c, nsteps, StatsBufs{c->stats_part, c->diss_part, c->stats_out}, ke, ens, diss);
+}
+
+int astr_hip_rk3_steps_stats_noalloc(astr_hip_ctx* c, int nsteps, double* ke, double* ens, double* diss) {
+  if (check_ctx(c)) return -1;
+  if (nsteps < 0) return fail("negative step count");
+  if (!c->scratch) return fail("no scratch buffer on this device: " + c->scratch_err);
+  const FlowArgs& a = c->a;
+  const int stats_prog = c->split_last ? GL_GRAD : GC_LAST;
+  const long long ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[stats_prog][a.ndir - 1]].TL);
+  const long long need = kTileStats * ntl + kDissBlocks + kStepStats * (long long)std::max(nsteps, 1);
+  if (need > kScratchDoubles)
+    return fail("astr_hip_rk3_steps_stats_noalloc: " + std::to_string(need) +
+                " doubles of statistics exceed the scratch buffer of " + std::to_string(kScratchDoubles) +
+                " (fewer steps per call, or astr_hip_rk3_steps_stats)");
+  double* s = c->scratch;
+  return rk3_run(c, nsteps, StatsBufs{s, s + kTileStats * ntl, s + kTileStats * ntl + kDissBlocks}, ke, ens,
+                 diss);
 }
 
 int astr_hip_sync(astr_hip_ctx* c) {

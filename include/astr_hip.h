@@ -109,6 +109,19 @@ int astr_hip_rk3_steps(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens);
  * sum 2 mu (s:s - div^2/3) / (im*jm*km) over i,j,k = 1..im,jm,km
  * (diss_rate_cal, statistic.F90:994-1043).  Any of the three may be NULL. */
 int astr_hip_rk3_steps_stats(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens, double *diss);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc): astr_hip_rk3_steps_stats with the same results, whose
+ * statistics buffers (per-tile partial sums, dissipation block partials, the
+ * per-step values) live in a fixed 8 MiB device array, one copy per GPU, held
+ * until the process ends.  It calls no hipMalloc/hipFree.  Limit:
+ * 2 x (tiles of the last gradient pass) + 1024 + 3 x nsteps doubles must fit
+ * in 2^20 (e.g. nsteps <= 170000 at 512^3); larger requests return an error
+ * before any work is queued.  Every call on one GPU shares that GPU's array,
+ * so calls that overlap on one GPU (other threads, other contexts on other
+ * streams) race; serialise them. */
+int astr_hip_rk3_steps_stats_noalloc(astr_hip_ctx *ctx, int nsteps, double *ke, double *ens,
+                                     double *diss);
 int astr_hip_sync(astr_hip_ctx *ctx);
 
 /* qswap, single rank, periodic (parallel.F90:4876-4884, 5018-5030,
