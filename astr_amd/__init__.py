@@ -17,6 +17,7 @@ import numpy as np
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("ASTR_HIP_LIB", os.path.join(PKG, "libastr_hip.so"))
 SEM_MINIAPP = 0
+SEM_MAIN = 1
 
 _dp = C.POINTER(C.c_double)
 
@@ -54,7 +55,8 @@ EXPORTS = [
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
     "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt", "astr_hip_updatefvar",
     "astr_hip_set_metrics", "astr_hip_qswap",
-    "astr_hip_get_last_split", "astr_hip_rk3_steps_stats_noalloc",
+    "astr_hip_get_last_split", "astr_hip_rk3_steps_stats_noalloc", "astr_hip_set_fvar",
+    "astr_hip_write_checkpoint", "astr_hip_read_checkpoint",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
           5: (8, 64, 5), 6: (4, 32, 9), 7: (2, 64, 5),
@@ -92,6 +94,10 @@ def lib():
         L.astr_hip_gridgeom_nt.argtypes = [C.c_void_p, C.POINTER(C.c_int), _dp, _dp, _dp]
         L.astr_hip_qswap.argtypes = [C.c_void_p]
         L.astr_hip_updatefvar.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]
+        L.astr_hip_set_fvar.argtypes = [C.c_void_p, _dp, _dp, _dp, _dp, C.c_int]
+        L.astr_hip_write_checkpoint.argtypes = [C.c_void_p, C.c_char_p, C.c_longlong, C.c_double]
+        L.astr_hip_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_longlong),
+                                               C.POINTER(C.c_double)]
         L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
@@ -129,13 +135,23 @@ def _p(a):
 
 
 def tgv_config(im, jm=None, km=None, deltat=1e-3, reynolds=1600.0, mach=0.1,
-               sutherland=110.4, ref_tem=273.15, lx=2 * math.pi, ly=None, lz=None):
-    """Configuration of the reference's TGV miniapp (tgvsolver.F90:152-261)."""
+               sutherland=110.4, ref_tem=273.15, lx=2 * math.pi, ly=None, lz=None,
+               semantics=SEM_MINIAPP):
+    """Configuration of the reference's TGV miniapp (tgvsolver.F90:152-261),
+    or with semantics=SEM_MAIN the main solver's stage on the same case."""
     jm = im if jm is None else jm
     km = im if km is None else km
-    return Config(im, jm, km, SEM_MINIAPP, 0.49, 1.11, 1.09, 1.4, mach, reynolds, 0.72,
+    return Config(im, jm, km, semantics, 0.49, 1.11, 1.09, 1.4, mach, reynolds, 0.72,
                   ref_tem, sutherland, lx, lx if ly is None else ly,
                   lx if lz is None else lz, deltat)
+
+
+def main_config(im, jm=None, km=None, deltat=1e-3, reynolds=1600.0, mach=0.1, ref_tem=273.15,
+                lx=2 * math.pi, ly=None, lz=None):
+    """The main solver's stage (time_integration_rk, src/mainloop.F90:396-482)
+    on a periodic box: Sutherland S = 110.3 K (src/solver.F90:122)."""
+    return tgv_config(im, jm, km, deltat=deltat, reynolds=reynolds, mach=mach, sutherland=110.3,
+                      ref_tem=ref_tem, lx=lx, ly=ly, lz=lz, semantics=SEM_MAIN)
 
 
 class Flow:
@@ -241,6 +257,22 @@ class Flow:
         vel = np.empty((3,) + shp)
         _chk(lib().astr_hip_updatefvar(self.h, _p(rho), _p(vel), _p(prs), _p(tmp), hm))
         return rho, vel, prs, tmp
+
+    def set_fvar(self, rho, vel, prs, tmp, hm=0):
+        """Main path: set the stored primitives (numpy (k, j, i) arrays with
+        halos of depth hm; vel with a leading component axis of 3)."""
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (rho, vel, prs, tmp)]
+        _chk(lib().astr_hip_set_fvar(self.h, *[_p(x) for x in arr], hm))
+
+    def writechkpt(self, path, nstep=0, time=0.0):
+        """Checkpoint of the device state (writechkpt, readwrite.F90:2390)."""
+        _chk(lib().astr_hip_write_checkpoint(self.h, os.fsencode(path), int(nstep), float(time)))
+
+    def readcheckpoint(self, path):
+        """Resume from a checkpoint; returns (nstep, time) of its header."""
+        n, t = C.c_longlong(), C.c_double()
+        _chk(lib().astr_hip_read_checkpoint(self.h, os.fsencode(path), C.byref(n), C.byref(t)))
+        return n.value, t.value
 
     # -- grid metrics (gridgeom, geom.F90:99-692) ------------------------------
     def gridgeom(self, x, npdc=None):

@@ -1,7 +1,9 @@
 """Build, in-tree: the HIP library astr_amd/libastr_hip.so for gfx950
 (hipcc), and the Fortran host (flang): the ISO_C_BINDING modules of
-astr_amd/fortran/astr_hip_mod.F90 and the TGV solver program
-astr_amd/fortran/tgvsolver_hip linked against the library."""
+astr_amd/fortran/astr_hip_mod.F90, the miniapp's TGV solver program
+astr_amd/fortran/tgvsolver_hip, the chemistry cell program and the main
+solver's run program astr_amd/fortran/astr_hip (astr_hip_run.F90), linked
+against the library."""
 from __future__ import annotations
 
 import os
@@ -19,6 +21,8 @@ FLANG = os.environ.get("FLANG", "/opt/rocm/lib/llvm/bin/flang")
 FDIR = os.path.join(PKG, "fortran")
 FBUILD = os.path.join(FDIR, "_build")
 FPROGS = ["tgvsolver_hip", "chemcells_hip"]
+# astr_hip_run.F90 holds the main solver's module and its program astr_hip
+FRUN = "astr_hip_run"
 
 FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
@@ -74,12 +78,15 @@ def build_fortran() -> str:
         steps.append([FLANG, "-O2", "-cpp", "-module-dir", FBUILD, "-o",
                       os.path.join(FDIR, prog + ".tmp"), os.path.join(FDIR, prog + ".F90"), mod_o,
                       "-L" + PKG, "-lastr_hip", "-Wl,-rpath,$ORIGIN/.."])
+    steps.append([FLANG, "-O2", "-cpp", "-module-dir", FBUILD, "-o",
+                  os.path.join(FDIR, "astr_hip.tmp"), os.path.join(FDIR, FRUN + ".F90"), mod_o,
+                  "-L" + PKG, "-lastr_hip", "-Wl,-rpath,$ORIGIN/.."])
     for cmd in steps:
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             sys.stderr.write(r.stdout + r.stderr)
             raise RuntimeError("flang failed: " + " ".join(cmd))
-    for prog in FPROGS:
+    for prog in FPROGS + ["astr_hip"]:
         os.replace(os.path.join(FDIR, prog + ".tmp"), os.path.join(FDIR, prog))
     return FDIR
 

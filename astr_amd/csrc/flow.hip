@@ -152,13 +152,16 @@ __global__ __launch_bounds__(256) void k_diss_part(FlowArgs a, double* __restric
     const int j = 1 + (int)(r % jm);
     const int k = km > 0 ? 1 + (int)(r / jm) : 0;
     const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
-    const Prim P = prim_of(a, a.q[o], a.q[fs + o], a.q[2 * fs + o], a.q[3 * fs + o], a.q[4 * fs + o]);
+    // the main path's diss_rate_cal reads the stored tmp (statistic.F90:1015)
+    const double t = a.main_sem ? a.pr[5 * fs + o]
+                                : prim_of(a, a.q[o], a.q[fs + o], a.q[2 * fs + o], a.q[3 * fs + o],
+                                          a.q[4 * fs + o]).t;
     double g[3][3];
 #pragma unroll
     for (int m = 0; m < 3; m++)
 #pragma unroll
       for (int d = 0; d < 3; d++) g[m][d] = d < a.ndir ? a.dv[(m + 3 * d) * fs + o] : 0.0;
-    acc += diss_point(a, P.t, g);
+    acc += diss_point(a, t, g);
   }
   const double b = block_sum256(acc);
   if (threadIdx.x == 0) part[blockIdx.x] = b;
@@ -470,6 +473,11 @@ struct astr_hip_ctx {
   // lookup failed; scratch_err then holds the reason)
   double* scratch = nullptr;
   std::string scratch_err;
+  // main path: the halo sources qswap keeps (hx then hy, one allocation),
+  // and whether they belong to the current q (cleared by every change of q
+  // outside qswap)
+  double* halo = nullptr;
+  bool halo_valid = false;
 };
 
 namespace {
@@ -722,6 +730,78 @@ __global__ void k_qswap_dir(FlowArgs a, int dir) {
   q[o0] = v;
   q[on] = v;
 }
+
+// Main path: the halo sources of the lines whose halo qswap fills before a
+// later step averages the source points (dir 0: the x-lines on the j and k
+// boundary planes, sources i = 1..5, im-5..im-1; dir 1: the y-lines on the k
+// boundary planes, sources j = 1..5, jm-5..jm-1), in halo_src_index's order.
+__global__ void k_halo_src(FlowArgs a, int dir, double* __restrict__ dst, long long dst_stride) {
+  const long long nl = dir == 0 ? hx_lines(a.n, a.ndir) : hy_lines(a.n, a.ndir);
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nl * kHaloSrcRows) return;
+  const int line = (int)(idx / kHaloSrcRows), hrow = (int)(idx % kHaloSrcRows);
+  const int n = a.n[dir];
+  const int r = hrow < 5 ? hrow + 1 : n - 5 + (hrow - 5);
+  int i, j, k;
+  if (dir == 0) {
+    i = r;
+    const int nA = 2 * (a.n[2] + 1);
+    if (line < nA) {
+      j = line / (a.n[2] + 1) == 0 ? 0 : a.n[1];
+      k = line % (a.n[2] + 1);
+    } else {
+      const int l2 = line - nA;
+      k = l2 / (a.n[1] + 1) == 0 ? 0 : a.n[2];
+      j = l2 % (a.n[1] + 1);
+    }
+  } else {
+    j = r;
+    k = line / (a.n[0] + 1) == 0 ? 0 : a.n[2];
+    i = line % (a.n[0] + 1);
+  }
+  const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+#pragma unroll
+  for (int m = 0; m < 5; m++) dst[m * dst_stride + idx] = a.q[m * a.fs + o];
+}
+
+// qswap's q2fvar of the boundary slabs (parallel.F90:4893-4899, 5048-5054,
+// 5189-5195) for the computed points: rho, u, v, w, p, T of the points on the
+// planes dir = 0 and dir = n[dir] from the averaged q
+__global__ void k_bprims(FlowArgs a, int dir) {
+  const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+  const long long n1 = a.n[d1] + 1, n2 = (long long)a.n[d2] + 1;
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= 2 * n1 * n2) return;
+  const int side = (int)(idx / (n1 * n2));
+  const long long r = idx - side * n1 * n2;
+  const long long st[3] = {1, a.pitch, a.plane};
+  const long long o = (r % n1) * st[d1] + (r / n1) * st[d2] + (side ? (long long)a.n[dir] * st[dir] : 0);
+  const long long fs = a.fs;
+  const Prim P = prim_of(a, a.q[o], a.q[fs + o], a.q[2 * fs + o], a.q[3 * fs + o], a.q[4 * fs + o]);
+  a.pr[o] = P.r;
+  a.pr[fs + o] = P.u;
+  a.pr[2 * fs + o] = P.v;
+  a.pr[3 * fs + o] = P.w;
+  a.pr[4 * fs + o] = P.p;
+  a.pr[5 * fs + o] = P.t;
+}
+
+// updatefvar (fludyna.F90:191-242): rho, u, v, w, p, T of q at every computed
+// point into the stored primitives
+__global__ void k_updatefvar(FlowArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y, k = blockIdx.z;
+  if (i > a.n[0]) return;
+  const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+  const long long fs = a.fs;
+  const Prim P = prim_of(a, a.q[o], a.q[fs + o], a.q[2 * fs + o], a.q[3 * fs + o], a.q[4 * fs + o]);
+  a.pr[o] = P.r;
+  a.pr[fs + o] = P.u;
+  a.pr[2 * fs + o] = P.v;
+  a.pr[3 * fs + o] = P.w;
+  a.pr[4 * fs + o] = P.p;
+  a.pr[5 * fs + o] = P.t;
+}
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -898,9 +978,67 @@ struct StatsBufs {
   double* out;
 };
 
+// updatefvar into the stored primitives (main path)
+int updatefvar_dev(astr_hip_ctx* c) {
+  hipLaunchKernelGGL(k_updatefvar, pt_grid(c->a), dim3(64), 0, c->stream, c->a);
+  HIP_OK(hipGetLastError());
+  c->halo_valid = false;
+  return 0;
+}
+
+// Main path qswap, single rank, periodic (parallel.F90:4876-4920,
+// 5018-5060, 5159-5200), for the device layout: the halo copies are the
+// passes' index wrap, except for the lines whose halo the reference copies
+// before a later step averages its source points, whose sources are kept
+// here first; then the duplicate-plane averages along i, j, k and the
+// primitives of the boundary planes.  average = false keeps the halo
+// sources of the current q only (the passes then see the halo the
+// reference's dataswap of q gives).
+int qswap_main(astr_hip_ctx* c, bool average) {
+  const FlowArgs& a = c->a;
+  auto avg = [&](int d) -> int {
+    const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+    const long long cnt = 5LL * (a.n[d1] + 1) * (a.n[d2] + 1);
+    hipLaunchKernelGGL(k_qswap_dir, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, a, d);
+    HIP_OK(hipGetLastError());
+    return 0;
+  };
+  const long long nx = hx_lines(a.n, a.ndir) * kHaloSrcRows;
+  hipLaunchKernelGGL(k_halo_src, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, c->stream, a, 0,
+                     const_cast<double*>(a.hx), a.hxs);
+  HIP_OK(hipGetLastError());
+  if (average && avg(0)) return -1;
+  if (a.ndir == 3) {
+    const long long ny = hy_lines(a.n, a.ndir) * kHaloSrcRows;
+    hipLaunchKernelGGL(k_halo_src, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, c->stream, a, 1,
+                       const_cast<double*>(a.hy), a.hys);
+    HIP_OK(hipGetLastError());
+  }
+  if (average) {
+    for (int d = 1; d < a.ndir; d++)
+      if (avg(d)) return -1;
+    for (int d = 0; d < a.ndir; d++) {
+      const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
+      const long long cnt = 2LL * (a.n[d1] + 1) * (a.n[d2] + 1);
+      hipLaunchKernelGGL(k_bprims, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, a, d);
+      HIP_OK(hipGetLastError());
+    }
+  }
+  c->halo_valid = true;
+  return 0;
+}
+
+// the main path's right-hand side passes read the kept halo sources
+int ensure_halo(astr_hip_ctx* c) {
+  if (c->a.main_sem && !c->halo_valid) return qswap_main(c, false);
+  return 0;
+}
+
 // steploop/rk3 (tgvsolver.F90:944-960, :994-1082) with the statistics of
 // each step's first stage in the buffers sb; every statistics value is
-// written before it is read, so the buffers need no clearing
+// written before it is read, so the buffers need no clearing.  The main
+// path's stage (time_integration_rk, src/mainloop.F90:396-482) is filterq,
+// qswap, gradcal, [statcal], rhscal, the RK3 update and updatefvar.
 int rk3_run(astr_hip_ctx* c, int nsteps, const StatsBufs& sb, double* ke, double* ens, double* diss) {
   static const double rk[3][3] = {{1.0, 0.0, 1.0}, {0.75, 0.25, 0.25}, {1.0 / 3.0, 2.0 / 3.0, 2.0 / 3.0}};
   const bool want = ke || ens || diss;
@@ -916,6 +1054,12 @@ int rk3_run(astr_hip_ctx* c, int nsteps, const StatsBufs& sb, double* ke, double
     for (int s = 0; s < 3 && !rc; s++) {
       a.stage = s;
       a.rk_a = rk[s][0]; a.rk_b = rk[s][1]; a.rk_c = rk[s][2];
+      if (a.main_sem) {
+        for (int d = 0; d < a.ndir && !rc; d++)
+          if (pass(c, F_ONLY, d)) rc = -1;
+        if (!rc && qswap_main(c, true)) rc = -1;
+        if (rc) break;
+      }
       if (run_rhs(c, true, want && s == 0, false,
                   diss && s == 0 ? sb.out + kStepStats * it + 2 : nullptr)) {
         rc = -1;
@@ -928,10 +1072,12 @@ int rk3_run(astr_hip_ctx* c, int nsteps, const StatsBufs& sb, double* ke, double
         if (e != hipSuccess) rc = fail(std::string("k_stats_reduce: ") + hipGetErrorString(e));
       }
       const int nd = a.ndir;
-      for (int d = 0; d < nd - 1 && !rc; d++)
+      for (int d = 0; d < nd - 1 && !rc && !a.main_sem; d++)
         if (pass(c, F_ONLY, d)) rc = -1;
     }
   }
+  // the RK passes changed q after the last qswap
+  c->halo_valid = false;
   a.stats = stats_keep;
   c->diss_part = diss_keep;
   if (rc) return rc;
@@ -959,7 +1105,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   *out = nullptr;
   if (cfg->im < 11 || cfg->jm < 11 || cfg->km < 0 || (cfg->km > 0 && cfg->km < 11))
     return fail("grid sizes must be >= 11 (km == 0 for two-dimensional cases)");
-  if (cfg->semantics != ASTR_SEM_MINIAPP) return fail("unsupported semantics mode");
+  if (cfg->semantics != ASTR_SEM_MINIAPP && cfg->semantics != ASTR_SEM_MAIN)
+    return fail("unsupported semantics mode");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail("no HIP device");
   astr_hip_ctx* c = new astr_hip_ctx();
@@ -990,6 +1137,9 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   a.inv_d[1] = (double)cfg->jm / cfg->ly;
   a.inv_d[2] = cfg->km > 0 ? (double)cfg->km / cfg->lz : 0.0;
   a.dt = cfg->deltat;
+  a.main_sem = cfg->semantics == ASTR_SEM_MAIN ? 1 : 0;
+  // the Jacobian of the uniform grid until set_metrics gives gridgeom's
+  a.jac = (cfg->lx / cfg->im) * (cfg->ly / cfg->jm) * (cfg->km > 0 ? cfg->lz / cfg->km : 1.0);
   a.alfa = cfg->alfa_filter;
   a.bhalo = cfg->beta_halo;
   double h33[33];
@@ -1008,8 +1158,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
       c->tuned_ms[pr][d] = 0.f;
     }
   }
-  // fields: q 5, qsave 5, qrhs 5, dv 12, sg 9
-  const long long nfield = 36;
+  // fields: q 5, qsave 5, qrhs 5, dv 12, sg 9; main path: primitives 6
+  const long long nfield = a.main_sem ? 42 : 36;
   size_t bytes = (size_t)(nfield * a.fs) * sizeof(double);
   if (hipMalloc(&c->base, bytes) != hipSuccess) { delete c; return fail("hipMalloc of fields failed"); }
   (void)hipMemset(c->base, 0, bytes);
@@ -1018,6 +1168,18 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   a.qrhs = a.qsave + 5 * a.fs;
   a.dv = a.qrhs + 5 * a.fs;
   a.sg = a.dv + 12 * a.fs;
+  a.pr = a.main_sem ? a.sg + 9 * a.fs : nullptr;
+  if (a.main_sem) {
+    // halo sources kept by qswap: hx then hy, 5 fields each
+    a.hxs = hx_lines(a.n, a.ndir) * kHaloSrcRows;
+    a.hys = hy_lines(a.n, a.ndir) * kHaloSrcRows;
+    if (hipMalloc(&c->halo, (size_t)(5 * (a.hxs + a.hys) + 1) * sizeof(double)) != hipSuccess) {
+      (void)hipFree(c->base); delete c; return fail("hipMalloc of the halo sources failed");
+    }
+    (void)hipMemset(c->halo, 0, (size_t)(5 * (a.hxs + a.hys) + 1) * sizeof(double));
+    a.hx = c->halo;
+    a.hy = c->halo + 5 * a.hxs;
+  }
   // halo-row filter coefficients, then the LU tables
   std::vector<double> all(h33, h33 + 33);
   size_t off[6];
@@ -1031,7 +1193,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
     all.insert(all.end(), t.begin(), t.end());
   }
   if (hipMalloc(&c->coef, all.size() * sizeof(double)) != hipSuccess) {
-    (void)hipFree(c->base); delete c; return fail("hipMalloc of tables failed");
+    (void)hipFree(c->base); if (c->halo) (void)hipFree(c->halo); delete c;
+    return fail("hipMalloc of tables failed");
   }
   (void)hipMemcpy(c->coef, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice);
   a.fc.ch = c->coef;
@@ -1045,7 +1208,8 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
   for (const ShapeInfo& s : kShapes) tl_min = std::min(tl_min, s.TL);
   const int ntl = tiles_of(a.n, a.ndir - 1, tl_min);
   if (hipMalloc(&c->stats_part, (kTileStats * (size_t)ntl + kDissBlocks) * sizeof(double)) != hipSuccess) {
-    (void)hipFree(c->base); (void)hipFree(c->coef); delete c; return fail("hipMalloc of stats failed");
+    (void)hipFree(c->base); (void)hipFree(c->coef); if (c->halo) (void)hipFree(c->halo); delete c;
+    return fail("hipMalloc of stats failed");
   }
   a.stats = c->stats_part;
   c->diss_part = c->stats_part + kTileStats * (size_t)ntl;
@@ -1064,6 +1228,7 @@ int astr_hip_destroy(astr_hip_ctx* c) {
   (void)hipFree(c->coef);
   (void)hipFree(c->stats_part);
   if (c->stats_out) (void)hipFree(c->stats_out);
+  if (c->halo) (void)hipFree(c->halo);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1088,7 +1253,9 @@ int astr_hip_init_tgv(astr_hip_ctx* c, double noise_amp, unsigned long long seed
   hipLaunchKernelGGL(k_init_tgv, pt_grid(c->a), dim3(64), 0, c->stream, c->a, noise_amp,
                      (uint64_t)seed);
   HIP_OK(hipGetLastError());
-  return 0;
+  c->halo_valid = false;
+  // main path: flowinit ends with the primitives of the initial field
+  return c->a.main_sem ? updatefvar_dev(c) : 0;
 }
 
 int astr_hip_init_vortex(astr_hip_ctx* c, double beta, double xc, double yc, double u0, double v0) {
@@ -1098,7 +1265,8 @@ int astr_hip_init_vortex(astr_hip_ctx* c, double beta, double xc, double yc, dou
   hipLaunchKernelGGL(k_init_vortex, pt_grid(c->a), dim3(64), 0, c->stream, c->a, beta, xc, yc, u0,
                      v0, c->cfg.lx, c->cfg.ly, g - 1.0, ma * ma);
   HIP_OK(hipGetLastError());
-  return 0;
+  c->halo_valid = false;
+  return c->a.main_sem ? updatefvar_dev(c) : 0;
 }
 
 // halo depth of a host buffer: 0..8 and at most the shortest direction, so a
@@ -1127,13 +1295,20 @@ static int xfer(astr_hip_ctx* c, double* host, int nf, double* dev, int hm, int 
 int astr_hip_upload(astr_hip_ctx* c, const double* q, int hm) {
   if (check_ctx(c)) return -1;
   if (!q || bad_halo(c->a, hm)) return fail("bad argument to astr_hip_upload");
-  return xfer(c, const_cast<double*>(q), 5, c->a.q, hm, 1);
+  if (xfer(c, const_cast<double*>(q), 5, c->a.q, hm, 1)) return -1;
+  c->halo_valid = false;
+  // main path: the primitives of the uploaded q, as updatefvar leaves them
+  if (c->a.main_sem && updatefvar_dev(c)) return -1;
+  if (c->a.main_sem) HIP_OK(hipStreamSynchronize(c->stream));
+  return 0;
 }
 
 int astr_hip_download(astr_hip_ctx* c, double* q, double* prims, int hm) {
   if (check_ctx(c)) return -1;
   if (bad_halo(c->a, hm)) return fail("bad argument to astr_hip_download");
   if (q && xfer(c, q, 5, c->a.q, hm, 0)) return -1;
+  // main path: the stored primitives, as the last updatefvar / qswap left them
+  if (prims && c->a.main_sem) return xfer(c, prims, 6, c->a.pr, hm, 0);
   if (prims) {
     const FlowArgs& a = c->a;
     const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
@@ -1152,6 +1327,16 @@ int astr_hip_updatefvar(astr_hip_ctx* c, double* rho, double* vel, double* prs, 
   if (bad_halo(c->a, hm)) return fail("bad argument to astr_hip_updatefvar");
   const FlowArgs& a = c->a;
   const size_t dn = (size_t)(a.n[0] + 1 + 2 * hm) * (a.n[1] + 1 + 2 * hm) * (a.n[2] + 1 + 2 * hm);
+  if (a.main_sem) {
+    // the stored primitives become those of the current q; each requested
+    // array is packed from them with its periodic halo
+    if (updatefvar_dev(c)) return -1;
+    if (rho && xfer(c, rho, 1, a.pr, hm, 0)) return -1;
+    if (vel && xfer(c, vel, 3, a.pr + a.fs, hm, 0)) return -1;
+    if (prs && xfer(c, prs, 1, a.pr + 4 * a.fs, hm, 0)) return -1;
+    if (tmp && xfer(c, tmp, 1, a.pr + 5 * a.fs, hm, 0)) return -1;
+    return 0;
+  }
   DevBuf pb;
   HIP_OK(pb.alloc(dn * 6 * sizeof(double)));
   hipLaunchKernelGGL(k_prims, dense_grid(a, hm), dim3(64), 0, c->stream, a, pb.p, hm);
@@ -1167,16 +1352,117 @@ int astr_hip_updatefvar(astr_hip_ctx* c, double* rho, double* vel, double* prs, 
   return 0;
 }
 
+int astr_hip_set_fvar(astr_hip_ctx* c, const double* rho, const double* vel, const double* prs,
+                      const double* tmp, int hm) {
+  if (check_ctx(c)) return -1;
+  const FlowArgs& a = c->a;
+  if (!a.main_sem) return fail("astr_hip_set_fvar: the stored primitives exist in ASTR_SEM_MAIN only");
+  if (!rho || !vel || !prs || !tmp || bad_halo(a, hm)) return fail("bad argument to astr_hip_set_fvar");
+  if (xfer(c, const_cast<double*>(rho), 1, a.pr, hm, 1) || xfer(c, const_cast<double*>(vel), 3, a.pr + a.fs, hm, 1) ||
+      xfer(c, const_cast<double*>(prs), 1, a.pr + 4 * a.fs, hm, 1) ||
+      xfer(c, const_cast<double*>(tmp), 1, a.pr + 5 * a.fs, hm, 1))
+    return -1;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// checkpoint / restart (writechkpt / readcheckpoint, src/readwrite.F90:2390,
+// 1381).  The reference writes ro, u1, u2, u3, p, t, nstep and time to an
+// HDF5 flowfield file and rebuilds q from them at restart (updateq); without
+// HDF5 this file is raw little-endian FP64 behind a fixed header, and it also
+// holds q, so that a resumed run continues bit for bit.
+//   bytes 0..7   "ASTRCHK1"
+//   int32        header bytes (64), im, jm, km, semantics, nfields (11)
+//   int64        nstep
+//   double       time
+//   (zero padding to 64 bytes)
+//   nfields fields of (im+1)(jm+1)(km+1) doubles, i fastest, in the order
+//   ro, u1, u2, u3, p, t (the stored primitives in the main semantics, the
+//   primitives of q otherwise), q1..q5
+// ---------------------------------------------------------------------------
+namespace {
+constexpr char kChkMagic[9] = "ASTRCHK1";
+constexpr int kChkHeader = 64;
+constexpr int kChkFields = 11;
+
+struct ChkHeader {
+  char magic[8];
+  int32_t hbytes, im, jm, km, semantics, nfields;
+  int64_t nstep;
+  double time;
+  char pad[kChkHeader - 8 - 6 * 4 - 8 - 8];
+};
+static_assert(sizeof(ChkHeader) == kChkHeader, "checkpoint header size");
+}  // namespace
+
+int astr_hip_write_checkpoint(astr_hip_ctx* c, const char* path, long long nstep, double time) {
+  if (check_ctx(c)) return -1;
+  if (!path) return fail("null checkpoint path");
+  const FlowArgs& a = c->a;
+  const size_t np = (size_t)(a.n[0] + 1) * (a.n[1] + 1) * (a.n[2] + 1);
+  std::vector<double> buf((size_t)kChkFields * np);
+  if (astr_hip_download(c, buf.data() + 6 * np, buf.data(), 0)) return -1;
+  ChkHeader h;
+  std::memset(&h, 0, sizeof(h));
+  std::memcpy(h.magic, kChkMagic, 8);
+  h.hbytes = kChkHeader;
+  h.im = a.n[0]; h.jm = a.n[1]; h.km = a.n[2];
+  h.semantics = a.main_sem ? ASTR_SEM_MAIN : ASTR_SEM_MINIAPP;
+  h.nfields = kChkFields;
+  h.nstep = nstep;
+  h.time = time;
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(std::string("cannot open checkpoint for writing: ") + path);
+  const bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 &&
+                  std::fwrite(buf.data(), sizeof(double), buf.size(), f) == buf.size();
+  if (std::fclose(f) != 0 || !ok) return fail(std::string("checkpoint write failed: ") + path);
+  return 0;
+}
+
+int astr_hip_read_checkpoint(astr_hip_ctx* c, const char* path, long long* nstep, double* time) {
+  if (check_ctx(c)) return -1;
+  if (!path) return fail("null checkpoint path");
+  const FlowArgs& a = c->a;
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(std::string("cannot open checkpoint: ") + path);
+  ChkHeader h;
+  const bool hok = std::fread(&h, sizeof(h), 1, f) == 1;
+  auto bad = [&](const std::string& m) {
+    std::fclose(f);
+    return fail("checkpoint " + std::string(path) + ": " + m);
+  };
+  if (!hok || std::memcmp(h.magic, kChkMagic, 8) != 0 || h.hbytes != kChkHeader)
+    return bad("not an astr_hip checkpoint");
+  if (h.im != a.n[0] || h.jm != a.n[1] || h.km != a.n[2]) return bad("grid size differs from the context's");
+  if (h.nfields != kChkFields) return bad("unexpected field count");
+  if (h.semantics != (a.main_sem ? ASTR_SEM_MAIN : ASTR_SEM_MINIAPP))
+    return bad("written under the other semantics mode");
+  const size_t np = (size_t)(a.n[0] + 1) * (a.n[1] + 1) * (a.n[2] + 1);
+  std::vector<double> buf((size_t)kChkFields * np);
+  if (std::fread(buf.data(), sizeof(double), buf.size(), f) != buf.size()) return bad("truncated");
+  std::fclose(f);
+  if (astr_hip_upload(c, buf.data() + 6 * np, 0)) return -1;
+  if (a.main_sem) {
+    // the stored primitives as written (they lag q by the filter)
+    const double* p = buf.data();
+    if (astr_hip_set_fvar(c, p, p + np, p + 4 * np, p + 5 * np, 0)) return -1;
+  }
+  if (nstep) *nstep = h.nstep;
+  if (time) *time = h.time;
+  return 0;
+}
+
 int astr_hip_filterq(astr_hip_ctx* c) {
   if (check_ctx(c)) return -1;
   for (int d = 0; d < c->a.ndir; d++)
     if (pass(c, F_ONLY, d)) return -1;
+  c->halo_valid = false;
   return 0;
 }
 
 int astr_hip_gradcal(astr_hip_ctx* c, double* dvel, double* dtmp) {
   if (check_ctx(c)) return -1;
-  if (run_rhs(c, false, false, true)) return -1;
+  if (ensure_halo(c) || run_rhs(c, false, false, true)) return -1;
   if (dvel && xfer(c, dvel, 9, c->a.dv, 0, 0)) return -1;
   if (dtmp && xfer(c, dtmp, 3, c->a.dv + 9 * c->a.fs, 0, 0)) return -1;
   return 0;
@@ -1184,7 +1470,7 @@ int astr_hip_gradcal(astr_hip_ctx* c, double* dvel, double* dtmp) {
 
 int astr_hip_rhscal(astr_hip_ctx* c, double* qrhs) {
   if (check_ctx(c)) return -1;
-  if (run_rhs(c, false, false, true)) return -1;
+  if (ensure_halo(c) || run_rhs(c, false, false, true)) return -1;
   if (qrhs && xfer(c, qrhs, 5, c->a.qrhs, 0, 0)) return -1;
   return 0;
 }
@@ -1299,12 +1585,16 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   if (reps < 1 || reps > 1000) return fail("autotune reps must be 1..1000");
   FlowArgs& a = c->a;
   const FlowArgs keep = a;
-  // the passes overwrite q and qsave: keep a copy and put it back afterwards
+  // the passes overwrite q and qsave (and the main path's stored
+  // primitives): keep a copy and put it back afterwards
   const size_t qbytes = (size_t)10 * a.fs * sizeof(double);
+  const size_t pbytes = a.main_sem ? (size_t)6 * a.fs * sizeof(double) : 0;
   DevBuf snapb;
-  HIP_OK(snapb.alloc(qbytes));
+  HIP_OK(snapb.alloc(qbytes + pbytes));
   double* snap = snapb.p;
   HIP_OK(hipMemcpyAsync(snap, a.q, qbytes, hipMemcpyDeviceToDevice, c->stream));
+  if (pbytes) HIP_OK(hipMemcpyAsync(snap + 10 * a.fs, a.pr, pbytes, hipMemcpyDeviceToDevice, c->stream));
+  if (ensure_halo(c)) return -1;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     if (e0) (void)hipEventDestroy(e0);
@@ -1368,6 +1658,8 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   }
   a = keep;
   hipError_t e = hipMemcpyAsync(a.q, snap, qbytes, hipMemcpyDeviceToDevice, c->stream);
+  if (e == hipSuccess && pbytes)
+    e = hipMemcpyAsync(a.pr, snap + 10 * a.fs, pbytes, hipMemcpyDeviceToDevice, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -1521,6 +1813,7 @@ int astr_hip_gridgeom(astr_hip_ctx* c, const double* x, double* jacob, double* d
 int astr_hip_qswap(astr_hip_ctx* c) {
   if (check_ctx(c)) return -1;
   HIP_OK(hipSetDevice(c->device));
+  if (c->a.main_sem) return qswap_main(c, true);
   const FlowArgs& a = c->a;
   for (int d = 0; d < a.ndir; d++) {
     const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
@@ -1565,6 +1858,9 @@ int astr_hip_set_metrics(astr_hip_ctx* c, const double* jacob, const double* dxi
   for (long long o = 0; o < fs; o++)
     if (!(std::fabs(jacob[o] - jm) <= tol * jm)) return fail("metrics: Jacobian inconsistent with d xi/d x");
   for (int d = 0; d < nd; d++) c->a.inv_d[d] = mean[d];
+  long double js = 0.0L;
+  for (long long o = 0; o < fs; o++) js += jacob[o];
+  c->a.jac = (double)(js / (long double)fs);
   return 0;
 }
 
@@ -1818,6 +2114,7 @@ int astr_hip_batch_create(astr_hip_ctx* const* cases, int ncases, astr_hip_batch
   for (int i = 0; i < ncases; i++) {
     if (!cases[i]) { delete b; return fail("null case"); }
     if (i && cases[i]->device != cases[0]->device) { delete b; return fail("cases on different devices"); }
+    if (cases[i]->a.main_sem) { delete b; return fail("batches take miniapp-semantics cases only"); }
     b->cases.push_back(cases[i]);
   }
   b->device = cases[0]->device;

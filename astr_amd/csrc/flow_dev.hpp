@@ -38,7 +38,44 @@ struct FlowArgs {
   const double* dac[3]; // derivative LU tables [ac1|ac2|ac3] per direction
   const double* fac[3]; // filter LU tables
   FilterCoef fc;
+  // Main-path semantics (src/mainloop.F90:396-482; passes instantiated with
+  // MAIN = true).  The metrics of a uniform Cartesian grid: inv_d[d] holds
+  // dxi(d,d) and jac the Jacobian; dxi(d,n) = 0 for d != n.
+  int main_sem;         // 1: main path, 0: miniapp
+  double jac;
+  double* pr;           // 6 fields: the primitives rho, u, v, w, p, T as the
+                        // last updatefvar / qswap left them
+  const double* hx;     // halo sources of the x-lines on j or k boundary planes
+  const double* hy;     // halo sources of the y-lines on k boundary planes
+  long long hxs, hys;   // field strides of hx, hy
 };
+
+// Halo sources kept by qswap for the lines whose halo the reference fills
+// before a later direction averages the source points (qswap's i step copies
+// the x-halo of the lines on the j and k boundary planes before the j and k
+// steps average those planes; its j step copies the y-halo of the lines on
+// the k boundary planes before the k step).  Ten rows per line: rows 1..5 and
+// n-5..n-1 of the line, whose copies are the halo rows n+1..n+5 and -5..-1.
+constexpr int kHaloSrcRows = 10;
+__host__ __device__ inline long long hx_lines(const int n[3], int ndir) {
+  return 2ll * (n[2] + 1) + (ndir == 3 ? 2ll * (n[1] + 1) : 0);
+}
+__host__ __device__ inline long long hy_lines(const int n[3], int ndir) {
+  return ndir == 3 ? 2ll * (n[0] + 1) : 0;
+}
+// index of the halo source (line, hrow) of an x-line (j,k) or y-line (i,k)
+// in hx / hy, or -1 when the line takes its halo from the current q
+__host__ __device__ inline int halo_src_index(const int n[3], int ndir, int dir, int i, int j, int k,
+                                              int hrow) {
+  if (dir == 0) {
+    if (j == 0 || j == n[1]) return ((j == 0 ? 0 : 1) * (n[2] + 1) + k) * kHaloSrcRows + hrow;
+    if (ndir == 3 && (k == 0 || k == n[2]))
+      return (2 * (n[2] + 1) + (k == 0 ? 0 : 1) * (n[1] + 1) + j) * kHaloSrcRows + hrow;
+  } else if (dir == 1 && ndir == 3) {
+    if (k == 0 || k == n[2]) return ((k == 0 ? 0 : 1) * (n[0] + 1) + i) * kHaloSrcRows + hrow;
+  }
+  return -1;
+}
 
 // Pass programs.  GL_GRAD + GL_CONV are GC_LAST split in two kernels (the
 // gradient/stress half and the convective half), each with fewer registers.
@@ -66,6 +103,8 @@ struct Slots {
                        // past the end of the lattice (partial last tile)
   int s[C::NPT];  // index of the point in a line buffer (l*LT + r)
   int h[C::NPT];  // line-buffer index of the slot's periodic halo copy
+  int ex[C::NPT]; // main path: index of the halo copy's value in hx / hy
+                  // (halo_src_index), or -1: the copy takes the point's q
   int interior;   // bit t: i,j,k >= 1 (stats domain)
 };
 
@@ -186,7 +225,7 @@ __device__ __forceinline__ FilterCoef stage_halo_coefs(const FilterCoef& fc, dou
 
 // Map the tile's points onto slots.  dir 0: lines are (j,k), rows i; dir 1:
 // lines (i,k) rows j; dir 2: lines (i,j) rows k.
-template <class C>
+template <class C, bool MAIN = false>
 __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int dir, int tile) {
   const int n = a.n[dir];
   const int np = C::TL * (n + 1);
@@ -199,6 +238,7 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
     S.g[t] = (unsigned)a.trash * 8u;
     S.s[t] = dummy;
     S.h[t] = dummy;
+    if (MAIN) S.ex[t] = -1;
     if (e < np) {
       int l, r, i, j, k;
       bool ok;
@@ -223,6 +263,8 @@ __device__ __forceinline__ void make_slots(Slots<C>& S, const FlowArgs& a, int d
       // n+1..n+5 (disjoint for n >= 11)
       if (r >= n - 5 && r <= n - 1) S.h[t] = S.s[t] - n;
       if (r >= 1 && r <= 5) S.h[t] = S.s[t] + n;
+      if (MAIN && ok && S.h[t] != dummy)
+        S.ex[t] = halo_src_index(a.n, a.ndir, dir, i, j, k, r <= 5 ? r - 1 : r - (n - 5) + 5);
       if (ok) {
         S.g[t] = (unsigned)((long long)k * a.plane + (long long)j * a.pitch + i) * 8u;
         const bool in = i >= 1 && j >= 1 && (a.ndir == 2 || k >= 1);
@@ -340,21 +382,68 @@ __device__ __forceinline__ void visc_flux(const FlowArgs& a, const double g[3][3
   out[8] = hcc * gt[2] + out[2] * P.u + out[4] * P.v + out[5] * P.w;
 }
 
+// Main path (convrsdcal6, src/solver.F90:2196-2207): flux component m along
+// dir, J (q_m U + dxi(dir,m) p) with the contravariant velocity
+// U = sum_n dxi(dir,n) u_n, for the metrics of a uniform Cartesian grid
+// (dxi(dir,n) = 0 for n != dir, so those products add exact zeros)
+__device__ __forceinline__ double conv_flux_main(const FlowArgs& a, int m, int dir, double qm,
+                                                 double q5, double ud, double p) {
+#pragma clang fp contract(off)
+  const double dx = a.inv_d[dir];
+  const double uu = dx * ud;
+  if (m == 0) return a.jac * qm * uu;
+  if (m == 4) return a.jac * (q5 + p) * uu;
+  if (m == dir + 1) return a.jac * (qm * uu + dx * p);
+  return a.jac * (qm * uu);
+}
+
+// Main path: q of the halo copy of slot t.  qswap fills the halo of a line
+// from the filtered q; the lines on boundary planes that a later qswap step
+// averages take the values kept before that step (hx / hy), the others the
+// current q of the source point, which the slot holds.
+template <class C>
+__device__ __forceinline__ void halo_q(const FlowArgs& a, const Slots<C>& S, int t, int dir,
+                                       const double (&qv)[5][C::NPT], double (&qh)[5]) {
+  if (S.ex[t] >= 0) {
+    const double* src = dir == 0 ? a.hx : a.hy;
+    const long long st = dir == 0 ? a.hxs : a.hys;
+#pragma unroll
+    for (int m = 0; m < 5; m++) qh[m] = src[m * st + S.ex[t]];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; m++) qh[m] = qv[m][t];
+  }
+}
+
+// main path RK3 update (src/mainloop.F90:441-449):
+// q <- (a qsave + b q J + c qrhs dt) / J, in the reference's order
+__device__ __forceinline__ double rk_update_main(const StageArgs& sa, double dt, double jac, double qs,
+                                                 double qo, double qr) {
+#pragma clang fp contract(off)
+  return (sa.rk_a * qs + sa.rk_b * qo * jac + sa.rk_c * qr * dt) / jac;
+}
+
 // ---------------------------------------------------------------------------
 // the pass kernel
 // ---------------------------------------------------------------------------
 // MINB: workgroups per CU the register allocation is held to (each
 // workgroup puts one wave on each SIMD, so MINB waves per SIMD)
 // One tile of one pass.  `a` describes the case, `sa` the stage.
-template <class C, int PROG>
+// MAIN: the main path's semantics (src/mainloop.F90:396-482): the gradients
+// and fluxes take the stored primitives (pr) as the last updatefvar left them
+// and their halo copies from the filtered q (qswap), the fluxes carry the
+// metrics J and dxi (convrsdcal6, diffrsdcal6), and the RK pass ends with
+// updatefvar instead of the filter.
+template <class C, int PROG, bool MAIN = false>
 __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa, int dir, int tile,
                                           double* smem) {
   constexpr int NPT = C::NPT;
+  constexpr int kDummy = C::TL * C::LT;
   const int n = a.n[dir];
   double* X = smem;                 // output lines, TL x LT (rows at offset 0)
   double* T = smem + C::TL * C::LT; // input lines, TL x LT
   Slots<C> S;
-  make_slots<C>(S, a, dir, tile);
+  make_slots<C, MAIN>(S, a, dir, tile);
   zero_pad<C>(T, n);
   const int ln = threadIdx.x / C::P, p = threadIdx.x % C::P;
   double* Tl = T + ln * C::LT;
@@ -386,7 +475,23 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     for (int c = 0; c < 4; c++) {
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        {
+        if constexpr (MAIN) {
+          // the stored u, v, w or T (gradcal, src/comsolver.F90:280-300);
+          // the halo copy from the primitives of the halo's q (qswap's q2fvar)
+          T[kLineOff + S.s[t]] = ldg(a.pr + (c < 3 ? 1 + c : 5) * fs, S.g[t]);
+          if (S.h[t] != kDummy) {
+            double qh[5];
+            halo_q<C>(a, S, t, dir, qv, qh);
+            double f;
+            if (c < 3) {
+              f = qh[c + 1] / qh[0];
+            } else {
+              const double pr = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+              f = pr / qh[0] * a.const2;
+            }
+            T[kLineOff + S.h[t]] = f;
+          }
+        } else {
           double f = 0.0;
           {
             if (c < 3) {
@@ -440,7 +545,19 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
             g[0][d] = 0.0; g[1][d] = 0.0; g[2][d] = 0.0; gt[d] = 0.0;
           }
         }
-        const Prim P = prim_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
+        Prim P;
+        if constexpr (MAIN) {
+          // diffrsdcal6 (src/solver.F90:2433-2545) and the statistics
+          // (statistic.F90:871-992) take the stored rho, vel, tmp
+          P.r = sa.dostats ? ldg(a.pr, o) : 0.0;
+          P.u = ldg(a.pr + fs, o);
+          P.v = ldg(a.pr + 2 * fs, o);
+          P.w = ldg(a.pr + 3 * fs, o);
+          P.p = 0.0;
+          P.t = ldg(a.pr + 5 * fs, o);
+        } else {
+          P = prim_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
+        }
         double f[9];
         visc_flux(a, g, gt, P, f);
 #pragma unroll
@@ -480,14 +597,32 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     double ud[NPT], pp[NPT];
 #pragma unroll
     for (int t = 0; t < NPT; t++) {
-      pp[t] = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
-      ud[t] = (dir == 0 ? qv[1][t] : (dir == 1 ? qv[2][t] : qv[3][t])) / qv[0][t];
+      if constexpr (MAIN) {
+        // convrsdcal6 takes the stored vel and prs (src/solver.F90:2196-2207)
+        pp[t] = ldg(a.pr + 4 * fs, S.g[t]);
+        ud[t] = ldg(a.pr + (1 + dir) * fs, S.g[t]);
+      } else {
+        pp[t] = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
+        ud[t] = (dir == 0 ? qv[1][t] : (dir == 1 ? qv[2][t] : qv[3][t])) / qv[0][t];
+      }
     }
 #pragma unroll
     for (int m = 0; m < 5; m++) {
 #pragma unroll
-      for (int t = 0; t < NPT; t++)
-        put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
+      for (int t = 0; t < NPT; t++) {
+        if constexpr (MAIN) {
+          T[kLineOff + S.s[t]] = conv_flux_main(a, m, dir, qv[m][t], qv[4][t], ud[t], pp[t]);
+          if (S.h[t] != kDummy) {
+            double qh[5];
+            halo_q<C>(a, S, t, dir, qv, qh);
+            const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+            const double uh = qh[1 + dir] / qh[0];
+            T[kLineOff + S.h[t]] = conv_flux_main(a, m, dir, qh[m], qh[4], uh, ph);
+          }
+        } else {
+          put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
+        }
+      }
       double* dst = a.qrhs + m * fs;
       double qr[NPT];
       if (PROG != GC_FIRST) {
@@ -507,7 +642,8 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
         {
-          const double v = X[S.s[t]] * idn;
+          // the main path's fluxes carry the metrics: no 1/dx after the solve
+          const double v = MAIN ? X[S.s[t]] : X[S.s[t]] * idn;
           if (PROG == GC_FIRST) stg(dst, S.g[t], v);
           else if (PROG == GC_MID) stg(dst, S.g[t], qr[t] + v);
           else stg(dst, S.g[t], -(qr[t] + v));
@@ -531,18 +667,47 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     double nx[NPT];
 #pragma unroll
     for (int t = 0; t < NPT; t++) nx[t] = ldg(a.sg + c0 * fs, S.g[t]);
-    if constexpr (!RK) {
+    if constexpr (!RK || MAIN) {
+      // MAIN && RK (time_integration_rk, src/mainloop.F90:430-482): each
+      // variable's RK3 update follows its viscous sum, density (no viscous
+      // term) first; updatefvar of the new q ends the pass
+      constexpr bool RKM = RK && MAIN;
+      double qn[5][NPT];
+      if constexpr (RKM) {
+#pragma unroll
+        for (int t = 0; t < NPT; t++) {
+          const unsigned o = S.g[t];
+          const double q0 = ldg(a.q, o);
+          double s0;
+          if (sa.stage == 0) {
+            s0 = q0 * a.jac;  // qsave = q J (:423-427)
+            stg(a.qsave, o, s0);
+          } else {
+            s0 = ldg(a.qsave, o);
+          }
+          qn[0][t] = rk_update_main(sa, a.dt, a.jac, s0, q0, ldg(a.qrhs, o));
+          stg(a.q, o, qn[0][t]);
+        }
+      }
 #pragma unroll
       for (int c = 0; c < 4; c++) {
+        // the main path's viscous fluxes carry the metrics:
+        // (sigma . dxi(dir,:)) J (diffrsdcal6, src/solver.F90:2623-2635)
 #pragma unroll
         for (int t = 0; t < NPT; t++)
-          put<C>(T, S, t, nx[t]);
+          put<C>(T, S, t, MAIN ? nx[t] * idn * a.jac : nx[t]);
         // loads for this column's accumulation and the next column's input are
         // issued before the solve so their latency overlaps it
-        double qr[NPT];
+        double qr[NPT], qo[NPT], qs[NPT];
         double* dst = a.qrhs + (1 + c) * fs;
 #pragma unroll
-        for (int t = 0; t < NPT; t++) qr[t] = ldg(dst, S.g[t]);
+        for (int t = 0; t < NPT; t++) {
+          qr[t] = ldg(dst, S.g[t]);
+          if constexpr (RKM) {
+            qo[t] = ldg(a.q + (1 + c) * fs, S.g[t]);
+            qs[t] = sa.stage == 0 ? 0.0 : ldg(a.qsave + (1 + c) * fs, S.g[t]);
+          }
+        }
         if (c < 3) {
           const int col = c == 0 ? c1 : (c == 1 ? c2 : 6 + dir);
 #pragma unroll
@@ -556,7 +721,34 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         }
         __syncthreads();
 #pragma unroll
-        for (int t = 0; t < NPT; t++) stg(dst, S.g[t], qr[t] + X[S.s[t]] * idn);
+        for (int t = 0; t < NPT; t++) {
+          const double v = qr[t] + (MAIN ? X[S.s[t]] : X[S.s[t]] * idn);
+          if constexpr (RKM) {
+            double s0 = qs[t];
+            if (sa.stage == 0) {
+              s0 = qo[t] * a.jac;
+              stg(a.qsave + (1 + c) * fs, S.g[t], s0);
+            }
+            qn[1 + c][t] = rk_update_main(sa, a.dt, a.jac, s0, qo[t], v);
+            stg(a.q + (1 + c) * fs, S.g[t], qn[1 + c][t]);
+          } else {
+            stg(dst, S.g[t], v);
+          }
+        }
+      }
+      if constexpr (RKM) {
+        // updatefvar (fludyna.F90:191-242): rho, u, v, w, p, T of the new q
+#pragma unroll
+        for (int t = 0; t < NPT; t++) {
+          const Prim P = prim_of(a, qn[0][t], qn[1][t], qn[2][t], qn[3][t], qn[4][t]);
+          const unsigned o = S.g[t];
+          stg(a.pr, o, P.r);
+          stg(a.pr + fs, o, P.u);
+          stg(a.pr + 2 * fs, o, P.v);
+          stg(a.pr + 3 * fs, o, P.w);
+          stg(a.pr + 4 * fs, o, P.p);
+          stg(a.pr + 5 * fs, o, P.t);
+        }
       }
     } else {
       const double* csf = stage_chunk_coefs<C>(a.fac[dir], n + 7, smem, 1);
@@ -662,10 +854,10 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 }
 
 // one pass of one case: block b handles tile xcd_tile(b)
-template <class C, int PROG, int MINB>
+template <class C, int PROG, int MINB, bool MAIN = false>
 __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
   extern __shared__ double smem[];
-  pass_body<C, PROG>(a, stage_of(a), dir, xcd_tile(blockIdx.x, gridDim.x), smem);
+  pass_body<C, PROG, MAIN>(a, stage_of(a), dir, xcd_tile(blockIdx.x, gridDim.x), smem);
 }
 
 // The tiles of one pass over several cases: case k owns global tiles
@@ -716,22 +908,29 @@ hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageA
   return hipGetLastError();
 }
 
-template <class C, int MINB>
-hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
+template <class C, int MINB, bool MAIN>
+void launch_pass_m(int prog, const FlowArgs& a, int dir, hipStream_t st) {
   const int nt = tiles_of(a.n, dir, C::TL);
   const size_t lds = (size_t)lds_doubles<C>(prog) * sizeof(double);
   dim3 g(nt), b(C::NT);
   switch (prog) {
-    case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST, MINB>), g, b, lds, st, a, dir); break;
-    case GC_MID: hipLaunchKernelGGL((k_pass<C, GC_MID, MINB>), g, b, lds, st, a, dir); break;
-    case GC_LAST: hipLaunchKernelGGL((k_pass<C, GC_LAST, MINB>), g, b, lds, st, a, dir); break;
-    case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID, MINB>), g, b, lds, st, a, dir); break;
-    case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS, MINB>), g, b, lds, st, a, dir); break;
-    case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB>), g, b, lds, st, a, dir); break;
-    case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB>), g, b, lds, st, a, dir); break;
-    case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB>), g, b, lds, st, a, dir); break;
-    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB>), g, b, lds, st, a, dir); break;
+    case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case GC_MID: hipLaunchKernelGGL((k_pass<C, GC_MID, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case GC_LAST: hipLaunchKernelGGL((k_pass<C, GC_LAST, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    // the filter pass is the same in both semantics
+    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB, false>), g, b, lds, st, a, dir); break;
   }
+}
+
+template <class C, int MINB>
+hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
+  if (a.main_sem) launch_pass_m<C, MINB, true>(prog, a, dir, st);
+  else launch_pass_m<C, MINB, false>(prog, a, dir, st);
   return hipGetLastError();
 }
 

@@ -13,6 +13,8 @@
 !                        rk3, steploop.  The fields live in HBM; the host
 !                        arrays are filled when a subroutine's outputs are
 !                        the reference's module arrays.
+! The main solver's run (astr run) is astr_hip_run.F90 (module
+! astr_hip_main, program astr_hip).
 ! module astr_hip_thermchem  chemread, chemrate, cpeval, enthpy,
 !                        temperature_calc, rgcmix_scar, convertxiyi with the
 !                        reference's argument lists (src/thermchem.F90:59,
@@ -23,7 +25,7 @@ module astr_hip_capi
   use iso_c_binding
   implicit none
 
-  integer(c_int), parameter :: ASTR_SEM_MINIAPP = 0
+  integer(c_int), parameter :: ASTR_SEM_MINIAPP = 0, ASTR_SEM_MAIN = 1
 
   type, bind(C) :: astr_hip_config
     integer(c_int) :: im, jm, km, semantics
@@ -154,6 +156,30 @@ module astr_hip_capi
       type(c_ptr), value :: ctx, jacob, dxi
       integer(c_int) :: astr_hip_set_metrics
     end function
+    ! the main path's stored primitives rho, vel(...,1:3), prs, tmp
+    function astr_hip_set_fvar(ctx, rho, vel, prs, tmp, hm) bind(C, name='astr_hip_set_fvar')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx, rho, vel, prs, tmp
+      integer(c_int), value :: hm
+      integer(c_int) :: astr_hip_set_fvar
+    end function
+    ! writechkpt / readcheckpoint (src/readwrite.F90:2390, 1381)
+    function astr_hip_write_checkpoint(ctx, path, nstep, time) bind(C, name='astr_hip_write_checkpoint')
+      import :: c_int, c_ptr, c_char, c_long_long, c_double
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_long_long), value :: nstep
+      real(c_double), value :: time
+      integer(c_int) :: astr_hip_write_checkpoint
+    end function
+    function astr_hip_read_checkpoint(ctx, path, nstep, time) bind(C, name='astr_hip_read_checkpoint')
+      import :: c_int, c_ptr, c_char, c_long_long, c_double
+      type(c_ptr), value :: ctx
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_long_long), intent(out) :: nstep
+      real(c_double), intent(out) :: time
+      integer(c_int) :: astr_hip_read_checkpoint
+    end function
     function astr_hip_sync(ctx) bind(C, name='astr_hip_sync')
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
@@ -266,10 +292,6 @@ module astr_hip_tgv
   type(c_ptr) :: ctx = c_null_ptr
   integer :: statfile = 13
   logical :: statopen = .false.
-  ! flowstate.dat of the main path (statistic.F90:777-859): written by
-  ! steploop after readinput selected a tgv run
-  logical :: lflowstate = .false., fsopen = .false.
-  integer :: fsfile
 
 contains
 
@@ -285,108 +307,6 @@ contains
     alfa_filter = 0.49d0
     write(*, '(3(A,I0))') '  ** dimension set: ', im, ' x ', jm, ' x ', km
   end subroutine program_init
-
-  ! readinput (src/readwrite.F90:466-716, the build without COMB): the
-  ! datin input file, read record by record as the reference reads it.  It
-  ! sets the grid, the filter coefficient and the reference values.  Options
-  ! outside what this build runs (a periodic box of flowtype tgv with the 643c
-  ! scheme, rk3, the filter, no species, no turbulence model, no grid file,
-  ! no restart) stop with a message, as the reference stops on bad input.
-  subroutine readinput(inputfile)
-    character(len=*), intent(in) :: inputfile
-    character(len=64) :: flowtype, gridfile
-    character(len=16) :: conschm, dschm, rkscheme, turbmode, iomode
-    character(len=255) :: lineread
-    logical :: lihomo, ljhomo, lkhomo, nondimen, diffterm, lfilter, lreadgrid, lfftk, &
-               limmbou, ltimrpt, lrestart, lchardecomp
-    integer :: fh, n, ia, ja, ka, kcutoff, recon_schem, num_species, ninit, bctype(6)
-    real(8) :: afil, rtem, re, ma, bfacmpld, shkcrt
-
-    open(newunit=fh, file=trim(inputfile), action='read', status='old')
-    read(fh, '(////)')
-    read(fh, *) flowtype
-    read(fh, '(/)')
-    read(fh, *) ia, ja, ka
-    read(fh, '(/)')
-    read(fh, *) lihomo, ljhomo, lkhomo
-    read(fh, '(/)')
-    read(fh, *) nondimen, diffterm, lfilter, lreadgrid, lfftk, limmbou, ltimrpt
-    read(fh, '(/)')
-    read(fh, *) lrestart
-    read(fh, '(/)')
-    read(fh, *) afil, kcutoff
-    if (.not. nondimen) call input_stop('dimensional input (nondimen = f)')
-    read(fh, '(/)')
-    read(fh, *) rtem, re, ma
-    read(fh, '(/)')
-    read(fh, *) conschm, dschm, rkscheme
-    read(fh, '(/)')
-    read(fh, *) recon_schem, lchardecomp, bfacmpld, shkcrt
-    read(fh, '(/)')
-    read(fh, *) num_species
-    read(fh, '(/)')
-    read(fh, *) turbmode, iomode
-    read(fh, '(/)')
-    do n = 1, 6
-      read(fh, *) bctype(n)
-    end do
-    read(fh, '(/)')
-    read(fh, *) ninit
-    read(fh, '(/)')
-    read(fh, '(A)') lineread
-    read(fh, '(/)')
-    read(fh, '(A)') gridfile
-    close(fh)
-
-    if (trim(flowtype) /= 'tgv') call input_stop('flowtype '//trim(flowtype))
-    if (.not. (lihomo .and. ljhomo .and. lkhomo)) call input_stop('non-homogeneous directions')
-    if (any(bctype /= 1)) call input_stop('boundary types other than periodic (1)')
-    if (.not. diffterm) call input_stop('diffterm = f')
-    if (.not. lfilter) call input_stop('lfilter = f')
-    if (lreadgrid .or. lfftk .or. limmbou .or. lrestart) &
-      call input_stop('lreadgrid, lfftz, limmbou or lrestart = t')
-    if (trim(conschm) /= '643c' .or. trim(dschm) /= '643c') &
-      call input_stop('schemes '//trim(conschm)//', '//trim(dschm))
-    if (trim(rkscheme) /= 'rk3') call input_stop('time scheme '//trim(rkscheme))
-    if (num_species /= 0) call input_stop('species (num_species > 0)')
-    if (trim(turbmode) /= 'none') call input_stop('turbmode '//trim(turbmode))
-    if (ninit /= 0) call input_stop('ninit /= 0')
-
-    im = ia; jm = ja; km = ka
-    numq = 5
-    lflowstate = .true.
-    alfa_filter = afil
-    ref_tem = rtem; reynolds = re; mach = ma
-    difschm = dschm(1:4)
-    write(*, '(3(A,I0))') '  ** dimension set: ', im, ' x ', jm, ' x ', km
-    write(*, '(A,4(1X,ES22.15))') '  ** ref_tem, reynolds, mach, alfa_filter:', ref_tem, reynolds, &
-      mach, alfa_filter
-    print *, ' >> ', trim(inputfile), ' ... done'
-  end subroutine readinput
-
-  ! readcont (src/readwrite.F90:729-769): datin/controller, the step count
-  ! and the time step
-  subroutine readcont(contfile)
-    character(len=*), intent(in) :: contfile
-    logical :: lwsequ, lwslic, lavg, lcracon
-    integer :: fh, feqchkpt, feqwsequ, feqslice, feqlist, feqavg
-    open(newunit=fh, file=trim(contfile), action='read', status='old')
-    read(fh, '(////)')
-    read(fh, *) lwsequ, lwslic, lavg, lcracon
-    read(fh, '(/)')
-    read(fh, *) maxstep, feqchkpt, feqwsequ, feqslice, feqlist, feqavg
-    read(fh, '(/)')
-    read(fh, *) deltat
-    close(fh)
-    write(*, '(A,I0,A,ES22.15)') '  ** maxstep ', maxstep, ', deltat', deltat
-    print *, ' >> ', trim(contfile), ' ... done'
-  end subroutine readcont
-
-  subroutine input_stop(what)
-    character(len=*), intent(in) :: what
-    print *, ' !! input not supported by the device solver: ', what
-    stop 1
-  end subroutine input_stop
 
   ! solver_init (tgvsolver.F90:236-261): scheme and filter coefficients,
   ! built on the device with the flow constants (flowfield_init :181-196)
@@ -491,46 +411,20 @@ contains
     n = maxstep - nstep
     if (n <= 0) return
     allocate(ke(n), en(n), di(n))
-    if (lflowstate) then
-      call hipcheck(astr_hip_rk3_steps_stats(ctx, int(n, c_int), c_loc(ke), c_loc(en), c_loc(di)), &
-                    'steploop')
-    else
-      call hipcheck(astr_hip_rk3_steps(ctx, int(n, c_int), c_loc(ke), c_loc(en)), 'steploop')
-    end if
+    call hipcheck(astr_hip_rk3_steps(ctx, int(n, c_int), c_loc(ke), c_loc(en)), 'steploop')
     do it = 1, n
       call stacal(ke(it), en(it))
-      if (lflowstate) call statout(ke(it), en(it), di(it))
       nstep = nstep + 1
       time = time + deltat
     end do
     if (statopen) flush(statfile)
-    if (fsopen) flush(fsfile)
   end subroutine steploop
-
-  ! statout for flowtype tgv (statistic.F90:777-859) with listinit and
-  ! listwrite (utility.F90:310-434): a title line of A20 words, then per
-  ! step nstep, time/ref_tim (ref_tim = 1 for nondimensional input),
-  ! kenergy, enstophy, dissipation
-  subroutine statout(kenergy, enstophy, dissipation)
-    real(8), intent(in) :: kenergy, enstophy, dissipation
-    if (.not. fsopen) then
-      open(newunit=fsfile, file='flowstate.dat')
-      fsopen = .true.
-      write(fsfile, '(5(1X,A20))') 'nstep', 'time', 'kenergy', 'enstophy', 'dissipation'
-    end if
-    write(fsfile, '(1X,I20,4(1X,E20.13E2))') nstep, time, kenergy, enstophy, dissipation
-  end subroutine statout
 
   subroutine solver_end
     if (c_associated(ctx)) call hipcheck(astr_hip_destroy(ctx), 'solver_end')
     ctx = c_null_ptr
     if (statopen) close(statfile)
     statopen = .false.
-    if (fsopen) then
-      close(fsfile)
-      print *, ' << flowstate.dat'
-    end if
-    fsopen = .false.
   end subroutine solver_end
 
 end module astr_hip_tgv

@@ -30,20 +30,31 @@ extern "C" {
 #endif
 
 /* semantics mode: ASTR_SEM_MINIAPP = the stage order, halo refresh and
- * constants of miniapps/tgv_solver_3d/tgvsolver.F90 (rk3 :994-1082). */
+ * constants of miniapps/tgv_solver_3d/tgvsolver.F90 (rk3 :994-1082).
+ * ASTR_SEM_MAIN = the main solver's (astr run) single-rank periodic stage,
+ * time_integration_rk (src/mainloop.F90:396-482): filterq at the start of
+ * every stage, qswap (duplicate-plane averages and the primitives of the
+ * boundary planes, src/parallel.F90:4876-4920, 5018-5060, 5159-5200),
+ * gradients and fluxes from the primitives the last updatefvar stored
+ * (src/comsolver.F90:244-497, src/solver.F90:2173-2873), fluxes carrying
+ * the metrics J and dxi, the RK3 update (q J-weighted), then updatefvar.
+ * The metrics are those of a uniform Cartesian grid (astr_hip_set_metrics).
+ * Batches take ASTR_SEM_MINIAPP cases only. */
 #define ASTR_SEM_MINIAPP 0
+#define ASTR_SEM_MAIN 1
 
 /* Solver set-up.  Replaces program_init / solver_init / refcal constants
  * (tgvsolver.F90:152-261; src/solver.F90:104-128). */
 typedef struct astr_hip_config {
   int im, jm, km;          /* computed lattice 0..im, 0..jm, 0..km; km = 0: 2-D */
-  int semantics;           /* ASTR_SEM_MINIAPP */
+  int semantics;           /* ASTR_SEM_MINIAPP or ASTR_SEM_MAIN */
   double alfa_filter;      /* compact filter alpha (0.49); filter.F90:79-80 */
   double beta_halo;        /* halo-row beta (1.11); filter.F90:57-62 */
   double beta_bound;       /* boundary-row beta (1.09; ntype 1,2,4 only) */
   double gamma, mach, reynolds, prandtl;
   double ref_tem;          /* reference temperature (K) */
-  double sutherland;       /* Sutherland constant (K): 110.4 in tgvsolver.F90:133 */
+  double sutherland;       /* Sutherland constant (K): 110.4 in tgvsolver.F90:133,
+                              110.3 in the main solver (src/solver.F90:122) */
   double lx, ly, lz;       /* domain lengths; dx = lx/im (tgvsolver.F90:199-201) */
   double deltat;           /* time step */
 } astr_hip_config;
@@ -91,6 +102,26 @@ int astr_hip_download(astr_hip_ctx *ctx, double *q, double *prims, int hm);
  * rho(-hm:im+hm,-hm:jm+hm,-hm:km+hm), vel(...,1:3), prs, tmp (each
  * nullable), periodic halos of depth hm. */
 int astr_hip_updatefvar(astr_hip_ctx *ctx, double *rho, double *vel, double *prs, double *tmp, int hm);
+
+/* ASTR_SEM_MAIN: set the stored primitives (the reference's module arrays
+ * rho, vel(...,1:3), prs, tmp, src/commarray.F90:273-285) from host arrays
+ * in the layout of astr_hip_updatefvar, e.g. after a flow initialisation
+ * that sets them and q = fvar2q of them (flowinit + updateq,
+ * src/initialisation.F90:60-140).  Halo cells are not read. */
+int astr_hip_set_fvar(astr_hip_ctx *ctx, const double *rho, const double *vel, const double *prs,
+                      const double *tmp, int hm);
+
+/* writechkpt / readcheckpoint (src/readwrite.F90:2390-2459, 1381-1531): the
+ * reference writes ro, u1, u2, u3, p, t, nstep and time to HDF5 and rebuilds
+ * q from them at restart; here one raw little-endian FP64 file with a fixed
+ * 64-byte header ("ASTRCHK1", int32 header bytes, im, jm, km, semantics,
+ * nfields = 11, int64 nstep, double time) holds ro, u1, u2, u3, p, t (the
+ * stored primitives under ASTR_SEM_MAIN) and q1..q5 on the computed points,
+ * i fastest, so a resumed run continues bit for bit.  read checks the grid
+ * size and the semantics mode; nstep and time (nullable) receive the
+ * header's values. */
+int astr_hip_write_checkpoint(astr_hip_ctx *ctx, const char *path, long long nstep, double time);
+int astr_hip_read_checkpoint(astr_hip_ctx *ctx, const char *path, long long *nstep, double *time);
 
 /* filterq: tgvsolver.F90:791-883 (x, y, z sweeps of the compact filter) */
 int astr_hip_filterq(astr_hip_ctx *ctx);

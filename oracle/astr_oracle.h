@@ -78,6 +78,30 @@ void or_rk3_steps(or_state *s, int nsteps, double *ke, double *ens);
 void or_diss_rate(or_state *s, double *diss);
 void or_rk3_steps3(or_state *s, int nsteps, double *ke, double *ens, double *diss);
 
+/* ---- main solver's stage (astr_oracle_main.c): arrays with halos of depth
+ *      OR_HM, as the reference keeps them ---- */
+typedef struct or_main or_main;
+or_main *or_main_create(const or_cfg *cfg);
+void or_main_destroy(or_main *s);
+long or_main_npoints(const or_main *s);
+long or_main_nhalo(const or_main *s);
+double *or_main_qh(or_main *s);      /* 5 x halo box, i fastest */
+double *or_main_primsh(or_main *s);  /* 6 x halo box: rho, u, v, w, p, T */
+double *or_main_qrhs(or_main *s);    /* 5 N */
+double *or_main_dvel(or_main *s);    /* 9 N, component m, direction n at m + 3n */
+double *or_main_dtmp(or_main *s);    /* 3 N */
+void or_main_set_metrics(or_main *s, const double *jac, const double *dxi);
+void or_main_set_q(or_main *s, const double *q);
+void or_main_set_q_only(or_main *s, const double *q);
+void or_main_get(or_main *s, double *q, double *prims);
+void or_main_updatefvar(or_main *s);
+void or_main_filterq(or_main *s);
+void or_main_qswap(or_main *s);
+void or_main_gradcal(or_main *s);
+void or_main_rhscal(or_main *s);
+void or_main_stats(or_main *s, double *ke, double *ens, double *diss);
+void or_main_rk3_steps(or_main *s, int nsteps, double *ke, double *ens, double *diss);
+
 /* counter-based noise shared with the device path (not a reference routine) */
 double or_gauss(uint64_t seed, uint32_t i, uint32_t j, uint32_t k, uint32_t c);
 

@@ -77,6 +77,24 @@ def lib():
         L.or_deriv_rhs.argtypes = [C.c_int, _dp, _dp]
         L.or_filter_rhs.argtypes = [C.c_int, _dp, _dp, _dp, _dp]
         L.or_filter_coefs.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp]
+        L.or_main_create.restype = C.c_void_p
+        L.or_main_create.argtypes = [C.POINTER(OrCfg)]
+        L.or_main_destroy.argtypes = [C.c_void_p]
+        for f in ("or_main_npoints", "or_main_nhalo"):
+            getattr(L, f).restype = C.c_long
+            getattr(L, f).argtypes = [C.c_void_p]
+        for f in ("or_main_qh", "or_main_primsh", "or_main_qrhs", "or_main_dvel", "or_main_dtmp"):
+            getattr(L, f).restype = _dp
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.or_main_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_main_set_q.argtypes = [C.c_void_p, _dp]
+        L.or_main_set_q_only.argtypes = [C.c_void_p, _dp]
+        L.or_main_get.argtypes = [C.c_void_p, _dp, _dp]
+        for f in ("or_main_updatefvar", "or_main_filterq", "or_main_qswap", "or_main_gradcal",
+                  "or_main_rhscal"):
+            getattr(L, f).argtypes = [C.c_void_p]
+        L.or_main_stats.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.or_main_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -110,8 +128,12 @@ class OracleState:
             self.h = None
 
     def _view(self, fn, nf):
+        """A numpy view of a C array that keeps this state alive while the
+        view lives (the C arrays are freed with the state)."""
         ptr = getattr(lib(), fn)(self.h)
-        return np.ctypeslib.as_array(ptr, shape=(nf * self.N,)).reshape((nf,) + self.shape)
+        buf = (C.c_double * (nf * self.N)).from_address(C.addressof(ptr.contents))
+        buf._owner = self
+        return np.frombuffer(buf, dtype=np.float64).reshape((nf,) + self.shape)
 
     @property
     def q(self):
@@ -166,6 +188,124 @@ class OracleState:
         di = np.zeros(nsteps) if dissipation else None
         lib().or_rk3_steps3(self.h, nsteps, _p(ke), _p(en), _p(di))
         return (ke, en, di) if dissipation else (ke, en)
+
+
+def uniform_metrics(cfg: OrCfg, jac=None, dxi_diag=None):
+    """Metrics of a uniform Cartesian grid as (jac (N), dxi (9, N)) with
+    dxi[d + 3 n] = d xi_d / d x_n: jac = dx dy dz (dx dy in 2-D) and
+    dxi(d,d) = 1/dx_d unless given."""
+    N = (cfg.im + 1) * (cfg.jm + 1) * (cfg.km + 1)
+    if dxi_diag is None:
+        dxi_diag = (1.0 / cfg.dx, 1.0 / cfg.dy, 1.0 / cfg.dz if cfg.km > 0 else 0.0)
+    if jac is None:
+        jac = cfg.dx * cfg.dy * (cfg.dz if cfg.km > 0 else 1.0)
+    dxi = np.zeros((9, N))
+    for d in range(3 if cfg.km > 0 else 2):
+        dxi[d + 3 * d] = dxi_diag[d]
+    return np.full(N, float(jac)), dxi
+
+
+class MainState:
+    """One field state of the main solver's restatement (astr_oracle_main.c):
+    q and the stored primitives with halos, the metrics, qrhs, dvel, dtmp."""
+
+    def __init__(self, cfg: OrCfg, jac=None, dxi=None):
+        self.cfg = cfg
+        self.h = lib().or_main_create(C.byref(cfg))
+        self.N = lib().or_main_npoints(self.h)
+        self.NH = lib().or_main_nhalo(self.h)
+        self.shape = (cfg.km + 1, cfg.jm + 1, cfg.im + 1)
+        hk = HM if cfg.km > 0 else 0
+        self.hshape = (cfg.km + 1 + 2 * hk, cfg.jm + 1 + 2 * HM, cfg.im + 1 + 2 * HM)
+        if jac is None:
+            jac, dxi = uniform_metrics(cfg)
+        self.set_metrics(jac, dxi)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().or_main_destroy(self.h)
+            self.h = None
+
+    def set_metrics(self, jac, dxi):
+        """jac (N) and dxi (9, N), dxi[d + 3 n] = d xi_d / d x_n."""
+        self._jac = np.ascontiguousarray(np.reshape(jac, (self.N,)), dtype=np.float64)
+        self._dxi = np.ascontiguousarray(np.reshape(dxi, (9, self.N)), dtype=np.float64)
+        lib().or_main_set_metrics(self.h, _p(self._jac), _p(self._dxi))
+
+    def _view(self, fn, nf, shape):
+        """A numpy view of a C array that keeps this state alive while the
+        view lives (the C arrays are freed with the state)."""
+        ptr = getattr(lib(), fn)(self.h)
+        size = int(np.prod(shape))
+        buf = (C.c_double * (nf * size)).from_address(C.addressof(ptr.contents))
+        buf._owner = self
+        return np.frombuffer(buf, dtype=np.float64).reshape((nf,) + shape)
+
+    @property
+    def qh(self):
+        """q with halos, (5,) + hshape; index [m, k+hk, j+hm, i+hm]."""
+        return self._view("or_main_qh", 5, self.hshape)
+
+    @property
+    def primsh(self):
+        """stored rho, u, v, w, p, T with halos, (6,) + hshape."""
+        return self._view("or_main_primsh", 6, self.hshape)
+
+    @property
+    def qrhs(self):
+        return self._view("or_main_qrhs", 5, self.shape)
+
+    @property
+    def dvel(self):
+        """dvel[m + 3*n] = d u_m / d x_n."""
+        return self._view("or_main_dvel", 9, self.shape)
+
+    @property
+    def dtmp(self):
+        return self._view("or_main_dtmp", 3, self.shape)
+
+    def set_q(self, q):
+        """q on the computed points, then updatefvar."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        lib().or_main_set_q(self.h, _p(q))
+
+    def set_q_only(self, q):
+        """q on the computed points; the stored primitives are kept."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        lib().or_main_set_q_only(self.h, _p(q))
+
+    def get(self):
+        """(q (5,)+shape, stored primitives (6,)+shape)."""
+        q = np.empty((5,) + self.shape)
+        pr = np.empty((6,) + self.shape)
+        lib().or_main_get(self.h, _p(q), _p(pr))
+        return q, pr
+
+    def updatefvar(self):
+        lib().or_main_updatefvar(self.h)
+
+    def filterq(self):
+        lib().or_main_filterq(self.h)
+
+    def qswap(self):
+        lib().or_main_qswap(self.h)
+
+    def gradcal(self):
+        lib().or_main_gradcal(self.h)
+
+    def rhscal(self):
+        lib().or_main_rhscal(self.h)
+
+    def stats(self):
+        a, b, c = np.zeros(1), np.zeros(1), np.zeros(1)
+        lib().or_main_stats(self.h, _p(a), _p(b), _p(c))
+        return a[0], b[0], c[0]
+
+    def rk3(self, nsteps):
+        """nsteps main-path RK3 steps; (ke, ens, diss) of each step's first stage."""
+        ke, en, di = np.zeros(nsteps), np.zeros(nsteps), np.zeros(nsteps)
+        lib().or_main_rk3_steps(self.h, nsteps, _p(ke), _p(en), _p(di))
+        return ke, en, di
 
 
 def qswap(q):

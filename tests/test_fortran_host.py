@@ -11,6 +11,7 @@ import pytest
 import astr_amd as A
 from astr_amd import chem as AC
 from oracle import chem as OC
+from oracle import oracle as O
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FDIR = os.path.join(ROOT, "astr_amd", "fortran")
@@ -97,33 +98,44 @@ def test_chemcells_hip_matches_oracle(tmp_path):
 DATIN = os.path.join(GOLD, "datin_tgv")
 
 
-def _datin_run(tmp_path, text_edit=None):
-    """tgvsolver_hip --datin on the reference's Taylor-Green example
-    (examples/Taylor_Green_Vortex/datin, kept as a fixture), optionally
-    with one line of the input replaced."""
-    inp = os.path.join(DATIN, "input.tgv")
-    if text_edit:
-        lines = open(inp).read().split("\n")
-        old, new = text_edit
-        lines = [new if ln.strip() == old else ln for ln in lines]
-        inp = str(tmp_path / "input.edit")
-        open(inp, "w").write("\n".join(lines))
-    return subprocess.run([os.path.join(FDIR, "tgvsolver_hip"), "--datin", inp,
-                           os.path.join(DATIN, "controller")],
-                          cwd=tmp_path, capture_output=True, text=True, timeout=300)
+def _edit(src, edits):
+    """The text of a datin file with whole lines replaced (old -> new)."""
+    lines = open(src).read().split("\n")
+    for old, new in edits:
+        hit = [i for i, ln in enumerate(lines) if ln.strip() == old]
+        assert hit, old
+        for i in hit:
+            lines[i] = new
+    return "\n".join(lines)
+
+
+def _astr_run(tmp_path, datin=DATIN, inp="input.tgv", edits=(), cont_edits=(), args=None):
+    """astr_hip run on a datin folder copied into tmp_path/datin, with lines
+    of the input and the controller replaced."""
+    d = tmp_path / "datin"
+    d.mkdir(exist_ok=True)
+    (d / "input.dat").write_text(_edit(os.path.join(datin, inp), edits))
+    (d / "controller").write_text(_edit(os.path.join(datin, "controller"), cont_edits))
+    return subprocess.run([os.path.join(FDIR, "astr_hip")] + (args or ["run"]),
+                          cwd=tmp_path, capture_output=True, text=True, timeout=600)
 
 
 def test_datin_input_is_read_as_the_reference_reads_it(tmp_path):
     """readinput / readcont (src/readwrite.F90:466-769) on the reference's TGV
     example: grid 128^3, Re 1600, Ma 0.1, T_ref 273.15, alpha_f 0.49,
     maxstep 10, deltat 1e-3 (what follows the parse needs a GPU)."""
-    r = _datin_run(tmp_path)
+    r = _astr_run(tmp_path)
     out = r.stdout
     assert "dimension set: 128 x 128 x 128" in out, out[-2000:]
     vals = out.split("ref_tem, reynolds, mach, alfa_filter:")[1].split("\n")[0].split()
     assert [float(v) for v in vals] == [273.15, 1600.0, 0.1, 0.49]
-    assert "maxstep 10, deltat" in out
-    assert float(out.split("maxstep 10, deltat")[1].split("\n")[0]) == 1e-3
+    assert "maxstep 10, feqchkpt 10, deltat" in out
+    assert float(out.split("maxstep 10, feqchkpt 10, deltat")[1].split("\n")[0]) == 1e-3
+    # the other two examples this build runs parse too
+    r2 = _astr_run(tmp_path, os.path.join(GOLD, "datin_2dvort"), "input.2dvort")
+    assert "dimension set: 400 x 200 x 0" in r2.stdout and "diffterm F" in r2.stdout
+    r3 = _astr_run(tmp_path, os.path.join(GOLD, "datin_tgv2d"), "input.tgv2d")
+    assert "dimension set: 128 x 128 x 0" in r3.stdout
 
 
 @pytest.mark.parametrize("edit", [("tgv", "chl"), ("643c, 643c, rk3, rk3", "m7c, 643c, rk3, rk3"),
@@ -131,36 +143,199 @@ def test_datin_input_is_read_as_the_reference_reads_it(tmp_path):
 def test_datin_unsupported_options_stop(tmp_path, edit):
     """Options the device solver does not run stop with a message and exit
     status 1 before any device work."""
-    r = _datin_run(tmp_path, edit)
+    r = _astr_run(tmp_path, edits=[edit])
     assert r.returncode == 1, r.stdout[-1000:] + r.stderr[-1000:]
     assert "input not supported by the device solver" in r.stdout
 
 
+def _host_init(n, km=None, lx=2 * np.pi):
+    """gridcube + tgvini + updateq (gridgeneration.F90:233-267,
+    initialisation.F90:621-691, fludyna.F90:254-360) in numpy, as the Fortran
+    host computes them: (x, q, rho, vel, prs, tmp) in (k, j, i) order."""
+    km = n if km is None else km
+    i = np.arange(n + 1)
+    k = np.arange(km + 1)
+    xx = lx / n * i
+    zz = lx / km * k if km > 0 else np.zeros(1)
+    Z, Y, X = np.meshgrid(zz, xx, xx, indexing="ij")
+    rho = np.ones_like(X)
+    u = 1.0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -1.0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros_like(X)
+    const2 = 1.4 * 0.1 ** 2
+    const1 = 1.0 / (1.4 * 0.4 * 0.1 ** 2)
+    p = 1.0 / const2 + rho / 16.0 * 1.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    t = p / rho * const2
+    q = np.array([rho, rho * u, rho * v, rho * w, rho * (t * const1 + 0.5 * (u * u + v * v + w * w))])
+    return np.array([X, Y, Z]), q, rho, np.array([u, v, w]), p, t
+
+
+def _host_init_2d(im, jm):
+    """gridcube(2 pi, 2 pi, 2 pi) with ka = 0 + tgvini + updateq in numpy."""
+    xx = 2 * np.pi / im * np.arange(im + 1)
+    yy = 2 * np.pi / jm * np.arange(jm + 1)
+    Y, X = np.meshgrid(yy, xx, indexing="ij")
+    X, Y = X[None], Y[None]
+    Z = np.zeros_like(X)
+    rho = np.ones_like(X)
+    u = 1.0 * np.sin(X) * np.cos(Y) * np.cos(Z)
+    v = -1.0 * np.cos(X) * np.sin(Y) * np.cos(Z)
+    w = np.zeros_like(X)
+    const2 = 1.4 * 0.1 ** 2
+    const1 = 1.0 / (1.4 * 0.4 * 0.1 ** 2)
+    p = 1.0 / const2 + rho / 16.0 * 1.0 * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0)
+    t = p / rho * const2
+    q = np.array([rho, rho * u, rho * v, rho * w, rho * (t * const1 + 0.5 * (u * u + v * v + w * w))])
+    return np.array([X, Y, Z]), q, rho, np.array([u, v, w]), p, t
+
+
 @pytest.mark.gpu
-def test_datin_run_reproduces_state_ref_128(tmp_path):
-    """The reference's own TGV input (128^3, Re 1600, Ma 0.1, dt 1e-3, 10 steps
-    from its controller) run through the Fortran host on the GPU reproduces the
-    first 10 lines of the reference-held state.ref_128 to 1e-9."""
-    r = _datin_run(tmp_path)
-    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
-    got = np.loadtxt(tmp_path / "state.dat")
-    ref = np.loadtxt(os.path.join(GOLD, "state_ref_128.txt"))[:10]
-    assert got.shape == ref.shape
-    assert np.array_equal(got[:, 0], ref[:, 0])
-    for c in (2, 3):
-        assert np.abs(got[:, c] - ref[:, c]).max() / np.abs(ref[:, c]).max() < 1e-9
-    # flowstate.dat in the main path's layout (statistic.F90:777-859): the same
-    # steps, the same KE and enstrophy to the 14 digits it prints, and the
-    # dissipation rate of the same run through the C ABI
-    lines = open(tmp_path / "flowstate.dat").read().split("\n")
-    assert lines[0].split() == ["nstep", "time", "kenergy", "enstophy", "dissipation"]
-    assert len(lines[0]) == 5 * 21
+def test_astr_hip_run_tgv_matches_oracle(tmp_path):
+    """astr_hip run on the reference's TGV input cut to 32^3 and 6 steps with
+    a checkpoint every 3: flowstate.dat (steps 1..6, the main path's
+    statistics) and the checkpoint at step 7 against the same run through
+    the C ABI from the same initial field (computed here in numpy, whose sin
+    and cos may differ from the host's libm in the last bit) and against the
+    main-path oracle."""
+    r = _astr_run(tmp_path, edits=[("128,128,128", "32,32,32")],
+                  cont_edits=[("10,      10,     100,      50,      1,    50",
+                               "6,      3,     100,      50,      1,    50")])
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
     fs = np.loadtxt(tmp_path / "flowstate.dat", skiprows=1)
-    assert fs.shape == (10, 5)
-    assert np.array_equal(fs[:, 0], got[:, 0])
-    for c in (2, 3):
-        assert np.abs(fs[:, c] / got[:, c] - 1).max() < 1e-12
-    f = A.Flow(A.tgv_config(128, deltat=1e-3))
-    f.init_tgv(0.0, 0)
-    ke, en, di = f.rk3(10, dissipation=True)
-    assert np.abs(fs[:, 4] / di - 1).max() < 1e-12
+    assert fs.shape == (6, 5) and list(fs[:, 0]) == [1, 2, 3, 4, 5, 6]
+    # the same run through the C ABI
+    n = 32
+    X, q, rho, vel, prs, tmp = _host_init(n)
+    f = A.Flow(A.main_config(n, deltat=1e-3))
+    jac, dxi = f.gridgeom(X)
+    f.set_metrics(jac, dxi)
+    f.upload(q)
+    f.set_fvar(rho, vel, prs, tmp)
+    ke, en, di = f.rk3(7, dissipation=True)
+    for c, v in ((2, ke), (3, en), (4, di)):
+        assert np.abs(fs[:, c] / v[1:] - 1).max() < 1e-12
+    qc, pc = f.download(prims=True)
+    g = A.Flow(A.main_config(n, deltat=1e-3))
+    g.set_metrics(jac, dxi)
+    nstep, time = g.readcheckpoint(str(tmp_path / "checkpoint" / "flowfield.bin"))
+    assert nstep == 7 and abs(time - 7e-3) < 1e-15
+    qg, pg = g.download(prims=True)
+    assert np.abs(qg - qc).max() / np.abs(qc).max() < 1e-13
+    assert np.abs(pg - pc).max() / np.abs(pc).max() < 1e-13
+    # the oracle on the same input (the device's metrics means)
+    ocfg = O.tgv_cfg(n, dt=1e-3, suth_dim=110.3)
+    N = (n + 1) ** 3
+    oj = np.full(N, float(np.mean(jac, dtype=np.longdouble)))
+    od = np.zeros((9, N))
+    for a in range(3):
+        od[a + 3 * a] = float(np.mean(dxi[a, a], dtype=np.longdouble))
+    s = O.MainState(ocfg, oj, od)
+    s.set_q(q)
+    ke0, en0, di0 = s.rk3(7)
+    for v, v0 in ((ke, ke0), (en, en0), (di, di0)):
+        assert np.abs(v / v0 - 1).max() < 1e-9
+    qo, _ = s.get()
+    assert np.abs(qc - qo).max() / np.abs(qo).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_astr_hip_restart_continues_bit_for_bit(tmp_path):
+    """A run of 6 steps with checkpoints every 3, and a second run restarted
+    from the checkpoint at step 4 (lrestart = t, maxstep 6): the restarted
+    run's last checkpoint equals the unbroken run's bit for bit."""
+    cut = [("10,      10,     100,      50,      1,    50", "6,      3,     100,      50,      1,    50")]
+    a_dir = tmp_path / "a"
+    a_dir.mkdir()
+    r = _astr_run(a_dir, edits=[("128,128,128", "24,24,24")], cont_edits=cut)
+    assert r.returncode == 0, r.stdout[-3000:]
+    full = (a_dir / "checkpoint" / "flowfield.bin").read_bytes()
+    b_dir = tmp_path / "b"
+    b_dir.mkdir()
+    cut3 = [("10,      10,     100,      50,      1,    50", "3,      3,     100,      50,      1,    50")]
+    r = _astr_run(b_dir, edits=[("128,128,128", "24,24,24")], cont_edits=cut3)
+    assert r.returncode == 0, r.stdout[-3000:]
+    first = (b_dir / "checkpoint" / "flowfield.bin").read_bytes()
+    assert first != full
+    r = _astr_run(b_dir, edits=[("128,128,128", "24,24,24"), ("f", "t")], cont_edits=cut)
+    assert r.returncode == 0, r.stdout[-3000:]
+    assert "lrestart T" in r.stdout
+    assert (b_dir / "checkpoint" / "flowfield.bin").read_bytes() == full
+
+
+def _host_init_vort(im, jm, mach=0.3):
+    """gridcube(20, 10, 1) with ka = 0 + vortini + updateq
+    (gridgeneration.F90:233-267, initialisation.F90:480-533) in numpy."""
+    xx = 20.0 / im * np.arange(im + 1)
+    yy = 10.0 / jm * np.arange(jm + 1)
+    Y, X = np.meshgrid(yy, xx, indexing="ij")
+    X, Y = X[None], Y[None]
+    Z = np.zeros_like(X)
+    xc, yc, rvor = 10.0, 5.0, 0.7
+    cvor = 0.1 * rvor
+    const2 = 1.4 * mach ** 2
+    const1 = 1.0 / (1.4 * 0.4 * mach ** 2)
+    radi2 = ((X - xc) ** 2 + (Y - yc) ** 2) / rvor / rvor
+    var1 = cvor / rvor / rvor * np.exp(-0.5 * radi2)
+    rho = np.ones_like(X)
+    u = 1.0 - var1 * (Y - yc)
+    v = 0.0 + var1 * (X - xc)
+    w = np.zeros_like(X)
+    p = 1.0 / const2 - 0.5 * rho * cvor ** 2 / rvor ** 2 * np.exp(-radi2)
+    t = p / rho * const2
+    q = np.array([rho, rho * u, rho * v, rho * w, rho * (t * const1 + 0.5 * (u * u + v * v + w * w))])
+    return np.array([X, Y, Z]), q, rho, np.array([u, v, w]), p, t
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["tgv2d", "2dvort"])
+def test_astr_hip_runs_2d_examples(tmp_path, case):
+    """The reference's 2-D examples (Taylor_Green_Vortex_2D; Vortex_Transport,
+    inviscid with diffterm = f) cut to 64 x 48 / 80 x 40 and 5 steps with a
+    checkpoint after the last: the host's checkpoint equals the same run
+    through the C ABI from the same initial field computed in numpy (to
+    1e-13: numpy's sin, cos and exp may differ from libm's in the last bit),
+    and the main-path oracle to 1e-12 (1/Re = 0 for the inviscid case)."""
+    if case == "tgv2d":
+        datin, inp, grid, new = os.path.join(GOLD, "datin_tgv2d"), "input.tgv2d", "128,128,0", "64,48,0"
+        cont = [("10,      10,     100,      50,      1,    50", "4,      4,     100,      50,      1,    50")]
+        dt, mach, re = 1e-3, 0.1, 1600.0
+    else:
+        datin, inp, grid, new = os.path.join(GOLD, "datin_2dvort"), "input.2dvort", "400,200,0", "80,40,0"
+        cont = [("100,     100,     100,      50,      1,    50", "4,     4,     100,      50,      1,    50")]
+        dt, mach, re = 5e-3, 0.3, float("inf")
+    r = _astr_run(tmp_path, datin, inp, edits=[(grid, new)], cont_edits=cont)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
+    im, jm = [int(v) for v in new.split(",")[:2]]
+    if case == "tgv2d":
+        X, q, rho, vel, prs, tmp = _host_init_2d(im, jm)
+        lx, ly = 2 * np.pi, 2 * np.pi
+    else:
+        X, q, rho, vel, prs, tmp = _host_init_vort(im, jm)
+        lx, ly = 20.0, 10.0
+    cfg = A.main_config(im, jm, 0, deltat=dt, mach=mach, lx=lx, ly=ly)
+    cfg.reynolds = re
+    f = A.Flow(cfg)
+    jac, dxi = f.gridgeom(X)
+    f.set_metrics(jac, dxi)
+    f.upload(q)
+    f.set_fvar(rho, vel, prs, tmp)
+    f.rk3(5, stats=False)
+    qc, pc = f.download(prims=True)
+    g = A.Flow(cfg)
+    nstep, _ = g.readcheckpoint(str(tmp_path / "checkpoint" / "flowfield.bin"))
+    assert nstep == 5
+    qg, pg = g.download(prims=True)
+    for grp in (slice(0, 1), slice(1, 3), slice(4, 5)):
+        assert np.abs(qg[grp] - qc[grp]).max() / np.abs(qc[grp]).max() < 1e-13
+    ocfg = O.tgv_cfg(im, jm, 0, dt=dt, suth_dim=110.3, mach=mach, reynolds=re, lx=lx, ly=ly)
+    N = (im + 1) * (jm + 1)
+    oj = np.full(N, float(np.mean(jac, dtype=np.longdouble)))
+    od = np.zeros((9, N))
+    for a in range(2):
+        od[a + 3 * a] = float(np.mean(dxi[a, a], dtype=np.longdouble))
+    s = O.MainState(ocfg, oj, od)
+    s.set_q(q)
+    s.rk3(5)
+    qo, _ = s.get()
+    for grp in (slice(0, 1), slice(1, 3), slice(4, 5)):
+        assert np.abs(qc[grp] - qo[grp]).max() / np.abs(qo[grp]).max() < 1e-12
