@@ -56,7 +56,7 @@ EXPORTS = [
     "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt", "astr_hip_updatefvar",
     "astr_hip_set_metrics", "astr_hip_qswap",
     "astr_hip_get_last_split", "astr_hip_rk3_steps_stats_noalloc", "astr_hip_set_fvar",
-    "astr_hip_write_checkpoint", "astr_hip_read_checkpoint",
+    "astr_hip_write_checkpoint", "astr_hip_read_checkpoint", "astr_hip_set_boundary",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
           5: (8, 64, 5), 6: (4, 32, 9), 7: (2, 64, 5),
@@ -99,6 +99,7 @@ def lib():
         L.astr_hip_read_checkpoint.argtypes = [C.c_void_p, C.c_char_p, C.POINTER(C.c_longlong),
                                                C.POINTER(C.c_double)]
         L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
+        L.astr_hip_set_boundary.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
                                           C.c_double, C.c_double]
@@ -150,8 +151,10 @@ def main_config(im, jm=None, km=None, deltat=1e-3, reynolds=1600.0, mach=0.1, re
                 lx=2 * math.pi, ly=None, lz=None):
     """The main solver's stage (time_integration_rk, src/mainloop.F90:396-482)
     on a periodic box: Sutherland S = 110.3 K (src/solver.F90:122)."""
-    return tgv_config(im, jm, km, deltat=deltat, reynolds=reynolds, mach=mach, sutherland=110.3,
-                      ref_tem=ref_tem, lx=lx, ly=ly, lz=lz, semantics=SEM_MAIN)
+    cfg = tgv_config(im, jm, km, deltat=deltat, reynolds=reynolds, mach=mach, sutherland=110.3,
+                     ref_tem=ref_tem, lx=lx, ly=ly, lz=lz, semantics=SEM_MAIN)
+    cfg.beta_bound = 0.98   # filter_coefficient_cal's beter_bouond (src/comsolver.F90:121)
+    return cfg
 
 
 class Flow:
@@ -292,9 +295,14 @@ class Flow:
             _chk(lib().astr_hip_gridgeom_nt(self.h, nt, _p(x), _p(jac), _p(buf)))
         return jac, buf.transpose((1, 0, 2, 3, 4))
 
+    def set_boundary(self, npdc):
+        """Main path: the boundary type per direction (3 periodic, 4 physical
+        at both ends)."""
+        _chk(lib().astr_hip_set_boundary(self.h, (C.c_int * 3)(*[int(v) for v in npdc])))
+
     def set_metrics(self, jac, dxi):
-        """Use the metrics (as gridgeom returns them); only the uniform
-        Cartesian metrics the passes support are accepted."""
+        """Use the metrics (as gridgeom returns them): those of a Cartesian
+        grid, uniform (both semantics) or stretched (main path)."""
         jac = np.ascontiguousarray(jac, dtype=np.float64)
         buf = np.ascontiguousarray(np.asarray(dxi, dtype=np.float64).transpose((1, 0, 2, 3, 4)))
         _chk(lib().astr_hip_set_metrics(self.h, _p(jac), _p(buf)))

@@ -478,6 +478,8 @@ struct astr_hip_ctx {
   // outside qswap)
   double* halo = nullptr;
   bool halo_valid = false;
+  // main path on a grid that is not uniform: J and dxi(d,n) per point (10 fields)
+  double* met = nullptr;
 };
 
 namespace {
@@ -978,6 +980,49 @@ struct StatsBufs {
   double* out;
 };
 
+// The coefficient table of a context: the filter's halo rows and the rows of
+// a physical boundary (kBndCoefs), then per direction the derivative's and
+// the filter's LU factors, of the periodic form or (a.bnd[d] != 0) of the
+// closed form; (re)allocated on the device and pointed at by a.fc, dac, fac
+int build_tables(astr_hip_ctx* c) {
+  FlowArgs& a = c->a;
+  const astr_hip_config& cfg = c->cfg;
+  FilterCoef fc;
+  std::vector<double> all(kBndCoefs);
+  filter_coefs(cfg.alfa_filter, cfg.beta_halo, fc, all.data());
+  filter_coefs_b(cfg.alfa_filter, cfg.beta_bound, all.data() + 33);
+  size_t off[6] = {0, 0, 0, 0, 0, 0};
+  for (int d = 0; d < a.ndir; d++) {
+    std::vector<double> t;
+    if (a.bnd[d]) deriv_tables_nt(a.n[d], a.bnd[d], t);
+    else deriv_tables(a.n[d], t);
+    off[d] = all.size();
+    all.insert(all.end(), t.begin(), t.end());
+    if (a.bnd[d]) filter_tables_nt(a.n[d], a.bnd[d], cfg.alfa_filter, cfg.beta_halo, t);
+    else filter_tables(a.n[d], cfg.alfa_filter, cfg.beta_halo, t);
+    off[3 + d] = all.size();
+    all.insert(all.end(), t.begin(), t.end());
+  }
+  double* tab = nullptr;
+  if (hipMalloc(&tab, all.size() * sizeof(double)) != hipSuccess) return fail("hipMalloc of tables failed");
+  if (hipMemcpy(tab, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(tab);
+    return fail("copy of the tables failed");
+  }
+  if (c->coef) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->coef);
+  }
+  c->coef = tab;
+  std::memcpy(a.fc.c10, fc.c10, sizeof(fc.c10));
+  a.fc.ch = c->coef;
+  for (int d = 0; d < a.ndir; d++) {
+    a.dac[d] = c->coef + off[d];
+    a.fac[d] = c->coef + off[3 + d];
+  }
+  return 0;
+}
+
 // updatefvar into the stored primitives (main path)
 int updatefvar_dev(astr_hip_ctx* c) {
   hipLaunchKernelGGL(k_updatefvar, pt_grid(c->a), dim3(64), 0, c->stream, c->a);
@@ -1007,7 +1052,8 @@ int qswap_main(astr_hip_ctx* c, bool average) {
   hipLaunchKernelGGL(k_halo_src, dim3((unsigned)((nx + 255) / 256)), dim3(256), 0, c->stream, a, 0,
                      const_cast<double*>(a.hx), a.hxs);
   HIP_OK(hipGetLastError());
-  if (average && avg(0)) return -1;
+  // only periodic directions are swapped (parallel.F90:4876, 5018, 5159)
+  if (average && !a.bnd[0] && avg(0)) return -1;
   if (a.ndir == 3) {
     const long long ny = hy_lines(a.n, a.ndir) * kHaloSrcRows;
     hipLaunchKernelGGL(k_halo_src, dim3((unsigned)((ny + 255) / 256)), dim3(256), 0, c->stream, a, 1,
@@ -1016,8 +1062,9 @@ int qswap_main(astr_hip_ctx* c, bool average) {
   }
   if (average) {
     for (int d = 1; d < a.ndir; d++)
-      if (avg(d)) return -1;
+      if (!a.bnd[d] && avg(d)) return -1;
     for (int d = 0; d < a.ndir; d++) {
+      if (a.bnd[d]) continue;
       const int d1 = d == 0 ? 1 : 0, d2 = d == 2 ? 1 : 2;
       const long long cnt = 2LL * (a.n[d1] + 1) * (a.n[d2] + 1);
       hipLaunchKernelGGL(k_bprims, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, a, d);
@@ -1180,27 +1227,12 @@ int astr_hip_create(const astr_hip_config* cfg, astr_hip_ctx** out) {
     a.hx = c->halo;
     a.hy = c->halo + 5 * a.hxs;
   }
-  // halo-row filter coefficients, then the LU tables
-  std::vector<double> all(h33, h33 + 33);
-  size_t off[6];
-  for (int d = 0; d < a.ndir; d++) {
-    std::vector<double> t;
-    deriv_tables(a.n[d], t);
-    off[d] = all.size();
-    all.insert(all.end(), t.begin(), t.end());
-    filter_tables(a.n[d], cfg->alfa_filter, cfg->beta_halo, t);
-    off[3 + d] = all.size();
-    all.insert(all.end(), t.begin(), t.end());
-  }
-  if (hipMalloc(&c->coef, all.size() * sizeof(double)) != hipSuccess) {
-    (void)hipFree(c->base); if (c->halo) (void)hipFree(c->halo); delete c;
-    return fail("hipMalloc of tables failed");
-  }
-  (void)hipMemcpy(c->coef, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice);
-  a.fc.ch = c->coef;
-  for (int d = 0; d < a.ndir; d++) {
-    a.dac[d] = c->coef + off[d];
-    a.fac[d] = c->coef + off[3 + d];
+  (void)h33;
+  if (build_tables(c)) {
+    (void)hipFree(c->base); if (c->halo) (void)hipFree(c->halo);
+    const std::string e = g_err;
+    delete c;
+    return fail(e);
   }
   // per-tile stats slots for the smallest tile any shape uses
   // (then the dissipation rate's block partials)
@@ -1229,6 +1261,7 @@ int astr_hip_destroy(astr_hip_ctx* c) {
   (void)hipFree(c->stats_part);
   if (c->stats_out) (void)hipFree(c->stats_out);
   if (c->halo) (void)hipFree(c->halo);
+  if (c->met) (void)hipFree(c->met);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1827,7 +1860,7 @@ int astr_hip_qswap(astr_hip_ctx* c) {
 int astr_hip_set_metrics(astr_hip_ctx* c, const double* jacob, const double* dxi) {
   if (check_ctx(c)) return -1;
   if (!jacob || !dxi) return fail("null metrics");
-  const FlowArgs& a = c->a;
+  FlowArgs& a = c->a;
   const long long fs = (long long)(a.n[0] + 1) * (a.n[1] + 1) * (a.n[2] + 1);
   const int nd = a.ndir;
   double mean[3] = {0.0, 0.0, 0.0};
@@ -1838,29 +1871,81 @@ int astr_hip_set_metrics(astr_hip_ctx* c, const double* jacob, const double* dxi
     long double s = 0.0L;
     for (long long o = 0; o < fs; o++) s += f[o];
     mean[d] = (double)(s / (long double)fs);
-    if (!(mean[d] > 0.0)) return fail("metrics: d xi/d x must be positive");
   }
-  // this build's passes take 1/dx per direction: accept only metrics that
-  // are diagonal and constant to rounding (a uniform Cartesian grid)
-  const double tol = 1e-12;
-  for (int q = 0; q < 3; q++)
-    for (int p = 0; p < 3; p++) {
-      const double* f = dxi + (p + 3 * q) * fs;
-      const bool diag = p == q && p < nd;
-      for (long long o = 0; o < fs; o++) {
-        const double ref = diag ? mean[p] : 0.0;
-        if (!(std::fabs(f[o] - ref) <= tol * mean[p < nd ? p : 0]))
-          return fail("metrics are not those of a uniform Cartesian grid; this build supports only those");
-      }
-    }
-  double jm = 1.0;
-  for (int d = 0; d < nd; d++) jm *= 1.0 / mean[d];
-  for (long long o = 0; o < fs; o++)
-    if (!(std::fabs(jacob[o] - jm) <= tol * jm)) return fail("metrics: Jacobian inconsistent with d xi/d x");
-  for (int d = 0; d < nd; d++) c->a.inv_d[d] = mean[d];
   long double js = 0.0L;
   for (long long o = 0; o < fs; o++) js += jacob[o];
-  c->a.jac = (double)(js / (long double)fs);
+  const double jmean = (double)(js / (long double)fs);
+  for (long long o = 0; o < fs; o++)
+    if (!(jacob[o] > 0.0)) return fail("metrics: the Jacobian must be positive");
+  // uniform Cartesian: dxi diagonal and constant, J constant, to 1e-12
+  const double tol = 1e-12;
+  bool uniform = true;
+  double scale = 0.0;
+  for (int d = 0; d < nd; d++) scale = std::max(scale, std::fabs(mean[d]));
+  for (int q = 0; q < 3 && uniform; q++)
+    for (int p = 0; p < 3 && uniform; p++) {
+      const double* f = dxi + (p + 3 * q) * fs;
+      const bool diag = p == q && p < nd;
+      for (long long o = 0; o < fs && uniform; o++) {
+        const double ref = diag ? mean[p] : 0.0;
+        if (!(std::fabs(f[o] - ref) <= tol * scale)) uniform = false;
+      }
+    }
+  for (long long o = 0; o < fs && uniform; o++)
+    if (!(std::fabs(jacob[o] - jmean) <= tol * jmean)) uniform = false;
+  if (uniform) {
+    for (int d = 0; d < nd; d++)
+      if (!(mean[d] > 0.0)) return fail("metrics: d xi/d x must be positive");
+    double jm = 1.0;
+    for (int d = 0; d < nd; d++) jm *= 1.0 / mean[d];
+    if (!(std::fabs(jmean - jm) <= tol * jm)) return fail("metrics: Jacobian inconsistent with d xi/d x");
+    for (int d = 0; d < nd; d++) a.inv_d[d] = mean[d];
+    a.jac = jmean;
+    if (c->met) {
+      HIP_OK(hipStreamSynchronize(c->stream));
+      (void)hipFree(c->met);
+      c->met = nullptr;
+    }
+    a.met = nullptr;
+    return 0;
+  }
+  // any other grid: the main path's passes read J and every dxi(d,n) per point
+  if (!a.main_sem)
+    return fail("metrics of a grid that is not uniform Cartesian: the miniapp semantics take a "
+                "uniform grid only (ASTR_SEM_MAIN takes any grid)");
+  std::vector<double> host((size_t)10 * fs, 0.0);
+  std::memcpy(host.data(), jacob, fs * sizeof(double));
+  std::memcpy(host.data() + fs, dxi, 9 * fs * sizeof(double));
+  if (!c->met) {
+    double* p = nullptr;
+    HIP_OK(hipMalloc(&p, (size_t)10 * a.fs * sizeof(double)));
+    c->met = p;
+    HIP_OK(hipMemset(c->met, 0, (size_t)10 * a.fs * sizeof(double)));
+  }
+  if (xfer(c, host.data(), 10, c->met, 0, 1)) return -1;
+  a.jac = jmean;
+  a.met = c->met;
+  return 0;
+}
+
+int astr_hip_set_boundary(astr_hip_ctx* c, const int* npdc) {
+  if (check_ctx(c)) return -1;
+  if (!npdc) return fail("null npdc");
+  FlowArgs& a = c->a;
+  if (!a.main_sem) return fail("physical boundaries are a main-path (ASTR_SEM_MAIN) feature");
+  int bits[3] = {0, 0, 0};
+  for (int d = 0; d < a.ndir; d++) {
+    if (npdc[d] != 3 && npdc[d] != 4)
+      return fail("npdc must be 3 (periodic) or 4 (physical at both ends) on one rank");
+    bits[d] = ntype_bits(npdc[d]);
+  }
+  const FlowArgs keep = a;
+  for (int d = 0; d < 3; d++) a.bnd[d] = bits[d];
+  if (build_tables(c)) {
+    a = keep;
+    return -1;
+  }
+  c->halo_valid = false;
   return 0;
 }
 

@@ -48,6 +48,13 @@ struct FlowArgs {
   const double* hx;     // halo sources of the x-lines on j or k boundary planes
   const double* hy;     // halo sources of the y-lines on k boundary planes
   long long hxs, hys;   // field strides of hx, hy
+  // Main path, physical boundaries and non-uniform grids: the boundary bits
+  // of each direction (bit 0: physical boundary at 0, bit 1: at n; 0 for a
+  // periodic direction) and, for a grid that is not uniform, the metrics per
+  // point as 10 fields: J, then dxi(d,n) at 1 + d + 3n (nullptr: the uniform
+  // grid's jac and inv_d above, dxi(d,n) = 0 for d != n)
+  int bnd[3];
+  const double* met;
 };
 
 // Halo sources kept by qswap for the lines whose halo the reference fills
@@ -141,8 +148,10 @@ __device__ __forceinline__ void stg(double* base, unsigned off, double v) {
 // (X + TL*LT = row -8 of T's first line), so no slot needs a guard.
 // After them sit the filter's halo-row coefficients (3 rows x 11), staged
 // once per tile so that no filter solve waits on a global load.
-constexpr int kHaloCoefs = 33;
-constexpr int kHaloSlot = 40;
+// The table also holds the rows of a physical boundary (boundary_row in
+// lines.hpp): coefh (33), coefb (21), coef6i (4), coef8i (5).
+constexpr int kHaloCoefs = kBndCoefs;
+constexpr int kHaloSlot = 64;
 template <class C>
 constexpr int lds_line_doubles() {
   return (2 * C::TL * C::LT > 2 * C::NT ? 2 * C::TL * C::LT : 2 * C::NT) + kLineOff + 8;
@@ -209,6 +218,21 @@ __device__ __forceinline__ void lds_chunk_coef_f(ChunkCoef<C::R, true>& CF, cons
   const int kl = M - 1 - u0;
   CF.klast = kl >= 0 && kl < C::R ? kl : -1;
   CF.c3last = c3last;
+}
+
+// A line solve of a pass: the periodic form, or (main path) the closed form
+// of a direction with physical ends
+template <class C, bool FILT, bool MAIN>
+__device__ __forceinline__ void solve_pass_line(const double* Tl, double* Xl, int n, int p,
+                                                const ChunkCoef<C::R, FILT>& CC, const FilterCoef& fc,
+                                                int bnd) {
+  if constexpr (MAIN) {
+    if (bnd) {
+      solve_line<C::P, C::R, FILT, true>(Tl, Xl, n, p, CC, fc, bnd);
+      return;
+    }
+  }
+  solve_line<C::P, C::R, FILT>(Tl, Xl, n, p, CC, fc);
 }
 
 // Copy the filter's halo-row coefficients into the tile's LDS; the barrier
@@ -386,15 +410,60 @@ __device__ __forceinline__ void visc_flux(const FlowArgs& a, const double g[3][3
 // dir, J (q_m U + dxi(dir,m) p) with the contravariant velocity
 // U = sum_n dxi(dir,n) u_n, for the metrics of a uniform Cartesian grid
 // (dxi(dir,n) = 0 for n != dir, so those products add exact zeros)
-__device__ __forceinline__ double conv_flux_main(const FlowArgs& a, int m, int dir, double qm,
+__device__ __forceinline__ double conv_flux_main(double jac, double dx, int m, int dir, double qm,
                                                  double q5, double ud, double p) {
 #pragma clang fp contract(off)
-  const double dx = a.inv_d[dir];
   const double uu = dx * ud;
-  if (m == 0) return a.jac * qm * uu;
-  if (m == 4) return a.jac * (q5 + p) * uu;
-  if (m == dir + 1) return a.jac * (qm * uu + dx * p);
-  return a.jac * (qm * uu);
+  if (m == 0) return jac * qm * uu;
+  if (m == 4) return jac * (q5 + p) * uu;
+  if (m == dir + 1) return jac * (qm * uu + dx * p);
+  return jac * (qm * uu);
+}
+
+// Main path: the Jacobian J and dxi(dir,dir) of a point on a uniform grid
+struct Met {
+  double j, d;
+};
+__device__ __forceinline__ Met met_of(const FlowArgs& a, int dir) { return Met{a.jac, a.inv_d[dir]}; }
+
+// Main path on a grid that is not uniform: J and the row dxi(dir, 0..2) of a point
+struct MetRow {
+  double j, r[3];
+};
+__device__ __forceinline__ MetRow met_row(const FlowArgs& a, int dir, unsigned g) {
+  MetRow M;
+  M.j = ldg(a.met, g);
+#pragma unroll
+  for (int nn = 0; nn < 3; nn++) M.r[nn] = ldg(a.met + (1 + dir + 3 * nn) * a.fs, g);
+  return M;
+}
+// the Jacobian of a point, either grid
+__device__ __forceinline__ double jac_of(const FlowArgs& a, unsigned g) {
+  return a.met ? ldg(a.met, g) : a.jac;
+}
+
+// convrsdcal6 (src/solver.F90:2196-2207) with the full metric row:
+// U = dxi(d,1) u + dxi(d,2) v + dxi(d,3) w, F_m = J (q_m U + dxi(d,m) p)
+__device__ __forceinline__ double conv_flux_full(const MetRow& M, int m, double qm, double q5,
+                                                 double u, double v, double w, double p) {
+#pragma clang fp contract(off)
+  const double uu = M.r[0] * u + M.r[1] * v + M.r[2] * w;
+  if (m == 0) return M.j * qm * uu;
+  if (m == 4) return M.j * (q5 + p) * uu;
+  return M.j * (qm * uu + M.r[m - 1] * p);
+}
+
+// diffrsdcal6 (src/solver.F90:2623-2635) with the full metric row: column
+// c of direction dir, (s_c1 dxi(d,1) + s_c2 dxi(d,2) + s_c3 dxi(d,3)) J with
+// s_c = the stress row c (c < 3) or the heat flux (c = 3)
+__device__ __forceinline__ double visc_col_full(const FlowArgs& a, int dir, int c, unsigned g) {
+#pragma clang fp contract(off)
+  const MetRow M = met_row(a, dir, g);
+  const int k0 = c == 0 ? 0 : (c == 1 ? 1 : (c == 2 ? 2 : 6));
+  const int k1 = c == 0 ? 1 : (c == 1 ? 3 : (c == 2 ? 4 : 7));
+  const int k2 = c == 0 ? 2 : (c == 1 ? 4 : (c == 2 ? 5 : 8));
+  const double s0 = ldg(a.sg + k0 * a.fs, g), s1 = ldg(a.sg + k1 * a.fs, g), s2 = ldg(a.sg + k2 * a.fs, g);
+  return (s0 * M.r[0] + s1 * M.r[1] + s2 * M.r[2]) * M.j;
 }
 
 // Main path: q of the halo copy of slot t.  qswap fills the halo of a line
@@ -508,16 +577,35 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       if constexpr (LAST) {
         ChunkCoef<C::R, false> CD;
         lds_chunk_coef<C>(CD, cs, p * C::R);
-        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
       } else {
-        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CDr, a.fc);
+        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CDr, a.fc, a.bnd[dir]);
       }
       __syncthreads();
       const int fld = c < 3 ? c + 3 * dir : 9 + dir;
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        {
-          const double v = X[S.s[t]] * idn;
+        if (MAIN && a.met) {
+          // gradcal's accumulation dvel(m,n) += D_d(u_m) dxi(d,n) over every
+          // n (src/comsolver.F90:300-321); the last pass adds its terms with
+          // the stress below
+          const double x = X[S.s[t]];
+          gz[c][t] = x;
+          if constexpr (!LAST) {
+            const unsigned o = S.g[t];
+#pragma unroll
+            for (int nn = 0; nn < 3; nn++) {
+#pragma clang fp contract(off)
+              if (nn >= a.ndir) continue;
+              const int f2 = c < 3 ? c + 3 * nn : 9 + nn;
+              const double v = x * ldg(a.met + (1 + dir + 3 * nn) * fs, o);
+              if (PROG == GC_FIRST) stg(a.dv + f2 * fs, o, v);
+              else stg(a.dv + f2 * fs, o, ldg(a.dv + f2 * fs, o) + v);
+            }
+          }
+        } else {
+          // dvel(m,n) = D_d(u_m) dxi(d,n): dxi(d,d) of the uniform grid
+          const double v = X[S.s[t]] * (MAIN ? met_of(a, dir).d : idn);
           if (!LAST || sa.store_last_grad) stg(a.dv + fld * fs, S.g[t], v);
           gz[c][t] = v;
         }
@@ -534,7 +622,23 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         double g[3][3], gt[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          if (d == dir) {
+          if (MAIN && a.met) {
+#pragma clang fp contract(off)
+            // the earlier passes' terms plus this direction's, D_dir(u_m) dxi(dir,d)
+            if (d < a.ndir) {
+              const double md = ldg(a.met + (1 + dir + 3 * d) * fs, o);
+#pragma unroll
+              for (int m = 0; m < 3; m++) g[m][d] = ldg(a.dv + (m + 3 * d) * fs, o) + gz[m][t] * md;
+              gt[d] = ldg(a.dv + (9 + d) * fs, o) + gz[3][t] * md;
+              if (sa.store_last_grad) {
+#pragma unroll
+                for (int m = 0; m < 3; m++) stg(a.dv + (m + 3 * d) * fs, o, g[m][d]);
+                stg(a.dv + (9 + d) * fs, o, gt[d]);
+              }
+            } else {
+              g[0][d] = 0.0; g[1][d] = 0.0; g[2][d] = 0.0; gt[d] = 0.0;
+            }
+          } else if (d == dir) {
             g[0][d] = gz[0][t]; g[1][d] = gz[1][t]; g[2][d] = gz[2][t]; gt[d] = gz[3][t];
           } else if (d < a.ndir) {
             g[0][d] = ldg(a.dv + (0 + 3 * d) * fs, o);
@@ -611,13 +715,29 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
         if constexpr (MAIN) {
-          T[kLineOff + S.s[t]] = conv_flux_main(a, m, dir, qv[m][t], qv[4][t], ud[t], pp[t]);
-          if (S.h[t] != kDummy) {
-            double qh[5];
-            halo_q<C>(a, S, t, dir, qv, qh);
-            const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
-            const double uh = qh[1 + dir] / qh[0];
-            T[kLineOff + S.h[t]] = conv_flux_main(a, m, dir, qh[m], qh[4], uh, ph);
+          // a periodic halo copy carries the source point's metrics, which are this point's
+          if (a.met) {
+            const unsigned o = S.g[t];
+            const MetRow M = met_row(a, dir, o);
+            const double u = ldg(a.pr + fs, o), v = ldg(a.pr + 2 * fs, o), w = ldg(a.pr + 3 * fs, o);
+            T[kLineOff + S.s[t]] = conv_flux_full(M, m, qv[m][t], qv[4][t], u, v, w, pp[t]);
+            if (S.h[t] != kDummy) {
+              double qh[5];
+              halo_q<C>(a, S, t, dir, qv, qh);
+              const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+              T[kLineOff + S.h[t]] =
+                  conv_flux_full(M, m, qh[m], qh[4], qh[1] / qh[0], qh[2] / qh[0], qh[3] / qh[0], ph);
+            }
+          } else {
+            const Met M = met_of(a, dir);
+            T[kLineOff + S.s[t]] = conv_flux_main(M.j, M.d, m, dir, qv[m][t], qv[4][t], ud[t], pp[t]);
+            if (S.h[t] != kDummy) {
+              double qh[5];
+              halo_q<C>(a, S, t, dir, qv, qh);
+              const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+              const double uh = qh[1 + dir] / qh[0];
+              T[kLineOff + S.h[t]] = conv_flux_main(M.j, M.d, m, dir, qh[m], qh[4], uh, ph);
+            }
           }
         } else {
           put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
@@ -634,9 +754,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       if constexpr (LAST) {
         ChunkCoef<C::R, false> CD;
         lds_chunk_coef<C>(CD, cs, p * C::R);
-        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
       } else {
-        solve_line<C::P, C::R, false>(Tl, Xl, n, p, CDr, a.fc);
+        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CDr, a.fc, a.bnd[dir]);
       }
       __syncthreads();
 #pragma unroll
@@ -677,15 +797,16 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
         for (int t = 0; t < NPT; t++) {
           const unsigned o = S.g[t];
+          const double jb = jac_of(a, o);
           const double q0 = ldg(a.q, o);
           double s0;
           if (sa.stage == 0) {
-            s0 = q0 * a.jac;  // qsave = q J (:423-427)
+            s0 = q0 * jb;  // qsave = q J (:423-427)
             stg(a.qsave, o, s0);
           } else {
             s0 = ldg(a.qsave, o);
           }
-          qn[0][t] = rk_update_main(sa, a.dt, a.jac, s0, q0, ldg(a.qrhs, o));
+          qn[0][t] = rk_update_main(sa, a.dt, jb, s0, q0, ldg(a.qrhs, o));
           stg(a.q, o, qn[0][t]);
         }
       }
@@ -694,8 +815,18 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         // the main path's viscous fluxes carry the metrics:
         // (sigma . dxi(dir,:)) J (diffrsdcal6, src/solver.F90:2623-2635)
 #pragma unroll
-        for (int t = 0; t < NPT; t++)
-          put<C>(T, S, t, MAIN ? nx[t] * idn * a.jac : nx[t]);
+        for (int t = 0; t < NPT; t++) {
+          if constexpr (MAIN) {
+            if (a.met) {
+              put<C>(T, S, t, visc_col_full(a, dir, c, S.g[t]));
+            } else {
+              const Met M = met_of(a, dir);
+              put<C>(T, S, t, nx[t] * M.d * M.j);
+            }
+          } else {
+            put<C>(T, S, t, nx[t]);
+          }
+        }
         // loads for this column's accumulation and the next column's input are
         // issued before the solve so their latency overlaps it
         double qr[NPT], qo[NPT], qs[NPT];
@@ -717,19 +848,20 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         {
           ChunkCoef<C::R, false> CD;
           lds_chunk_coef<C>(CD, cs, p * C::R);
-          solve_line<C::P, C::R, false>(Tl, Xl, n, p, CD, a.fc);
+          solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
         }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < NPT; t++) {
           const double v = qr[t] + (MAIN ? X[S.s[t]] : X[S.s[t]] * idn);
           if constexpr (RKM) {
+            const double jb = jac_of(a, S.g[t]);
             double s0 = qs[t];
             if (sa.stage == 0) {
-              s0 = qo[t] * a.jac;
+              s0 = qo[t] * jb;
               stg(a.qsave + (1 + c) * fs, S.g[t], s0);
             }
-            qn[1 + c][t] = rk_update_main(sa, a.dt, a.jac, s0, qo[t], v);
+            qn[1 + c][t] = rk_update_main(sa, a.dt, jb, s0, qo[t], v);
             stg(a.q + (1 + c) * fs, S.g[t], qn[1 + c][t]);
           } else {
             stg(dst, S.g[t], v);
@@ -842,7 +974,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       {
         ChunkCoef<C::R, true> CF;
         lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
-        solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+        solve_pass_line<C, true, MAIN>(Tl, Xl, n, p, CF, fl, a.bnd[dir]);
       }
       __syncthreads();
       double* dst = a.q + m * fs;
@@ -922,8 +1054,8 @@ void launch_pass_m(int prog, const FlowArgs& a, int dir, hipStream_t st) {
     case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB, MAIN>), g, b, lds, st, a, dir); break;
     case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB, MAIN>), g, b, lds, st, a, dir); break;
     case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    // the filter pass is the same in both semantics
-    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB, false>), g, b, lds, st, a, dir); break;
+    // the filter pass of the main path also takes physical boundaries
+    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB, MAIN>), g, b, lds, st, a, dir); break;
   }
 }
 

@@ -197,7 +197,7 @@ contains
     cfg%semantics = ASTR_SEM_MAIN
     cfg%alfa_filter = alfa_filter
     cfg%beta_halo = 1.11d0
-    cfg%beta_bound = 1.09d0
+    cfg%beta_bound = 0.98d0   ! filter_coefficient_cal(beter_bouond=0.98d0), comsolver.F90:121
     cfg%gamma = gamma; cfg%mach = mach; cfg%prandtl = prandtl
     if (diffterm) then
       cfg%reynolds = reynolds

@@ -26,6 +26,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TGV 256³ FP64 Mpt-updates/s + % HBM peak; batched/served cases/s, p50/p99"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 
 # Algorithmic FP64 words per lattice point moved by the reference operators
 # each pass fuses (one streaming sweep per operator, reads + writes):
@@ -152,9 +153,22 @@ def chem_batch(ncells=1 << 20, reps=20):
     mass = float(np.abs(wd.sum(axis=0)).max() / np.abs(wd).max())
     evals = ncells / (ms * 1e-3)
     gbs = 176.0 * ncells / (ms * 1e-3) / 1e9
+    valu = None
+    cnt_path = os.path.join(ROOT, "profiles", "chem_valu_counters.json")
+    if os.path.exists(cnt_path):
+        # FP64 operations per cell from the kernel's own instruction counters
+        # (rocprofv3 SQ_INSTS_VALU_{ADD,MUL,FMA,TRANS}_F64 of this batch), times
+        # the cells per second measured here
+        with open(cnt_path) as fh:
+            fpc = json.load(fh)["flops_per_cell"]
+        tfs = fpc * evals / 1e12
+        valu = {"flops_per_cell": fpc, "achieved": round(tfs, 2), "peak": FP64_PEAK_TFS,
+                "unit": "TFLOP/s", "frac": round(tfs / FP64_PEAK_TFS, 4),
+                "source": "profiles/chem_valu_counters.json"}
     return {"cells": ncells, "ms_per_batch": round(ms, 4), "evals_per_s": round(evals, 1),
             "hbm": {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(gbs / HBM_PEAK_GBS, 4), "alg_bytes_per_cell": 176},
+            "valu": valu,
             "mass_conservation_relerr": mass, "dtype": "f64",
             "mechanism": "Jachimowski88_H2-9-19 (9 species, 19 reactions)"}
 
@@ -316,6 +330,39 @@ def serve_form(rates=(10.0, 100.0, 1000.0), counts=(40, 150, 300), seed=7, rtol=
     return out
 
 
+def main_path_step(n, dt, steps, warmup):
+    """The main solver's stage (ASTR_SEM_MAIN: filter at the stage start,
+    qswap, stored primitives, metric-weighted fluxes, J-weighted update) on
+    the same TGV workload and field: ms per step, Mpt-updates/s and the
+    algorithmic fraction of HBM peak (the same 1536 B per point per stage of
+    reference operators), with the per-pass times of the timed steps."""
+    import astr_amd as A
+
+    f = A.Flow(A.main_config(n, deltat=dt))
+    f.init_tgv(1e-3, 2024)
+    f.autotune(reps=3)
+    if warmup:
+        f.rk3(warmup, stats=True)
+    f.sync()
+    f.profile_begin(3 * 9 * steps + 8)
+    f.sync()
+    t0 = time.perf_counter()
+    ke, en = f.rk3(steps, stats=True)
+    f.sync()
+    wall = time.perf_counter() - t0
+    prof = f.profile_end()
+    pts = (n + 1) ** 3
+    step_alg = STAGE_BYTES * pts * 3 * steps / wall / 1e9
+    out = {"workload": f"TGV {n}^3 RK3 step in the main solver's stage order (astr run), Sutherland 110.3 K",
+           "ms_per_step": round(wall / steps * 1e3, 4),
+           "value": round(pts * 3 * steps / wall / 1e6, 3), "unit": "Mpt-updates/s",
+           "step_roofline": {"achieved_GBs": round(step_alg, 1), "frac": round(step_alg / HBM_PEAK_GBS, 4)},
+           "kernels_ms_per_launch": {f"{k[0]}_{k[1]}": round(v[0] / v[1], 4) for k, v in prof.items()},
+           "ke_last": float(ke[-1])}
+    f.close()
+    return out
+
+
 def load_traffic(n):
     path = os.path.join(ROOT, "profiles", f"traffic_tgv{n}.json")
     if os.path.exists(path):
@@ -347,6 +394,9 @@ def main():
     ap.add_argument("--no-chem", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-form measurement")
+    ap.add_argument("--no-main", action="store_true", help="skip the main-path step measurement")
+    ap.add_argument("--chem-only", action="store_true",
+                    help="only the chemistry batch (for counter runs of its kernel)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch selection instead of timing the candidates")
     ap.add_argument("--selection", metavar="JSON",
@@ -363,6 +413,9 @@ def main():
 
     import astr_amd as A
 
+    if args.chem_only:
+        print(json.dumps({"chem_batch": chem_batch()}), flush=True)
+        return
     n = args.n
     dt = 5e-4 * 256.0 / n
     f = A.Flow(A.tgv_config(n, deltat=dt))
@@ -441,6 +494,9 @@ def main():
                    "shape_is": "(lines per workgroup, chunks per line, unknowns per chunk)",
                    "passes": launch},
     }
+    f.close()
+    if not args.no_main:
+        out["main_path"] = main_path_step(n, dt, args.steps, args.warmup)
     if not args.no_chem:
         out["chem_batch"] = chem_batch()
     if not args.no_batch:

@@ -176,11 +176,24 @@ int astr_hip_gridgeom(astr_hip_ctx *ctx, const double *x, double *jacob, double 
  * 229-289) and reads no halo. */
 int astr_hip_gridgeom_nt(astr_hip_ctx *ctx, const int *npdc, const double *x, double *jacob,
                          double *dxi);
-/* Metrics for the passes, in gridgeom's layout.  This build's passes take one
- * 1/dx per direction, so only the metrics of a uniform Cartesian grid
- * (dxi diagonal and constant, jacob = 1/prod(dxi), both to 1e-12) are
- * accepted; 1/dx becomes the mean of dxi(d,d).  Others return an error. */
+/* Metrics for the passes, in gridgeom's layout.  The metrics of a uniform
+ * Cartesian grid (dxi diagonal and constant, jacob constant, to 1e-12) run
+ * on constants, the means of dxi(d,d) and jacob.  Any other grid
+ * (ASTR_SEM_MAIN only: stretched, curvilinear) keeps J and all nine dxi(d,n)
+ * per point, which the main path's fluxes, gradients and RK update read as
+ * the reference does (src/solver.F90:2196-2207, 2623-2635,
+ * src/comsolver.F90:280-321, src/mainloop.F90:441-449); the miniapp
+ * semantics refuse it. */
 int astr_hip_set_metrics(astr_hip_ctx *ctx, const double *jacob, const double *dxi);
+/* ASTR_SEM_MAIN: the boundary type of each direction, npdc[d] = npdci,
+ * npdcj, npdck of one rank (src/parallel.F90:1042-1106): 3 periodic (the
+ * default), 4 physical at both ends.  A physical direction takes the
+ * derivative's and the filter's boundary closures (derivative.F90:128-139,
+ * 229-289; filter.F90:31-100, 141-142, 175-262, with beta_bound of the
+ * config, 0.98 in the main solver) in every pass, reads no halo, and qswap
+ * neither averages nor refreshes it.  Boundary conditions (boucon) are not
+ * applied. */
+int astr_hip_set_boundary(astr_hip_ctx *ctx, const int *npdc);
 
 /* Per-pass timing with HIP events on the context's stream (no reference
  * counterpart; the reference's timerept accumulators, src/utility.F90:27-297,

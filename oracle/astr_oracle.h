@@ -91,6 +91,7 @@ double *or_main_qrhs(or_main *s);    /* 5 N */
 double *or_main_dvel(or_main *s);    /* 9 N, component m, direction n at m + 3n */
 double *or_main_dtmp(or_main *s);    /* 3 N */
 void or_main_set_metrics(or_main *s, const double *jac, const double *dxi);
+void or_main_set_npdc(or_main *s, const int *npdc);
 void or_main_set_q(or_main *s, const double *q);
 void or_main_set_q_only(or_main *s, const double *q);
 void or_main_get(or_main *s, double *q, double *prims);

@@ -73,6 +73,7 @@ struct or_main {
   double *sigma, *qflux;              /* 6 NH, 3 NH */
   msys ds[3], fs[3];
   double c10[6], ch[33];
+  int npdc[3];       /* boundary type per direction: 3 periodic, 4 physical at both ends */
 };
 
 static inline long H(const or_main *s, int i, int j, int k) {
@@ -130,7 +131,18 @@ or_main *or_main_create(const or_cfg *cfg) {
     msys_init(&s->fs[d], s->n[d], 1, cfg);
   }
   or_filter_coefs(cfg->alfa_filter, cfg->beta_halo, cfg->beta_bound, s->c10, s->ch);
+  s->npdc[0] = s->npdc[1] = s->npdc[2] = 3;
   return s;
+}
+
+/* The boundary type of each direction, npdci/npdcj/npdck of one rank
+ * (parallel.F90:1042-1106): 3 periodic (the default), 4 physical at both
+ * ends.  A physical direction takes the derivative's and the filter's
+ * boundary closures (or_df_compact_nt, or_compact_filter_nt with the main
+ * path's beta_bound 0.98, comsolver.F90:121), reads no halo, and qswap skips
+ * it (parallel.F90:5018: only homogeneous directions are swapped). */
+void or_main_set_npdc(or_main *s, const int *npdc) {
+  for (int d = 0; d < 3; d++) s->npdc[d] = npdc[d];
 }
 
 void or_main_destroy(or_main *s) {
@@ -223,6 +235,31 @@ void or_main_get(or_main *s, double *q, double *prims) {
       }
 }
 
+/* one line's derivative (f: f(-hm..n+hm), out: 0..n) in direction dir:
+ * the periodic system's Thomas solve, or the closed system of a physical
+ * direction */
+static void line_deriv_dir(const or_main *s, int dir, const double *f, double *d, double *x, double *out) {
+  const int n = s->n[dir];
+  if (s->npdc[dir] == 3) {
+    or_deriv_rhs(n, f, d);
+    or_thomas_solve(s->ds[dir].m, s->ds[dir].ac, d, x);
+    for (int t = 0; t <= n; t++) out[t] = x[t + 1];
+  } else {
+    or_df_compact_nt(n, s->npdc[dir], f, out);
+  }
+}
+
+static void line_filter_dir(const or_main *s, int dir, const double *f, double *d, double *x, double *out) {
+  const int n = s->n[dir];
+  if (s->npdc[dir] == 3) {
+    or_filter_rhs(n, s->c10, s->ch, f, d);
+    or_thomas_solve(s->fs[dir].m, s->fs[dir].ac, d, x);
+    for (int t = 0; t <= n; t++) out[t] = x[t + 3];
+  } else {
+    or_compact_filter_nt(n, s->npdc[dir], s->c.alfa_filter, s->c.beta_halo, 0.98, f, out);
+  }
+}
+
 /* ---- line access: direction dir, line through (i0,j0,k0) ---- */
 static inline long stride_of(const or_main *s, int dir) {
   return dir == 0 ? 1 : (dir == 1 ? s->ex : s->ex * s->ey);
@@ -254,8 +291,7 @@ static void dataswap_dir(or_main *s, double *var, int nf, int dir) {
 void or_main_filterq(or_main *s) {
   const long NH = s->NH;
   for (int dir = 0; dir < s->ndir; dir++) {
-    dataswap_dir(s, s->q, 5, dir);
-    const msys *y = &s->fs[dir];
+    if (s->npdc[dir] == 3) dataswap_dir(s, s->q, 5, dir);
     const int n = s->n[dir];
     const long st = stride_of(s, dir);
     const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
@@ -272,9 +308,7 @@ void or_main_filterq(or_main *s) {
         for (int m = 0; m < 5; m++) {
           double *qm = s->q + m * NH + h0;
           for (int t = -HM; t <= n + HM; t++) f[t + HM] = qm[t * st];
-          or_filter_rhs(n, s->c10, s->ch, f, d);
-          or_thomas_solve(y->m, y->ac, d, x);
-          for (int t = 0; t <= n; t++) ff[t] = x[t + 3];
+          line_filter_dir(s, dir, f, d, x, ff);
           for (int t = 0; t <= n; t++) qm[t * st] = ff[t];
         }
         free(f);
@@ -305,6 +339,7 @@ static void slab_q2fvar(or_main *s, int dir) {
 void or_main_qswap(or_main *s) {
   const long NH = s->NH;
   for (int dir = 0; dir < s->ndir; dir++) {
+    if (s->npdc[dir] != 3) continue;
     dataswap_dir(s, s->q, 5, dir);
     const long st = stride_of(s, dir);
     const int n = s->n[dir];
@@ -330,7 +365,6 @@ void or_main_gradcal(or_main *s) {
   memset(s->dvel, 0, sizeof(double) * 9 * N);
   memset(s->dtmp, 0, sizeof(double) * 3 * N);
   for (int dir = 0; dir < s->ndir; dir++) {
-    const msys *y = &s->ds[dir];
     const int n = s->n[dir];
     const long st = stride_of(s, dir);
     const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
@@ -338,7 +372,7 @@ void or_main_gradcal(or_main *s) {
     for (int b = 0; b <= s->n[d2]; b++)
       for (int a = 0; a <= s->n[d1]; a++) {
         const int L = n + 2 * HM + 1;
-        double *f = malloc(sizeof(double) * L * 3), *d = f + L, *x = d + L;
+        double *f = malloc(sizeof(double) * L * 4), *d = f + L, *x = d + L;
         int ijk[3] = {0, 0, 0};
         ijk[d1] = a;
         ijk[d2] = b;
@@ -346,14 +380,13 @@ void or_main_gradcal(or_main *s) {
         for (int col = 0; col < 4; col++) {
           const double *src = s->pv + (col < 3 ? 1 + col : 5) * NH + h0;
           for (int t = -HM; t <= n + HM; t++) f[t + HM] = src[t * st];
-          or_deriv_rhs(n, f, d);
-          or_thomas_solve(y->m, y->ac, d, x);
+          line_deriv_dir(s, dir, f, d, x, x + L);
           for (int t = 0; t <= n; t++) {
             int p3[3] = {ijk[0], ijk[1], ijk[2]};
             p3[dir] = t;
             const long p = P0(s, p3[0], p3[1], p3[2]);
             const long h = h0 + t * st;
-            const double df = x[t + 1];
+            const double df = x[L + t];
             for (int nn = 0; nn < 3; nn++) {
               const double mt = s->dxi[(dir + 3 * nn) * NH + h];
               if (col < 3) s->dvel[(col + 3 * nn) * N + p] = s->dvel[(col + 3 * nn) * N + p] + df * mt;
@@ -370,7 +403,6 @@ void or_main_gradcal(or_main *s) {
 static void convrsdcal6(or_main *s) {
   const long N = s->N, NH = s->NH;
   for (int dir = 0; dir < s->ndir; dir++) {
-    const msys *y = &s->ds[dir];
     const int n = s->n[dir];
     const long st = stride_of(s, dir);
     const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
@@ -378,7 +410,7 @@ static void convrsdcal6(or_main *s) {
     for (int b = 0; b <= s->n[d2]; b++)
       for (int a = 0; a <= s->n[d1]; a++) {
         const int L = n + 2 * HM + 1;
-        double *fc = malloc(sizeof(double) * L * 7), *d = fc + 5 * L, *x = d + L;
+        double *fc = malloc(sizeof(double) * L * 8), *d = fc + 5 * L, *x = d + L, *df = x + L;
         int ijk[3] = {0, 0, 0};
         ijk[d1] = a;
         ijk[d2] = b;
@@ -396,13 +428,12 @@ static void convrsdcal6(or_main *s) {
           fc[4 * L + t + HM] = jb * (s->q[4 * NH + h] + pr) * uu;
         }
         for (int m = 0; m < 5; m++) {
-          or_deriv_rhs(n, fc + m * L, d);
-          or_thomas_solve(y->m, y->ac, d, x);
+          line_deriv_dir(s, dir, fc + m * L, d, x, df);
           for (int t = 0; t <= n; t++) {
             int p3[3] = {ijk[0], ijk[1], ijk[2]};
             p3[dir] = t;
             const long p = P0(s, p3[0], p3[1], p3[2]);
-            s->qrhs[m * N + p] = s->qrhs[m * N + p] + x[t + 1];
+            s->qrhs[m * N + p] = s->qrhs[m * N + p] + df[t];
           }
         }
         free(fc);
@@ -449,13 +480,13 @@ static void diffrsdcal6(or_main *s) {
       }
   /* dataswap(sigma), dataswap(qflux): every direction */
   for (int dir = 0; dir < s->ndir; dir++) {
+    if (s->npdc[dir] != 3) continue;
     dataswap_dir(s, s->sigma, 6, dir);
     dataswap_dir(s, s->qflux, 3, dir);
   }
   /* sigma columns per component row: (11,12,13), (12,22,23), (13,23,33) */
   static const int sc[3][3] = {{0, 1, 2}, {1, 3, 4}, {2, 4, 5}};
   for (int dir = 0; dir < s->ndir; dir++) {
-    const msys *y = &s->ds[dir];
     const int n = s->n[dir];
     const long st = stride_of(s, dir);
     const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
@@ -463,7 +494,7 @@ static void diffrsdcal6(or_main *s) {
     for (int b = 0; b <= s->n[d2]; b++)
       for (int a = 0; a <= s->n[d1]; a++) {
         const int L = n + 2 * HM + 1;
-        double *ff = malloc(sizeof(double) * L * 6), *d = ff + 4 * L, *x = d + L;
+        double *ff = malloc(sizeof(double) * L * 7), *d = ff + 4 * L, *x = d + L, *df = x + L;
         int ijk[3] = {0, 0, 0};
         ijk[d1] = a;
         ijk[d2] = b;
@@ -479,13 +510,12 @@ static void diffrsdcal6(or_main *s) {
           ff[3 * L + t + HM] = (s->qflux[h] * x1 + s->qflux[NH + h] * x2 + s->qflux[2 * NH + h] * x3) * jb;
         }
         for (int c = 0; c < 4; c++) {
-          or_deriv_rhs(n, ff + c * L, d);
-          or_thomas_solve(y->m, y->ac, d, x);
+          line_deriv_dir(s, dir, ff + c * L, d, x, df);
           for (int t = 0; t <= n; t++) {
             int p3[3] = {ijk[0], ijk[1], ijk[2]};
             p3[dir] = t;
             const long p = P0(s, p3[0], p3[1], p3[2]);
-            s->qrhs[(1 + c) * N + p] = s->qrhs[(1 + c) * N + p] + x[t + 1];
+            s->qrhs[(1 + c) * N + p] = s->qrhs[(1 + c) * N + p] + df[t];
           }
         }
         free(ff);

@@ -87,6 +87,7 @@ def lib():
             getattr(L, f).restype = _dp
             getattr(L, f).argtypes = [C.c_void_p]
         L.or_main_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
+        L.or_main_set_npdc.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.or_main_set_q.argtypes = [C.c_void_p, _dp]
         L.or_main_set_q_only.argtypes = [C.c_void_p, _dp]
         L.or_main_get.argtypes = [C.c_void_p, _dp, _dp]
@@ -225,6 +226,10 @@ class MainState:
         if getattr(self, "h", None):
             lib().or_main_destroy(self.h)
             self.h = None
+
+    def set_npdc(self, npdc):
+        """Boundary type per direction (3 periodic, 4 physical at both ends)."""
+        lib().or_main_set_npdc(self.h, (C.c_int * 3)(*[int(v) for v in npdc]))
 
     def set_metrics(self, jac, dxi):
         """jac (N) and dxi (9, N), dxi[d + 3 n] = d xi_d / d x_n."""
