@@ -57,6 +57,7 @@ EXPORTS = [
     "astr_hip_set_metrics", "astr_hip_qswap",
     "astr_hip_get_last_split", "astr_hip_rk3_steps_stats_noalloc", "astr_hip_set_fvar",
     "astr_hip_write_checkpoint", "astr_hip_read_checkpoint", "astr_hip_set_boundary",
+    "astr_hip_set_convection",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
           5: (8, 64, 5), 6: (4, 32, 9), 7: (2, 64, 5),
@@ -100,6 +101,8 @@ def lib():
                                                C.POINTER(C.c_double)]
         L.astr_hip_set_metrics.argtypes = [C.c_void_p, _dp, _dp]
         L.astr_hip_set_boundary.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.astr_hip_set_convection.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double,
+                                              C.c_double]
         L.astr_hip_sync.argtypes = [C.c_void_p]
         L.astr_hip_line_batch.argtypes = [C.c_int, C.c_int, C.c_longlong, _dp, _dp,
                                           C.c_double, C.c_double]
@@ -299,6 +302,26 @@ class Flow:
         """Main path: the boundary type per direction (3 periodic, 4 physical
         at both ends)."""
         _chk(lib().astr_hip_set_boundary(self.h, (C.c_int * 3)(*[int(v) for v in npdc])))
+
+    def set_conschm(self, conschm="643c", recon_schem=0, lchardecomp=False, bfacmpld=0.3, shkcrt=0.05):
+        """Main path: the convective scheme as the reference's input file
+        names it (conschm, then the line recon_schem, lchardecomp, bfacmpld,
+        shkcrt; src/readwrite.F90:541-546).  An even first digit of conschm
+        selects the central compact scheme (643c only); an odd first digit
+        with suffix 'e' the explicit upwind-biased scheme (convrsduwd,
+        src/solver.F90:548-1201); the compact-upwind suffix 'c' is refused."""
+        conschm = conschm.strip()
+        if len(conschm) != 4 or not conschm[0].isdigit() or conschm[3] not in "ce":
+            raise AstrError(f"conschm {conschm!r}: expected three digits and 'c' or 'e'")
+        if int(conschm[0]) % 2 == 0:
+            if conschm != "643c":
+                raise AstrError(f"central scheme {conschm!r}: only 643c is provided")
+            _chk(lib().astr_hip_set_convection(self.h, 0, 0, 0, 0.0, 0.0))
+            return
+        if conschm[3] != "e":
+            raise AstrError(f"conschm {conschm!r}: the compact upwind-biased scheme is not provided")
+        _chk(lib().astr_hip_set_convection(self.h, 1, int(recon_schem), int(bool(lchardecomp)),
+                                           float(bfacmpld), float(shkcrt)))
 
     def set_metrics(self, jac, dxi):
         """Use the metrics (as gridgeom returns them): those of a Cartesian

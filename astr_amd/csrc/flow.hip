@@ -35,6 +35,7 @@
 
 #include "../../include/astr_hip.h"
 #include "flow_dev.hpp"
+#include "upwind.hpp"
 
 using namespace astr;
 
@@ -480,6 +481,10 @@ struct astr_hip_ctx {
   bool halo_valid = false;
   // main path on a grid that is not uniform: J and dxi(d,n) per point (10 fields)
   double* met = nullptr;
+  // main path, upwind-biased convection (a.uwd): its state and one
+  // allocation for the split buffer, the sensor and the shock flags
+  UwArgs uw{};
+  double* uwbuf = nullptr;
 };
 
 namespace {
@@ -683,14 +688,22 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* 
   FlowArgs& a = c->a;
   const int nd = a.ndir;
   a.dostats = stats ? 1 : 0;
-  a.store_last_grad = store_last_grad || diss_out ? 1 : 0;
+  // the shock sensor reads the whole velocity gradient
+  const bool sensor = a.uwd && c->uw.sensor;
+  a.store_last_grad = store_last_grad || diss_out || sensor ? 1 : 0;
   for (int d = 0; d < nd; d++) {
     if (d == nd - 1 && c->split_last) {
-      if (pass(c, GL_GRAD, d) || pass(c, GL_CONV, d)) return -1;
+      if (pass(c, GL_GRAD, d) || (!a.uwd && pass(c, GL_CONV, d))) return -1;
       continue;
     }
     const int prog = d == nd - 1 ? GC_LAST : (d == 0 ? GC_FIRST : GC_MID);
     if (pass(c, prog, d)) return -1;
+  }
+  if (a.uwd) {
+    // rhscal's upwind branch (solver.F90:219-226): ducrossensor when
+    // recon_schem = 5 or lchardecomp, then convrsduwd along i, j, k
+    if (sensor) HIP_OK(uw_launch_sensor(a, c->uw, c->stream));
+    for (int d = 0; d < nd; d++) HIP_OK(uw_launch_dir(a, c->uw, d, d == 0, d == nd - 1, c->stream));
   }
   if (diss_out) {
     const long long npts = (long long)a.n[0] * a.n[1] * (a.n[2] > 0 ? a.n[2] : 1);
@@ -1070,6 +1083,13 @@ int qswap_main(astr_hip_ctx* c, bool average) {
       hipLaunchKernelGGL(k_bprims, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, c->stream, a, d);
       HIP_OK(hipGetLastError());
     }
+    // a 2-D case still takes the k step (parallel.F90:5159-5170, 5184-5196):
+    // its slabs -hm..0 and 0..hm hold the plane k = 0, so q2fvar refreshes
+    // the primitives of every computed point
+    if (a.ndir == 2) {
+      hipLaunchKernelGGL(k_updatefvar, pt_grid(a), dim3(64), 0, c->stream, a);
+      HIP_OK(hipGetLastError());
+    }
   }
   c->halo_valid = true;
   return 0;
@@ -1262,6 +1282,7 @@ int astr_hip_destroy(astr_hip_ctx* c) {
   if (c->stats_out) (void)hipFree(c->stats_out);
   if (c->halo) (void)hipFree(c->halo);
   if (c->met) (void)hipFree(c->met);
+  if (c->uwbuf) (void)hipFree(c->uwbuf);
   for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
   if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1938,6 +1959,7 @@ int astr_hip_set_boundary(astr_hip_ctx* c, const int* npdc) {
     if (npdc[d] != 3 && npdc[d] != 4)
       return fail("npdc must be 3 (periodic) or 4 (physical at both ends) on one rank");
     bits[d] = ntype_bits(npdc[d]);
+    if (bits[d] && a.uwd) return fail("the upwind-biased convection runs on periodic directions only");
   }
   const FlowArgs keep = a;
   for (int d = 0; d < 3; d++) a.bnd[d] = bits[d];
@@ -1946,6 +1968,56 @@ int astr_hip_set_boundary(astr_hip_ctx* c, const int* npdc) {
     return -1;
   }
   c->halo_valid = false;
+  return 0;
+}
+
+int astr_hip_set_convection(astr_hip_ctx* c, int upwind, int recon_schem, int lchardecomp, double bfacmpld,
+                            double shkcrt) {
+  if (check_ctx(c)) return -1;
+  FlowArgs& a = c->a;
+  if (!upwind) {
+    (void)hipStreamSynchronize(c->stream);
+    if (c->uwbuf) (void)hipFree(c->uwbuf);
+    c->uwbuf = nullptr;
+    c->uw = UwArgs{};
+    a.uwd = 0;
+    return 0;
+  }
+  if (!a.main_sem) return fail("the upwind-biased convection is a main-path (ASTR_SEM_MAIN) feature");
+  if (!(recon_schem == -1 || recon_schem == 0 || recon_schem == 1 || recon_schem == 2 || recon_schem == 3 ||
+        recon_schem == 5 || recon_schem == 6))
+    return fail("recon_schem must be -1, 0, 1, 2, 3, 5 or 6 (recons_exp, flux.F90:305-335)");
+  for (int d = 0; d < a.ndir; d++)
+    if (a.bnd[d]) return fail("the upwind-biased convection runs on periodic directions only");
+  const int nf = lchardecomp ? kUwFieldsCd : kUwFields;
+  long long sws = 0;
+  for (int d = 0; d < a.ndir; d++) sws = std::max(sws, uw_points(a.n, d));
+  sws = (sws + 63) / 64 * 64;
+  const long long lsh_doubles = (a.fs + 7) / 8;
+  const size_t bytes = (size_t)(nf * sws + a.fs + lsh_doubles) * sizeof(double);
+  (void)hipStreamSynchronize(c->stream);
+  if (c->uwbuf) (void)hipFree(c->uwbuf);
+  c->uwbuf = nullptr;
+  a.uwd = 0;
+  if (test_fail_alloc() || hipMalloc(&c->uwbuf, bytes) != hipSuccess) {
+    c->uwbuf = nullptr;
+    c->uw = UwArgs{};
+    return fail("hipMalloc of the upwind split buffer failed");
+  }
+  HIP_OK(hipMemset(c->uwbuf, 0, bytes));
+  UwArgs& u = c->uw;
+  u.recon = recon_schem;
+  u.chardec = lchardecomp ? 1 : 0;
+  u.sensor = recon_schem == 5 || lchardecomp;
+  u.bfac = bfacmpld;
+  u.shkcrt = shkcrt;
+  u.gamma = c->cfg.gamma;
+  u.mach = c->cfg.mach;
+  u.sw = c->uwbuf;
+  u.sws = sws;
+  u.ssf = c->uwbuf + nf * sws;
+  u.lsh = reinterpret_cast<unsigned char*>(u.ssf + a.fs);
+  a.uwd = 1;
   return 0;
 }
 

@@ -55,6 +55,9 @@ struct FlowArgs {
   // grid's jac and inv_d above, dxi(d,n) = 0 for d != n)
   int bnd[3];
   const double* met;
+  // main path: the convective term is the upwind-biased one (upwind.hip),
+  // which the gradient passes then leave out
+  int uwd;
 };
 
 // Halo sources kept by qswap for the lines whose halo the reference fills
@@ -696,6 +699,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     }
     }
 
+    // the main path's upwind-biased convection replaces this part (uniform
+    // over the workgroup, after the last barrier of the gradient part)
+    if (MAIN && a.uwd) return;
     if constexpr (DO_CONV) {
     // convective flux derivatives along dir (convection, tgvsolver.F90:539-633)
     double ud[NPT], pp[NPT];

@@ -1,0 +1,624 @@
+// upwind.hip -- the reference's explicit upwind-biased convective term
+// (convrsduwd, src/solver.F90:548-1201) and its shock sensor (ducrossensor,
+// src/commcal.F90:196-360) on gfx950, for the main path (stored primitives,
+// qswap's halo sources) with periodic directions on one rank.
+//
+// Per direction two kernels:
+//   k_uw_split   the Steger-Warming split (flux_steger_warming,
+//                src/riemann.F90:23-166) of every point of every line,
+//                rows -4..n+4 (the halo rows from qswap's halo q and the
+//                primitives q2fvar gives it), into a split buffer laid out
+//                so that the next kernel reads it coalesced: rows
+//                contiguous for x-lines, lines contiguous for y- and
+//                z-lines;
+//   k_uw_face_*  per interface i = -1..n the 8-point windows of F+ and F-
+//                (iwind8, solver.F90:1213-1234), optionally projected on
+//                the Roe-state characteristics (chardecomp,
+//                solver.F90:1958-2162), reconstructed (recons_exp,
+//                src/flux.F90:269-383) and summed, then qrhs(i) =
+//                qrhs(i) + Fh(i) - Fh(i-1).  x-lines: one lane per
+//                interface, 63 points per wavefront, Fh(i-1) from the
+//                neighbouring lane; y- and z-lines: one lane per line and
+//                a run of 16 points, Fh(i-1) carried in registers.
+// The sensor: k_ssf (the Ducros ratio from the stored velocity gradient and
+// the pressure with its halo values) and k_lshock (its maximum over the
+// cross of offsets -4..5 against shkcrt).
+//
+// Expressions keep the reference's left-to-right order and its decimal
+// constants, without FMA contraction.
+#include <hip/hip_runtime.h>
+
+#include "upwind.hpp"
+
+#pragma clang fp contract(off)
+
+namespace astr {
+namespace {
+
+constexpr int kRun = 16;  // points per lane of a y- or z-line
+
+// ---------------------------------------------------------------------------
+// pointwise pieces
+// ---------------------------------------------------------------------------
+
+// flux_steger_warming at one point (riemann.F90:45-162, nondimen)
+__device__ __forceinline__ void sw_split(double gamma, double mach, double jac, const double dx[3],
+                                         const double q[5], const double pv[6], double fp[5],
+                                         double fm[5]) {
+  const double eps = 0.04;
+  const double rho = pv[0], u = pv[1], v = pv[2], w = pv[3], prs = pv[4], tmp = pv[5];
+  const double uu = dx[0] * u + dx[1] * v + dx[2] * w;
+  const double var0 = 1.0 / sqrt(dx[0] * dx[0] + dx[1] * dx[1] + dx[2] * dx[2]);
+  const double gpd0 = dx[0] * var0, gpd1 = dx[1] * var0, gpd2 = dx[2] * var0;
+  const double gm2 = 0.5 / gamma;
+  const double css = sqrt(tmp) / mach;
+  const double csa = css / var0;
+  const double lmach = uu / csa;
+  if (lmach >= 1.0 || lmach <= -1.0) {
+    double* f = lmach >= 1.0 ? fp : fm;
+    double* z = lmach >= 1.0 ? fm : fp;
+    f[0] = jac * q[0] * uu;
+    f[1] = jac * (q[1] * uu + dx[0] * prs);
+    f[2] = jac * (q[2] * uu + dx[1] * prs);
+    f[3] = jac * (q[3] * uu + dx[2] * prs);
+    f[4] = jac * (q[4] + prs) * uu;
+#pragma unroll
+    for (int m = 0; m < 5; m++) z[m] = 0.0;
+    return;
+  }
+  const double l0 = uu, l3 = uu + csa, l4 = uu - csa;
+  const double p0 = 0.5 * (l0 + sqrt(l0 * l0 + eps * eps));
+  const double p3 = 0.5 * (l3 + sqrt(l3 * l3 + eps * eps));
+  const double p4 = 0.5 * (l4 + sqrt(l4 * l4 + eps * eps));
+  const double fhi = 0.5 * (gamma - 1.0) * (u * u + v * v + w * w);
+  const double jro = jac * rho;
+#pragma unroll
+  for (int side = 0; side < 2; side++) {
+    const double e0 = side == 0 ? p0 : l0 - p0;
+    const double e3 = side == 0 ? p3 : l3 - p3;
+    const double e4 = side == 0 ? p4 : l4 - p4;
+    double* f = side == 0 ? fp : fm;
+    const double var1 = e0;
+    const double var2 = e3 - e4;
+    const double var3 = 2.0 * e0 - e3 - e4;
+    const double var4 = var1 - var3 * gm2;
+    f[0] = jro * var4;
+    f[1] = jro * (var4 * u + var2 * css * gpd0 * gm2);
+    f[2] = jro * (var4 * v + var2 * css * gpd1 * gm2);
+    f[3] = jro * (var4 * w + var2 * css * gpd2 * gm2);
+    f[4] = jac * (var1 * q[4] +
+                  rho * (var2 * uu * var0 * css * gm2 - var3 * (fhi + css * css) * gm2 / (gamma - 1.0)));
+  }
+}
+
+// minmod2, minmod4 (commfunc.F90:699-738)
+__device__ __forceinline__ double minmod2(double a, double b) {
+  if (a > 0.0 && b > 0.0) return fmin(fabs(a), fabs(b));
+  if (a < 0.0 && b < 0.0) return -1.0 * fmin(fabs(a), fabs(b));
+  return 0.0;
+}
+__device__ __forceinline__ double minmod4(double a, double b, double c, double d) {
+  if (a > 0.0 && b > 0.0 && c > 0.0 && d > 0.0) return fmin(fmin(fmin(fabs(a), fabs(b)), fabs(c)), fabs(d));
+  if (a < 0.0 && b < 0.0 && c < 0.0 && d < 0.0)
+    return -1.0 * fmin(fmin(fmin(fabs(a), fabs(b)), fabs(c)), fabs(d));
+  return 0.0;
+}
+
+// SUW7 (flux.F90:418-424) on u(1:7) = f[0..6]
+__device__ __forceinline__ double suw7(const double* f) {
+  return (-3.0 * f[0] + 25.0 * f[1] - 101.0 * f[2] + 319.0 * f[3] + 214.0 * f[4] - 38.0 * f[5] +
+          4.0 * f[6]) / 420.0;
+}
+
+// WENO7 (flux.F90:820-892) and WENO7-Z (:977-1055) on u(1:7) = f[0..6]
+__device__ __forceinline__ double weno7(const double* f, bool z) {
+  const double eps = 1.0e-6;
+  const double u1 = f[0], u2 = f[1], u3 = f[2], u4 = f[3], u5 = f[4], u6 = f[5], u7 = f[6];
+  const double uh1 = -0.25 * u1 + 1.083333333333333 * u2 - 1.916666666666667 * u3 + 2.083333333333333 * u4;
+  const double uh2 = 8.333333333333333e-2 * u2 - 4.166666666666667e-1 * u3 + 1.083333333333333 * u4 + 0.25 * u5;
+  const double uh3 = -8.333333333333333e-2 * u3 + 5.833333333333333e-1 * u4 + 5.833333333333333e-1 * u5 -
+                     8.333333333333333e-2 * u6;
+  const double uh4 = 0.25 * u4 + 1.083333333333333 * u5 - 4.166666666666667e-1 * u6 + 8.333333333333333e-2 * u7;
+  double a, b, c;
+  a = -2.0 * u1 + 9.0 * u2 - 18.0 * u3 + 11.0 * u4;
+  b = -1.0 * u1 + 4.0 * u2 - 5.0 * u3 + 2.0 * u4;
+  c = -1.0 * u1 + 3.0 * u2 - 3.0 * u3 + 1.0 * u4;
+  const double bt1 = 2.777777777777778e-2 * (a * a) + 1.083333333333333 * (b * b) + 1.084722222222222 * (c * c);
+  a = u2 - 6.0 * u3 + 3.0 * u4 + 2.0 * u5;
+  b = u3 - 2.0 * u4 + u5;
+  c = -1.0 * u2 + 3.0 * u3 - 3.0 * u4 + 1.0 * u5;
+  const double bt2 = 2.777777777777778e-2 * (a * a) + 1.083333333333333 * (b * b) + 1.084722222222222 * (c * c);
+  a = -2.0 * u3 - 3.0 * u4 + 6.0 * u5 - 1.0 * u6;
+  b = u3 - 2.0 * u4 + u5;
+  c = -1.0 * u3 + 3.0 * u4 - 3.0 * u5 + 1.0 * u6;
+  const double bt3 = 2.777777777777778e-2 * (a * a) + 1.083333333333333 * (b * b) + 1.084722222222222 * (c * c);
+  a = -11.0 * u4 + 18.0 * u5 - 9.0 * u6 + 2.0 * u7;
+  b = 2.0 * u4 - 5.0 * u5 + 4.0 * u6 - u7;
+  c = -1.0 * u4 + 3.0 * u5 - 3.0 * u6 + 1.0 * u7;
+  const double bt4 = 2.777777777777778e-2 * (a * a) + 1.083333333333333 * (b * b) + 1.084722222222222 * (c * c);
+  const double C1 = 2.857142857142857e-2, C2 = 3.428571428571429e-1, C3 = 5.142857142857143e-1,
+               C4 = 1.142857142857143e-1;
+  double a1, a2, a3, a4;
+  if (z) {
+    const double tau7 = fabs(bt4 - bt1);
+    a1 = C1 + C1 * (tau7 / ((bt1 + eps) * (bt1 + eps)));
+    a2 = C2 + C2 * (tau7 / ((bt2 + eps) * (bt2 + eps)));
+    a3 = C3 + C3 * (tau7 / ((bt3 + eps) * (bt3 + eps)));
+    a4 = C4 + C4 * (tau7 / ((bt4 + eps) * (bt4 + eps)));
+  } else {
+    a1 = C1 / ((bt1 + eps) * (bt1 + eps));
+    a2 = C2 / ((bt2 + eps) * (bt2 + eps));
+    a3 = C3 / ((bt3 + eps) * (bt3 + eps));
+    a4 = C4 / ((bt4 + eps) * (bt4 + eps));
+  }
+  const double s = a1 + a2 + a3 + a4;
+  const double w1 = a1 / s, w2 = a2 / s, w3 = a3 / s, w4 = a4 / s;
+  return w1 * uh1 + w2 * uh2 + w3 * uh3 + w4 * uh4;
+}
+
+// the MP limiter of MP7 / MP7-LD (flux.F90:515-546, 669-703), u(0:5) = f[0..5]
+__device__ __forceinline__ double mp_limit(const double* u, double ul) {
+  const double dm1 = u[1] - 2.0 * u[2] + u[3];
+  const double d0 = u[2] - 2.0 * u[3] + u[4];
+  const double d1 = u[3] - 2.0 * u[4] + u[5];
+  const double dhm1 = minmod4(4.0 * dm1 - d0, 4.0 * d0 - dm1, dm1, d0);
+  const double dh0 = minmod4(4.0 * d0 - d1, 4.0 * d1 - d0, d0, d1);
+  const double uUL = u[3] + 4.0 * (u[3] - u[2]);
+  const double uAV = 0.5 * (u[3] + u[4]);
+  const double uMD = uAV - 0.5 * dh0;
+  const double uLC = u[3] + 0.5 * (u[3] - u[2]) + 1.333333333333333 * dhm1;
+  const double uMIN = fmax(fmin(fmin(u[3], u[4]), uMD), fmin(fmin(u[3], uUL), uLC));
+  const double uMAX = fmin(fmax(fmax(u[3], u[4]), uMD), fmax(fmax(u[3], uUL), uLC));
+  return ul + minmod2(uMIN - ul, uMAX - ul);
+}
+
+// MP7 (flux.F90:498-551), u(0:6) = f[0..6]
+__device__ __forceinline__ double mp7(const double* u) {
+  const double ul = suw7(u);
+  const double ump = u[3] + minmod2(u[4] - u[3], 4.0 * (u[3] - u[2]));
+  if ((ul - u[3]) * (ul - ump) >= 1.0e-10) return mp_limit(u, ul);
+  return ul;
+}
+
+// MP7-LD (flux.F90:636-708), u(0:7) = f[0..7]
+__device__ __forceinline__ double mp7ld(const double* u, double bw, bool shock) {
+  const double vadp = 3.571428571428571e-3 * bw;
+  const double b0 = 1.0 * vadp - 7.142857142857143e-3;
+  const double b1 = -7.0 * vadp + 5.952380952380952e-2;
+  const double b2 = 21.0 * vadp - 0.240476190476190;
+  const double b3 = -35.0 * vadp + 0.759523809523809;
+  const double b4 = 35.0 * vadp + 0.509523809523809;
+  const double b5 = -21.0 * vadp - 9.047619047619047e-2;
+  const double b6 = 7.0 * vadp + 9.523809523809525e-3;
+  const double b7 = -1.0 * vadp;
+  const double ul =
+      b0 * u[0] + b1 * u[1] + b2 * u[2] + b3 * u[3] + b4 * u[4] + b5 * u[5] + b6 * u[6] + b7 * u[7];
+  return shock ? mp_limit(u, ul) : ul;
+}
+
+// ROUND (flux.F90:717-752), u(1:3) = f[2..4]
+__device__ __forceinline__ double round3(const double* f) {
+  const double eps = 1.0e-16;
+  const double u1 = f[2], u2 = f[3], u3 = f[4];
+  const double z = (u2 - u1 + eps) / (u3 - u1 + eps);
+  const double a1 = 1.0 + 12.0 * z * z;
+  const double a2 = 1.0 + 5.0 * (z - 1.0) * (z - 1.0);
+  const double pl = 1100.0 * (z - 0.05) * (z - 0.05) * (z - 0.05) * (0.47 - z) * (0.47 - z) * (0.47 - z);
+  const double pr = 18000.0 * (z - 0.55) * (z - 0.55) * (z - 0.55) * (0.97 - z) * (0.97 - z) * (0.97 - z) *
+                    (0.97 - z) * (0.97 - z);
+  const double p1 = 0.833333333333333 * z + 0.333333333333333 + fmax(pl, 0.0) + fmax(pr, 0.0);
+  const double p2 = 1.5 * z;
+  const double p3 = 0.5 * z + 0.5;
+  const double w1 = 1.0 / a1 / a1 / a1 / a1;
+  const double w2 = 1.0 / a2 / a2 / a2 / a2 / a2 / a2 / a2 / a2;
+  const double g = (p1 * (1.0 - w1) + p2 * w1) * (1.0 - w2) + p3 * w2;
+  return g * (u3 - u1) + u1;
+}
+
+// recons_exp's periodic branch (flux.F90:305-335); the scheme is checked on
+// the host
+__device__ __forceinline__ double recons(int scheme, const double* f, double bfac, bool shock) {
+  switch (scheme) {
+    case -1: return f[3];
+    case 0: return suw7(f);
+    case 1: return weno7(f, false);
+    case 2: return weno7(f, true);
+    case 3: return mp7(f);
+    case 5: return mp7ld(f, bfac, shock);
+    default: return round3(f);
+  }
+}
+
+// chardecomp (solver.F90:1958-2162, no species); l, r = rho, u, v, w, p, E,
+// dxi row; rev, lev row-major 5 x 5.  A zero averaged metric row (where the
+// reference stops) cannot occur on the grids the host accepts.
+__device__ __forceinline__ void chardecomp(double gamma, const double* l, const double* r, double* rev,
+                                           double* lev) {
+  const double rero = 1.0e-12;
+  const double WRoe = sqrt(l[0]) / (sqrt(l[0]) + sqrt(r[0]));
+  const double WRoe1 = 1.0 - WRoe;
+  const double u1 = WRoe * l[1] + WRoe1 * r[1];
+  const double u2 = WRoe * l[2] + WRoe1 * r[2];
+  const double u3 = WRoe * l[3] + WRoe1 * r[3];
+  const double KRoe = 0.5 * (u1 * u1 + u2 * u2 + u3 * u3);
+  const double HL = (l[5] + l[4]) / l[0];
+  const double HR = (r[5] + r[4]) / r[0];
+  const double HRoe = WRoe * HL + WRoe1 * HR;
+  const double Css = sqrt((gamma - 1.0) * (HRoe - KRoe));
+  const double rcs = 1.0 / Css;
+  const double var1 = 0.5 * (l[6] + r[6]);
+  const double var2 = 0.5 * (l[7] + r[7]);
+  const double var3 = 0.5 * (l[8] + r[8]);
+  const double var4 = 1.0 / sqrt(var1 * var1 + var2 * var2 + var3 * var3);
+  const double g1 = var1 * var4, g2 = var2 * var4, g3 = var3 * var4;
+  const double ugp = u1 * g1 + u2 * g2 + u3 * g3;
+  const double b1 = (gamma - 1.0) / (Css * Css);
+  const double b2 = 1.0 + 2.0 * b1 * KRoe - b1 * HRoe;
+  lev[0] = 0.5 * (b2 + ugp * rcs);
+  lev[1] = -0.5 * (b1 * u1 + g1 * rcs);
+  lev[2] = -0.5 * (b1 * u2 + g2 * rcs);
+  lev[3] = -0.5 * (b1 * u3 + g3 * rcs);
+  lev[4] = 0.5 * b1;
+  lev[5] = 1.0 - b2;
+  lev[6] = b1 * u1;
+  lev[7] = b1 * u2;
+  lev[8] = b1 * u3;
+  lev[9] = -b1;
+  rev[0] = 1.0; rev[5] = u1 - Css * g1; rev[10] = u2 - Css * g2; rev[15] = u3 - Css * g3;
+  rev[20] = HRoe - ugp * Css;
+  rev[1] = 1.0; rev[6] = u1; rev[11] = u2; rev[16] = u3; rev[21] = HRoe - 1.0 / b1;
+  if (fabs(var1) > rero) {
+    const double rgp = 1.0 / g1;
+    lev[10] = (ugp * g2 - u2) * rgp; lev[11] = -g2; lev[12] = (1.0 - g2 * g2) * rgp;
+    lev[13] = -(g2 * g3 * rgp); lev[14] = 0.0;
+    lev[15] = (ugp * g3 - u3) * rgp; lev[16] = -g3; lev[17] = -(g2 * g3 * rgp);
+    lev[18] = (1.0 - g3 * g3) * rgp; lev[19] = 0.0;
+    rev[2] = 0.0; rev[7] = -g2; rev[12] = g1; rev[17] = 0.0; rev[22] = u2 * g1 - u1 * g2;
+    rev[3] = 0.0; rev[8] = -g3; rev[13] = 0.0; rev[18] = g1; rev[23] = u3 * g1 - u1 * g3;
+  } else if (fabs(var2) > rero) {
+    const double rgp = 1.0 / g2;
+    lev[10] = (ugp * g1 - u1) * rgp; lev[11] = (1.0 - g1 * g1) * rgp; lev[12] = -g1;
+    lev[13] = -(g1 * g3 * rgp); lev[14] = 0.0;
+    lev[15] = (ugp * g3 - u3) * rgp; lev[16] = -(g1 * g3 * rgp); lev[17] = -g3;
+    lev[18] = (1.0 - g3 * g3) * rgp; lev[19] = 0.0;
+    rev[2] = 0.0; rev[7] = g2; rev[12] = -g1; rev[17] = 0.0; rev[22] = u1 * g2 - u2 * g1;
+    rev[3] = 0.0; rev[8] = 0.0; rev[13] = -g3; rev[18] = g2; rev[23] = u3 * g2 - u2 * g3;
+  } else {
+    const double rgp = 1.0 / g3;
+    lev[10] = (ugp * g1 - u1) * rgp; lev[11] = (1.0 - g1 * g1) * rgp; lev[12] = -(g1 * g2 * rgp);
+    lev[13] = -g1; lev[14] = 0.0;
+    lev[15] = (ugp * g2 - u2) * rgp; lev[16] = -(g1 * g2 * rgp); lev[17] = (1.0 - g2 * g2) * rgp;
+    lev[18] = -g2; lev[19] = 0.0;
+    rev[2] = 0.0; rev[7] = g3; rev[12] = 0.0; rev[17] = -g1; rev[22] = u1 * g3 - u3 * g1;
+    rev[3] = 0.0; rev[8] = 0.0; rev[13] = g3; rev[18] = -g2; rev[23] = u2 * g3 - u3 * g2;
+  }
+  lev[20] = 0.5 * (b2 - ugp * rcs);
+  lev[21] = -0.5 * (b1 * u1 - g1 * rcs);
+  lev[22] = -0.5 * (b1 * u2 - g2 * rcs);
+  lev[23] = -0.5 * (b1 * u3 - g3 * rcs);
+  lev[24] = 0.5 * b1;
+  rev[4] = 1.0; rev[9] = u1 + Css * g1; rev[14] = u2 + Css * g2; rev[19] = u3 + Css * g3;
+  rev[24] = HRoe + ugp * Css;
+}
+
+// ---------------------------------------------------------------------------
+// index helpers
+// ---------------------------------------------------------------------------
+
+// offset of the point of line L of direction dir at row r (0..n)
+__device__ __forceinline__ long long line_base(const FlowArgs& a, int dir, long long L, int* c3) {
+  if (dir == 0) {
+    c3[1] = (int)(L % (a.n[1] + 1));
+    c3[2] = (int)(L / (a.n[1] + 1));
+    c3[0] = 0;
+  } else {
+    c3[0] = (int)(L % (a.n[0] + 1));
+    const int o = (int)(L / (a.n[0] + 1));
+    c3[1] = dir == 1 ? 0 : o;
+    c3[2] = dir == 1 ? o : 0;
+  }
+  return (long long)c3[2] * a.plane + (long long)c3[1] * a.pitch + c3[0];
+}
+__device__ __forceinline__ long long dir_stride(const FlowArgs& a, int dir) {
+  return dir == 0 ? 1 : (dir == 1 ? (long long)a.pitch : a.plane);
+}
+// the halo-source row index of the periodic copy of row rs (1..4 or n-4..n-1)
+__device__ __forceinline__ int hrow_of(int rs, int n) { return rs <= 5 ? rs - 1 : rs - (n - 5) + 5; }
+
+// q of the halo copy (row r outside 0..n) of line (c3) of direction dir:
+// the kept halo source where qswap filled the halo before a later step
+// averaged the source (x-lines on j or k boundary planes, y-lines on k
+// boundary planes), the current q of the source point otherwise
+__device__ __forceinline__ void halo_q_at(const FlowArgs& a, int dir, const int* c3, int rs, long long os,
+                                          double* q) {
+  const int ex = dir < 2 ? halo_src_index(a.n, a.ndir, dir, c3[0], c3[1], c3[2], hrow_of(rs, a.n[dir])) : -1;
+  if (ex >= 0) {
+    const double* src = dir == 0 ? a.hx : a.hy;
+    const long long hs = dir == 0 ? a.hxs : a.hys;
+#pragma unroll
+    for (int m = 0; m < 5; m++) q[m] = src[m * hs + ex];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; m++) q[m] = a.q[m * a.fs + os];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u, int dir) {
+  const int n = a.n[dir];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, dir);
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= NL * NR) return;
+  long long L;
+  int rr;
+  if (dir == 0) {
+    L = idx / NR;
+    rr = (int)(idx - L * NR);
+  } else {
+    rr = (int)(idx / NL);
+    L = idx - (long long)rr * NL;
+  }
+  const int r = rr - 4;
+  int c3[3];
+  const long long o0 = line_base(a, dir, L, c3);
+  const bool halo = r < 0 || r > n;
+  const int rs = r < 0 ? r + n : (r > n ? r - n : r);
+  const long long o = o0 + rs * dir_stride(a, dir);
+  const long long fs = a.fs;
+  double q[5], pv[6];
+  if (halo) {
+    halo_q_at(a, dir, c3, rs, o, q);
+    const Prim P = prim_of(a, q[0], q[1], q[2], q[3], q[4]);
+    pv[0] = P.r; pv[1] = P.u; pv[2] = P.v; pv[3] = P.w; pv[4] = P.p; pv[5] = P.t;
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; m++) q[m] = a.q[m * fs + o];
+#pragma unroll
+    for (int m = 0; m < 6; m++) pv[m] = a.pr[m * fs + o];
+  }
+  // a periodic halo copy carries its source point's metrics
+  double dx[3], jb;
+  if (a.met) {
+    jb = a.met[o];
+#pragma unroll
+    for (int nn = 0; nn < 3; nn++) dx[nn] = a.met[(1 + dir + 3 * nn) * fs + o];
+  } else {
+    jb = a.jac;
+#pragma unroll
+    for (int nn = 0; nn < 3; nn++) dx[nn] = nn == dir ? a.inv_d[dir] : 0.0;
+  }
+  double fp[5], fm[5];
+  sw_split(u.gamma, u.mach, jb, dx, q, pv, fp, fm);
+  double* sw = u.sw + idx;
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    sw[m * u.sws] = fp[m];
+    sw[(5 + m) * u.sws] = fm[m];
+  }
+  if (u.chardec) {
+    const double st[9] = {pv[0], pv[1], pv[2], pv[3], pv[4], q[4], dx[0], dx[1], dx[2]};
+#pragma unroll
+    for (int m = 0; m < 9; m++) sw[(kUwFields + m) * u.sws] = st[m];
+  }
+}
+
+// the shock flag of interface i (solver.F90:630-644) on line (o0, stride)
+__device__ __forceinline__ bool face_shock(const UwArgs& u, long long o0, long long st, int i, int n) {
+  if (!u.sensor) return true;
+  if (i < 0) return u.lsh[o0] != 0;
+  if (i + 1 > n) return u.lsh[o0 + (long long)n * st] != 0;
+  return u.lsh[o0 + (long long)i * st] != 0 || u.lsh[o0 + (long long)(i + 1) * st] != 0;
+}
+
+// Fh of the interface between rows i and i+1; c0 = the split-buffer index
+// of row i, rs = the buffer stride between rows
+template <bool CD>
+__device__ __forceinline__ void face_flux(const UwArgs& u, long long c0, long long rs, bool lsh, double* Fh) {
+  const double* sw = u.sw;
+  const long long S = u.sws;
+  double Fhc[5];
+  if constexpr (CD) {
+    if (lsh) {
+      double sl[9], sr[9], lev[25], rev[25];
+#pragma unroll
+      for (int m = 0; m < 9; m++) {
+        sl[m] = sw[(kUwFields + m) * S + c0];
+        sr[m] = sw[(kUwFields + m) * S + c0 + rs];
+      }
+      chardecomp(u.gamma, sl, sr, rev, lev);
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+        double wp[8], wm[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+          // iwind8: '+' element k is row i+k-3, '-' element k is row i+4-k
+          const long long ip = c0 + (long long)(k - 3) * rs, im = c0 + (long long)(4 - k) * rs;
+          wp[k] = lev[5 * m] * sw[ip] + lev[5 * m + 1] * sw[S + ip] + lev[5 * m + 2] * sw[2 * S + ip] +
+                  lev[5 * m + 3] * sw[3 * S + ip] + lev[5 * m + 4] * sw[4 * S + ip];
+          wm[k] = lev[5 * m] * sw[5 * S + im] + lev[5 * m + 1] * sw[6 * S + im] + lev[5 * m + 2] * sw[7 * S + im] +
+                  lev[5 * m + 3] * sw[8 * S + im] + lev[5 * m + 4] * sw[9 * S + im];
+        }
+        const double v1 = recons(u.recon, wp, u.bfac, lsh);
+        const double v2 = recons(u.recon, wm, u.bfac, lsh);
+        Fhc[m] = v1 + v2;
+      }
+#pragma unroll
+      for (int m = 0; m < 5; m++)
+        Fh[m] = rev[5 * m] * Fhc[0] + rev[5 * m + 1] * Fhc[1] + rev[5 * m + 2] * Fhc[2] + rev[5 * m + 3] * Fhc[3] +
+                rev[5 * m + 4] * Fhc[4];
+      return;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    double wp[8], wm[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      wp[k] = sw[m * S + c0 + (long long)(k - 3) * rs];
+      wm[k] = sw[(5 + m) * S + c0 + (long long)(4 - k) * rs];
+    }
+    const double v1 = recons(u.recon, wp, u.bfac, lsh);
+    const double v2 = recons(u.recon, wm, u.bfac, lsh);
+    Fh[m] = v1 + v2;
+  }
+}
+
+// qrhs(t) = qrhs(t) + Fh(t) - Fh(t-1) (solver.F90:789-795); the first
+// direction starts from rhscal's zero, the last one negates (:242)
+__device__ __forceinline__ void face_update(const FlowArgs& a, long long o, const double* cur, const double* prev,
+                                            bool first, bool last) {
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    double* dst = a.qrhs + m * a.fs + o;
+    const double old = first ? 0.0 : *dst;
+    const double v = old + cur[m] - prev[m];
+    *dst = last ? -v : v;
+  }
+}
+
+// x-lines: lane l of a wavefront takes interface i0 - 1 + l of segment
+// (L, sg); lanes 1..63 update points i0..i0+62
+template <bool CD>
+__global__ __launch_bounds__(256) void k_uw_face_x(FlowArgs a, UwArgs u, int first, int last) {
+  const int n = a.n[0];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, 0);
+  const int segs = (n + 1 + 62) / 63;
+  const int lane = threadIdx.x & 63;
+  const long long w = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+  const long long L = w / segs;
+  if (L >= NL) return;  // whole wavefronts only
+  const int sg = (int)(w - L * segs);
+  const int i = sg * 63 - 1 + lane;
+  const bool ok = i <= n;
+  int c3[3];
+  const long long o0 = line_base(a, 0, L, c3);
+  double Fh[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  if (ok) face_flux<CD>(u, L * NR + (i + 4), 1, face_shock(u, o0, 1, i, n), Fh);
+  double prev[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) prev[m] = __shfl_up(Fh[m], 1, 64);
+  if (ok && lane >= 1) face_update(a, o0 + i, Fh, prev, first, last);
+}
+
+// y- and z-lines: one lane per line and a run of kRun points
+template <bool CD>
+__global__ __launch_bounds__(256) void k_uw_face_yz(FlowArgs a, UwArgs u, int dir, int first, int last) {
+  const int n = a.n[dir];
+  const long long NL = uw_lines(a.n, dir);
+  const int nruns = (n + 1 + kRun - 1) / kRun;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= NL * nruns) return;
+  const int run = (int)(idx / NL);
+  const long long L = idx - (long long)run * NL;
+  const int r0 = run * kRun, r1 = min(r0 + kRun - 1, n);
+  int c3[3];
+  const long long o0 = line_base(a, dir, L, c3);
+  const long long st = dir_stride(a, dir);
+  double prev[5], cur[5];
+  face_flux<CD>(u, (long long)(r0 - 1 + 4) * NL + L, NL, face_shock(u, o0, st, r0 - 1, n), prev);
+  for (int t = r0; t <= r1; t++) {
+    face_flux<CD>(u, (long long)(t + 4) * NL + L, NL, face_shock(u, o0, st, t, n), cur);
+    face_update(a, o0 + (long long)t * st, cur, prev, first, last);
+#pragma unroll
+    for (int m = 0; m < 5; m++) prev[m] = cur[m];
+  }
+}
+
+// the stored pressure at the neighbour s = +-1 along d of point c3 (o), or
+// the pressure q2fvar gives its halo copy
+__device__ __forceinline__ double prs_nb(const FlowArgs& a, const int* c3, long long o, int d, int s) {
+  const int n = a.n[d];
+  const int c = c3[d] + s;
+  const long long st = dir_stride(a, d);
+  if (c >= 0 && c <= n) return a.pr[4 * a.fs + o + s * st];
+  const int rs = c < 0 ? c + n : c - n;
+  double q[5];
+  halo_q_at(a, d, c3, rs, o + (long long)(rs - c3[d]) * st, q);
+  return pressure_of(a, q[0], q[1], q[2], q[3], q[4]);
+}
+
+// ducrossensor's ratio (commcal.F90:226-267) at every computed point
+__global__ __launch_bounds__(64) void k_ssf(FlowArgs a, UwArgs u) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i > a.n[0]) return;
+  const int c3[3] = {i, (int)blockIdx.y, (int)blockIdx.z};
+  const long long o = (long long)c3[2] * a.plane + (long long)c3[1] * a.pitch + i;
+  const long long fs = a.fs;
+  double g[3][3];
+#pragma unroll
+  for (int m = 0; m < 3; m++)
+#pragma unroll
+    for (int d = 0; d < 3; d++) g[m][d] = d < a.ndir ? a.dv[(m + 3 * d) * fs + o] : 0.0;
+  const double dsum = g[0][0] + g[1][1] + g[2][2];
+  const double div2 = dsum * dsum;
+  const double vx = g[2][1] - g[1][2];
+  const double vy = g[0][2] - g[2][0];
+  const double vz = g[1][0] - g[0][1];
+  const double vort = vx * vx + vy * vy + vz * vz;
+  const double p0 = a.pr[4 * fs + o];
+  double dmax = 0.0;
+  for (int d = 0; d < a.ndir; d++) {
+    const double pp = prs_nb(a, c3, o, d, 1), pm = prs_nb(a, c3, o, d, -1);
+    const double r = fabs(pp - 2.0 * p0 + pm) / (pp + 2.0 * p0 + pm);
+    dmax = d == 0 ? r : fmax(dmax, r);
+  }
+  u.ssf[o] = div2 / (div2 + vort + 1.0e-30) * dmax;
+}
+
+__device__ __forceinline__ int wrapn(int t, int n) { return t < 0 ? t + n : (t > n ? t - n : t); }
+
+// the shock flags: ssf's maximum over offsets -4..5 along each direction
+// (periodic copies) against shkcrt (commcal.F90:273-306)
+__global__ __launch_bounds__(64) void k_lshock(FlowArgs a, UwArgs u) {
+  const int i = blockIdx.x * 64 + threadIdx.x;
+  if (i > a.n[0]) return;
+  const int j = blockIdx.y, k = blockIdx.z;
+  const long long o = (long long)k * a.plane + (long long)j * a.pitch + i;
+  double mx = 0.0;
+  for (int s = -4; s <= 5; s++) {
+    mx = fmax(mx, u.ssf[o + (wrapn(i + s, a.n[0]) - i)]);
+    mx = fmax(mx, u.ssf[o + (long long)(wrapn(j + s, a.n[1]) - j) * a.pitch]);
+    if (a.ndir == 3) mx = fmax(mx, u.ssf[o + (long long)(wrapn(k + s, a.n[2]) - k) * a.plane]);
+  }
+  u.lsh[o] = mx > u.shkcrt ? 1 : 0;
+}
+
+}  // namespace
+
+hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st) {
+  const dim3 g((unsigned)((a.n[0] + 64) / 64), (unsigned)(a.n[1] + 1), (unsigned)(a.n[2] + 1));
+  hipLaunchKernelGGL(k_ssf, g, dim3(64), 0, st, a, u);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_lshock, g, dim3(64), 0, st, a, u);
+  return hipGetLastError();
+}
+
+hipError_t uw_launch_dir(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last, hipStream_t st) {
+  const long long np = uw_points(a.n, dir);
+  hipLaunchKernelGGL(k_uw_split, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, a, u, dir);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const long long NL = uw_lines(a.n, dir);
+  if (dir == 0) {
+    const long long waves = NL * ((a.n[0] + 1 + 62) / 63);
+    const dim3 g((unsigned)((waves + 3) / 4));
+    if (u.chardec) hipLaunchKernelGGL(k_uw_face_x<true>, g, dim3(256), 0, st, a, u, (int)first, (int)last);
+    else hipLaunchKernelGGL(k_uw_face_x<false>, g, dim3(256), 0, st, a, u, (int)first, (int)last);
+  } else {
+    const long long nt = NL * ((a.n[dir] + 1 + kRun - 1) / kRun);
+    const dim3 g((unsigned)((nt + 255) / 256));
+    if (u.chardec)
+      hipLaunchKernelGGL(k_uw_face_yz<true>, g, dim3(256), 0, st, a, u, dir, (int)first, (int)last);
+    else
+      hipLaunchKernelGGL(k_uw_face_yz<false>, g, dim3(256), 0, st, a, u, dir, (int)first, (int)last);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace astr

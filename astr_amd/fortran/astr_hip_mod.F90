@@ -156,6 +156,16 @@ module astr_hip_capi
       type(c_ptr), value :: ctx, jacob, dxi
       integer(c_int) :: astr_hip_set_metrics
     end function
+    ! the convective scheme: central 643c (upwind = 0) or the explicit
+    ! upwind-biased one (convrsduwd, src/solver.F90:548-1201)
+    function astr_hip_set_convection(ctx, upwind, recon_schem, lchardecomp, bfacmpld, shkcrt) &
+        bind(C, name='astr_hip_set_convection')
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: upwind, recon_schem, lchardecomp
+      real(c_double), value :: bfacmpld, shkcrt
+      integer(c_int) :: astr_hip_set_convection
+    end function
     ! the main path's stored primitives rho, vel(...,1:3), prs, tmp
     function astr_hip_set_fvar(ctx, rho, vel, prs, tmp, hm) bind(C, name='astr_hip_set_fvar')
       import :: c_int, c_ptr

@@ -1,8 +1,10 @@
 !-----------------------------------------------------------------------
 ! astr_hip_run.F90 -- the main solver's run ("astr run", src/astr.F90:39-93)
 ! on the device, for the cases this build's main-path semantics cover: a
-! periodic box (flowtype tgv, 3-D or 2-D, and 2dvort), scheme 643c, rk3,
-! the compact filter, no species, no turbulence model.
+! periodic box (flowtype tgv, 3-D or 2-D, and 2dvort), the convective
+! scheme 643c or an explicit upwind-biased one (odd first digit, suffix 'e',
+! with recon_schem, lchardecomp, bfacmpld, shkcrt), diffusion 643c, rk3, the
+! compact filter, no species, no turbulence model.
 !
 ! module astr_hip_main   the reference's module state (commvar, commarray)
 !                        and subroutines under their own names: readinput,
@@ -34,6 +36,10 @@ module astr_hip_main
   real(8) :: const1, const2, const5, const6, tempconst
   real(8) :: roinf, uinf, vinf, winf, tinf, pinf
   character(len=512) :: contfile = 'datin/controller'
+  ! the convective scheme (readwrite.F90:541-546)
+  logical :: lupwind = .false., lchardecomp = .false.
+  integer :: recon_schem = 0
+  real(8) :: bfacmpld = 0.3d0, shkcrt = 0.05d0
 
   ! commarray (src/commarray.F90:273-285), the computed points 0..n
   real(8), allocatable, target :: x(:,:,:,:), jacob(:,:,:), dxi(:,:,:,:,:)
@@ -59,10 +65,8 @@ contains
     character(len=64) :: gridfile
     character(len=16) :: conschm, dschm, rkscheme, turbmode, iomode
     character(len=255) :: lineread
-    logical :: lihomo, ljhomo, lkhomo, nondimen, lfilter, lreadgrid, lfftk, limmbou, ltimrpt, &
-               lchardecomp
-    integer :: fh, n, kcutoff, recon_schem, num_species, ninit, bctype(6)
-    real(8) :: bfacmpld, shkcrt
+    logical :: lihomo, ljhomo, lkhomo, nondimen, lfilter, lreadgrid, lfftk, limmbou, ltimrpt
+    integer :: fh, n, kcutoff, num_species, ninit, bctype(6), nconv, ios
 
     open(newunit=fh, file=trim(inputfile), action='read', status='old')
     read(fh, '(////)')
@@ -105,8 +109,18 @@ contains
     if (any(bctype /= 1)) call input_stop('boundary types other than periodic (1)')
     if (.not. lfilter) call input_stop('lfilter = f')
     if (lreadgrid .or. lfftk .or. limmbou) call input_stop('lreadgrid, lfftz or limmbou = t')
-    if (trim(conschm) /= '643c' .or. trim(dschm) /= '643c') &
-      call input_stop('schemes '//trim(conschm)//', '//trim(dschm))
+    ! rhscal's choice (solver.F90:210-236): an even first digit is the
+    ! central scheme, an odd one the upwind-biased scheme of suffix 'e'
+    read(conschm(1:1), '(I1)', iostat=ios) nconv
+    if (ios /= 0 .or. len_trim(conschm) /= 4) call input_stop('convective scheme '//trim(conschm))
+    if (mod(nconv, 2) == 0) then
+      if (trim(conschm) /= '643c') call input_stop('central scheme '//trim(conschm))
+      lupwind = .false.
+    else
+      if (conschm(4:4) /= 'e') call input_stop('compact upwind-biased scheme '//trim(conschm))
+      lupwind = .true.
+    end if
+    if (trim(dschm) /= '643c') call input_stop('diffusion scheme '//trim(dschm))
     if (trim(rkscheme) /= 'rk3') call input_stop('time scheme '//trim(rkscheme))
     if (num_species /= 0) call input_stop('species (num_species > 0)')
     if (trim(turbmode) /= 'none') call input_stop('turbmode '//trim(turbmode))
@@ -211,6 +225,12 @@ contains
     cfg%lz = merge(x(0, 0, km, 3) - x(0, 0, 0, 3), 1.d0, km > 0)
     cfg%deltat = deltat
     call hipcheck(astr_hip_create(cfg, ctx), 'solvrinit')
+    if (lupwind) then
+      call hipcheck(astr_hip_set_convection(ctx, 1, recon_schem, merge(1, 0, lchardecomp), bfacmpld, &
+                                            shkcrt), 'solvrinit: convective scheme')
+      write(*, '(A,I0,A,L1)') '  ** upwind-biased convection, recon_schem ', recon_schem, &
+        ', lchardecomp ', lchardecomp
+    end if
     print *, ' ** solver initialised on the device'
   end subroutine solvrinit
 

@@ -195,6 +195,24 @@ int astr_hip_set_metrics(astr_hip_ctx *ctx, const double *jacob, const double *d
  * applied. */
 int astr_hip_set_boundary(astr_hip_ctx *ctx, const int *npdc);
 
+/* Main path: the convective scheme (rhscal, src/solver.F90:208-237).
+ * upwind = 0: the central compact scheme (convrsdcal6, conschm 643c);
+ * upwind = 1: the explicit upwind-biased scheme (conschm with an odd first
+ * digit and suffix 'e'; convrsduwd, src/solver.F90:548-1201): Steger-Warming
+ * splitting (src/riemann.F90:23-166), reconstruction recon_schem = -1, 0
+ * SUW7, 1 WENO7, 2 WENO7-Z, 3 MP7, 5 MP7-LD (bandwidth factor bfacmpld),
+ * 6 ROUND (recons_exp, src/flux.F90:269-383), characteristic decomposition
+ * where the shock sensor flags an interface when lchardecomp
+ * (chardecomp, :1958-2162), and the Ducros sensor with threshold shkcrt
+ * when recon_schem = 5 or lchardecomp (ducrossensor,
+ * src/commcal.F90:196-360) -- the input line "recon_schem, lchardecomp,
+ * bfacmpld, shkcrt" of the reference's input file (readwrite.F90:546).
+ * Every direction must be periodic (npdc 3); the scheme is kept until
+ * changed.  The compact-upwind scheme (suffix 'c', convrsdcmp) is not
+ * provided. */
+int astr_hip_set_convection(astr_hip_ctx *ctx, int upwind, int recon_schem, int lchardecomp,
+                            double bfacmpld, double shkcrt);
+
 /* Per-pass timing with HIP events on the context's stream (no reference
  * counterpart; the reference's timerept accumulators, src/utility.F90:27-297,
  * are no-ops).  Slot = pass*3 + direction, pass 0..8 = convective+gradient

@@ -102,6 +102,15 @@ void or_main_gradcal(or_main *s);
 void or_main_rhscal(or_main *s);
 void or_main_stats(or_main *s, double *ke, double *ens, double *diss);
 void or_main_rk3_steps(or_main *s, int nsteps, double *ke, double *ens, double *diss);
+int or_main_set_conv(or_main *s, int upwind, int recon, int chardec, double bfac, double shkcrt);
+double *or_main_ssf(or_main *s);              /* N */
+unsigned char *or_main_lshock(or_main *s);    /* N */
+
+/* pointwise pieces of the upwind-biased convection (astr_oracle_upwind.c) */
+void or_sw_split(double gamma, double mach, double jac, const double *dxi, const double *q,
+                 const double *pv, double *fp, double *fm);
+double or_recons(int scheme, const double *f, double bfac, int shock);
+int or_chardecomp(double gamma, const double *l, const double *r, double *rev, double *lev);
 
 /* counter-based noise shared with the device path (not a reference routine) */
 double or_gauss(uint64_t seed, uint32_t i, uint32_t j, uint32_t k, uint32_t c);

@@ -74,6 +74,12 @@ struct or_main {
   msys ds[3], fs[3];
   double c10[6], ch[33];
   int npdc[3];       /* boundary type per direction: 3 periodic, 4 physical at both ends */
+  /* convection: central (convrsdcal6) or upwind-biased (convrsduwd) with
+   * recon_schem, lchardecomp, bfacmpld, shkcrt (readwrite.F90:546) */
+  int upwind, recon, chardec;
+  double bfac, shkcrt;
+  double *ssf;              /* N: Ducros sensor */
+  unsigned char *lshock;    /* N */
 };
 
 static inline long H(const or_main *s, int i, int j, int k) {
@@ -148,7 +154,7 @@ void or_main_set_npdc(or_main *s, const int *npdc) {
 void or_main_destroy(or_main *s) {
   if (!s) return;
   free(s->q); free(s->pv); free(s->jac); free(s->dxi); free(s->qrhs); free(s->qsave);
-  free(s->dvel); free(s->dtmp); free(s->sigma); free(s->qflux);
+  free(s->dvel); free(s->dtmp); free(s->sigma); free(s->qflux); free(s->ssf); free(s->lshock);
   for (int d = 0; d < s->ndir; d++) { free(s->ds[d].ac); free(s->fs[d].ac); }
   free(s);
 }
@@ -160,6 +166,32 @@ double *or_main_primsh(or_main *s) { return s->pv; }
 double *or_main_qrhs(or_main *s) { return s->qrhs; }
 double *or_main_dvel(or_main *s) { return s->dvel; }
 double *or_main_dtmp(or_main *s) { return s->dtmp; }
+double *or_main_ssf(or_main *s) { return s->ssf; }
+unsigned char *or_main_lshock(or_main *s) { return s->lshock; }
+
+/* The convective scheme (rhscal, solver.F90:208-237): central (upwind = 0)
+ * or the explicit upwind-biased one (conschm with an odd first digit and
+ * suffix 'e', upwind = 1) with recon_schem, lchardecomp, bfacmpld, shkcrt.
+ * The shock sensor runs when recon = 5 or chardec (solver.F90:223).
+ * Returns -1 for a combination the restatement does not cover: a
+ * reconstruction the reference does not define, or a direction that is not
+ * periodic. */
+int or_main_set_conv(or_main *s, int upwind, int recon, int chardec, double bfac, double shkcrt) {
+  if (upwind) {
+    if (!(recon == -1 || recon == 0 || recon == 1 || recon == 2 || recon == 3 || recon == 5 || recon == 6))
+      return -1;
+    for (int d = 0; d < s->ndir; d++)
+      if (s->npdc[d] != 3) return -1;
+  }
+  s->upwind = upwind;
+  s->recon = recon;
+  s->chardec = chardec;
+  s->bfac = bfac;
+  s->shkcrt = shkcrt;
+  if (!s->ssf) s->ssf = calloc(s->N, sizeof(double));
+  if (!s->lshock) s->lshock = calloc(s->N, 1);
+  return 0;
+}
 
 /* periodic index of a halo cell (dataswap, single rank: var(-h) = var(n-h),
  * var(n+h) = var(h)) */
@@ -335,7 +367,10 @@ static void slab_q2fvar(or_main *s, int dir) {
 
 /* qswap, single rank, periodic: parallel.F90:4876-4920 (i), 5018-5060 (j),
  * 5159-5200 (k): halo copy, then q(0) = q(n) = (q(0) + q(n))/2, then q2fvar
- * of the slabs n..n+hm and -hm..0 */
+ * of the slabs n..n+hm and -hm..0.  A 2-D case (ka = 0, k homogeneous) still
+ * takes the k step (parallel.F90:5159-5170, 5184-5196): the k halo planes
+ * copy k = 0 and the slabs -hm..0 and 0..hm hold the plane k = 0 itself, so
+ * q2fvar refreshes the primitives of every computed point. */
 void or_main_qswap(or_main *s) {
   const long NH = s->NH;
   for (int dir = 0; dir < s->ndir; dir++) {
@@ -356,6 +391,7 @@ void or_main_qswap(or_main *s) {
         }
     slab_q2fvar(s, dir);
   }
+  if (s->ndir == 2) or_main_updatefvar(s);
 }
 
 /* gradcal: comsolver.F90:244-497 (stored vel, tmp; dvel(m,n) accumulates
@@ -523,12 +559,165 @@ static void diffrsdcal6(or_main *s) {
   }
 }
 
+/* ducrossensor (commcal.F90:196-360), single rank, periodic: the Ducros
+ * sensor ssf = div^2 / (div^2 + |omega|^2 + 1e-30) * max_d |p(+1) - 2 p +
+ * p(-1)| / (p(+1) + 2 p + p(-1)) from gradcal's dvel and the stored
+ * pressure with its halos, then lshock = the maximum of ssf over the cross
+ * of offsets -hm+1..hm along each direction (periodic copies) > shkcrt.  In
+ * 2-D the k halos hold the plane itself (qswap's k step), so the k term of
+ * the pressure ratio is 0 and the k offsets add nothing. */
+static void ducros(or_main *s) {
+  const long N = s->N, NH = s->NH;
+  const double *prs = s->pv + 4 * NH;
+  const long st[3] = {1, s->ex, s->ex * s->ey};
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k <= s->n[2]; k++)
+    for (int j = 0; j <= s->n[1]; j++)
+      for (int i = 0; i <= s->n[0]; i++) {
+        const long p = P0(s, i, j, k), h = H(s, i, j, k);
+#define DV(m, n_) s->dvel[((m) + 3 * (n_)) * N + p]
+        const double dsum = DV(0, 0) + DV(1, 1) + DV(2, 2);
+        const double div2 = dsum * dsum;
+        const double vx = DV(2, 1) - DV(1, 2);
+        const double vy = DV(0, 2) - DV(2, 0);
+        const double vz = DV(1, 0) - DV(0, 1);
+#undef DV
+        const double vort = vx * vx + vy * vy + vz * vz;
+        double dmax = 0.0;
+        for (int d = 0; d < s->ndir; d++) {
+          const double pp = prs[h + st[d]], p0 = prs[h], pm = prs[h - st[d]];
+          const double r = fabs(pp - 2.0 * p0 + pm) / (pp + 2.0 * p0 + pm);
+          dmax = d == 0 ? r : fmax(dmax, r);
+        }
+        s->ssf[p] = div2 / (div2 + vort + 1.0e-30) * dmax;
+      }
+#pragma omp parallel for collapse(2)
+  for (int k = 0; k <= s->n[2]; k++)
+    for (int j = 0; j <= s->n[1]; j++)
+      for (int i = 0; i <= s->n[0]; i++) {
+        double mx = 0.0;
+        for (int o = -HM + 1; o <= HM; o++) {
+          mx = fmax(mx, s->ssf[P0(s, wrapn(i + o, s->n[0]), j, k)]);
+          mx = fmax(mx, s->ssf[P0(s, i, wrapn(j + o, s->n[1]), k)]);
+          if (s->ndir == 3) mx = fmax(mx, s->ssf[P0(s, i, j, wrapn(k + o, s->n[2]))]);
+        }
+        s->lshock[P0(s, i, j, k)] = mx > s->shkcrt;
+      }
+}
+
+/* convrsduwd (solver.F90:548-1201), single rank, periodic (npdc = 3: the
+ * split over -hm..n+hm, interfaces i = -1..n, iwind8 never clamps), no
+ * crinod (hdiss false): per line the Steger-Warming split of every point,
+ * per interface the windows Flcp(m, n) = fplus(i+n-4), Flcm(m, n) =
+ * fminus(i+5-n), n = 1..8, optionally projected on the Roe-state
+ * characteristics (lchardecomp where the interface is flagged), each
+ * reconstructed by recons_exp, summed, projected back, and qrhs(i) =
+ * qrhs(i) + Fh(i) - Fh(i-1) */
+static void convrsduwd(or_main *s) {
+  const long N = s->N, NH = s->NH;
+  const int sson = s->recon == 5 || s->chardec;
+  for (int dir = 0; dir < s->ndir; dir++) {
+    const int n = s->n[dir];
+    const long st = stride_of(s, dir);
+    const int d1 = dir == 0 ? 1 : 0, d2 = dir == 2 ? 1 : 2;
+#pragma omp parallel for collapse(2)
+    for (int b = 0; b <= s->n[d2]; b++)
+      for (int a = 0; a <= s->n[d1]; a++) {
+        const int L = n + 2 * HM + 1;
+        double *fp = malloc(sizeof(double) * (10 * L + 5 * (n + 2)));
+        double *fm = fp + 5 * L, *Fh = fm + 5 * L; /* fp[m L + t + HM], Fh[m (n+2) + i + 1] */
+        int ijk[3] = {0, 0, 0};
+        ijk[d1] = a;
+        ijk[d2] = b;
+        const long h0 = H(s, ijk[0], ijk[1], ijk[2]);
+        for (int t = -HM; t <= n + HM; t++) {
+          const long h = h0 + t * st;
+          double q[5], pv[6], dx[3], a5[5], b5[5];
+          for (int m = 0; m < 5; m++) q[m] = s->q[m * NH + h];
+          for (int m = 0; m < 6; m++) pv[m] = s->pv[m * NH + h];
+          for (int nn = 0; nn < 3; nn++) dx[nn] = s->dxi[(dir + 3 * nn) * NH + h];
+          or_sw_split(s->c.gamma, s->c.mach, s->jac[h], dx, q, pv, a5, b5);
+          for (int m = 0; m < 5; m++) {
+            fp[m * L + t + HM] = a5[m];
+            fm[m * L + t + HM] = b5[m];
+          }
+        }
+        for (int i = -1; i <= n; i++) {
+          int lsh = 1;
+          if (sson) {
+            int p3[3] = {ijk[0], ijk[1], ijk[2]};
+            p3[dir] = i < 0 ? i + 1 : i;
+            lsh = s->lshock[P0(s, p3[0], p3[1], p3[2])];
+            if (i >= 0 && i + 1 <= n) {
+              p3[dir] = i + 1;
+              lsh = lsh || s->lshock[P0(s, p3[0], p3[1], p3[2])];
+            }
+          }
+          const int cd = s->chardec && lsh;
+          double lev[25], rev[25];
+          if (cd) {
+            double sl[9], sr[9];
+            for (int side = 0; side < 2; side++) {
+              const long h = h0 + (i + side) * st;
+              double *o = side == 0 ? sl : sr;
+              o[0] = s->pv[h];
+              o[1] = s->pv[NH + h];
+              o[2] = s->pv[2 * NH + h];
+              o[3] = s->pv[3 * NH + h];
+              o[4] = s->pv[4 * NH + h];
+              o[5] = s->q[4 * NH + h];
+              for (int nn = 0; nn < 3; nn++) o[6 + nn] = s->dxi[(dir + 3 * nn) * NH + h];
+            }
+            or_chardecomp(s->c.gamma, sl, sr, rev, lev);
+          }
+          double Fhc[5];
+          for (int m = 0; m < 5; m++) {
+            double wp[8], wm[8];
+            for (int k = 1; k <= 8; k++) {
+              const int np_ = i + k - 4 + HM, nm_ = i + 5 - k + HM;
+              if (cd) {
+                wp[k - 1] = lev[5 * m] * fp[np_] + lev[5 * m + 1] * fp[L + np_] + lev[5 * m + 2] * fp[2 * L + np_] +
+                            lev[5 * m + 3] * fp[3 * L + np_] + lev[5 * m + 4] * fp[4 * L + np_];
+                wm[k - 1] = lev[5 * m] * fm[nm_] + lev[5 * m + 1] * fm[L + nm_] + lev[5 * m + 2] * fm[2 * L + nm_] +
+                            lev[5 * m + 3] * fm[3 * L + nm_] + lev[5 * m + 4] * fm[4 * L + nm_];
+              } else {
+                wp[k - 1] = fp[m * L + np_];
+                wm[k - 1] = fm[m * L + nm_];
+              }
+            }
+            const double v1 = or_recons(s->recon, wp, s->bfac, lsh);
+            const double v2 = or_recons(s->recon, wm, s->bfac, lsh);
+            Fhc[m] = v1 + v2;
+          }
+          for (int m = 0; m < 5; m++)
+            Fh[m * (n + 2) + i + 1] =
+                cd ? rev[5 * m] * Fhc[0] + rev[5 * m + 1] * Fhc[1] + rev[5 * m + 2] * Fhc[2] +
+                         rev[5 * m + 3] * Fhc[3] + rev[5 * m + 4] * Fhc[4]
+                   : Fhc[m];
+        }
+        for (int t = 0; t <= n; t++) {
+          int p3[3] = {ijk[0], ijk[1], ijk[2]};
+          p3[dir] = t;
+          const long p = P0(s, p3[0], p3[1], p3[2]);
+          for (int m = 0; m < 5; m++)
+            s->qrhs[m * N + p] = s->qrhs[m * N + p] + Fh[m * (n + 2) + t + 1] - Fh[m * (n + 2) + t];
+        }
+        free(fp);
+      }
+  }
+}
+
 /* rhscal: solver.F90:185-282 after time_integration_rk's qrhs = 0
  * (mainloop.F90:409); gradcal must have run */
 void or_main_rhscal(or_main *s) {
   const long N = s->N;
   memset(s->qrhs, 0, sizeof(double) * 5 * N);
-  convrsdcal6(s);
+  if (s->upwind) {
+    if (s->recon == 5 || s->chardec) ducros(s);
+    convrsduwd(s);
+  } else {
+    convrsdcal6(s);
+  }
   for (long e = 0; e < 5 * N; e++) s->qrhs[e] = -s->qrhs[e];
   diffrsdcal6(s);
 }

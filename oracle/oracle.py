@@ -96,6 +96,17 @@ def lib():
             getattr(L, f).argtypes = [C.c_void_p]
         L.or_main_stats.argtypes = [C.c_void_p, _dp, _dp, _dp]
         L.or_main_rk3_steps.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp]
+        L.or_main_set_conv.restype = C.c_int
+        L.or_main_set_conv.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+        L.or_main_ssf.restype = _dp
+        L.or_main_ssf.argtypes = [C.c_void_p]
+        L.or_main_lshock.restype = C.POINTER(C.c_ubyte)
+        L.or_main_lshock.argtypes = [C.c_void_p]
+        L.or_sw_split.argtypes = [C.c_double, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp]
+        L.or_recons.restype = C.c_double
+        L.or_recons.argtypes = [C.c_int, _dp, C.c_double, C.c_int]
+        L.or_chardecomp.restype = C.c_int
+        L.or_chardecomp.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -311,6 +322,53 @@ class MainState:
         ke, en, di = np.zeros(nsteps), np.zeros(nsteps), np.zeros(nsteps)
         lib().or_main_rk3_steps(self.h, nsteps, _p(ke), _p(en), _p(di))
         return ke, en, di
+
+    def set_conv(self, upwind, recon=0, chardecomp=False, bfacmpld=0.3, shkcrt=0.05):
+        """Convective scheme: central (upwind False) or the explicit
+        upwind-biased one with recon_schem, lchardecomp, bfacmpld, shkcrt."""
+        rc = lib().or_main_set_conv(self.h, int(bool(upwind)), int(recon), int(bool(chardecomp)),
+                                    float(bfacmpld), float(shkcrt))
+        if rc:
+            raise ValueError("convection scheme not covered by the restatement")
+
+    @property
+    def ssf(self):
+        """The Ducros sensor of the last right-hand side, shape."""
+        return self._view("or_main_ssf", 1, self.shape)[0]
+
+    @property
+    def lshock(self):
+        """The shock flags of the last right-hand side (bool copy), shape."""
+        ptr = lib().or_main_lshock(self.h)
+        return np.ctypeslib.as_array(ptr, shape=(self.N,)).reshape(self.shape).astype(bool)
+
+
+def sw_split(gamma, mach, jac, dxi, q, prims):
+    """flux_steger_warming at one point: (fplus (5,), fminus (5,))."""
+    dxi = np.ascontiguousarray(dxi, dtype=np.float64)
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    prims = np.ascontiguousarray(prims, dtype=np.float64)
+    fp, fm = np.zeros(5), np.zeros(5)
+    lib().or_sw_split(gamma, mach, jac, _p(dxi), _p(q), _p(prims), _p(fp), _p(fm))
+    return fp, fm
+
+
+def recons(scheme, f, bfac=0.3, shock=True):
+    """recons_exp's periodic branch on the 8-point window f(1:8)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    assert f.shape == (8,)
+    return lib().or_recons(int(scheme), _p(f), float(bfac), int(bool(shock)))
+
+
+def chardecomp(gamma, left, right):
+    """chardecomp between two points, each (rho, u, v, w, p, E, dxi row):
+    (REV, LEV), 5 x 5."""
+    l = np.ascontiguousarray(left, dtype=np.float64)
+    r = np.ascontiguousarray(right, dtype=np.float64)
+    rev, lev = np.zeros(25), np.zeros(25)
+    if lib().or_chardecomp(gamma, _p(l), _p(r), _p(rev), _p(lev)):
+        raise ValueError("zero metric row")
+    return rev.reshape(5, 5), lev.reshape(5, 5)
 
 
 def qswap(q):

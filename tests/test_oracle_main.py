@@ -174,6 +174,26 @@ def test_main_qswap_order_and_primitives():
     assert np.array_equal(ph[:, k + HM, 0 + HM, -3 + HM], pri[:, k, 0, i_src])
 
 
+def test_main_qswap_2d_refreshes_every_point():
+    """A 2-D case still takes qswap's k step (parallel.F90:5159-5196): the
+    slabs -hm..0 and 0..hm of km = 0 are the plane itself, so after qswap the
+    stored primitives of every point are q2fvar of the averaged q, where in
+    3-D only the boundary planes are."""
+    cfg = O.tgv_cfg(14, 12, 0, dt=1e-3)
+    q = periodic_field(cfg, 5)
+    s = O.MainState(cfg)
+    s.set_q(q)
+    s.filterq()
+    qf, pr_stale = s.get()
+    s.qswap()
+    qa, pra = s.get()
+    fresh = O.MainState(cfg)
+    fresh.set_q(qa)
+    _, prf = fresh.get()
+    assert np.array_equal(pra, prf)
+    assert not np.array_equal(pra[:, :, 3:-3, 3:-3], pr_stale[:, :, 3:-3, 3:-3])
+
+
 def test_main_gradcal_reads_stored_primitives():
     """gradcal differentiates the stored primitives, not those of the current
     q: after filterq (which changes q only) its interior gradients equal those
