@@ -62,9 +62,11 @@ EXPORTS = [
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
           5: (8, 64, 5), 6: (4, 32, 9), 7: (2, 64, 5),
           8: (1, 64, 5), 9: (2, 32, 9), 10: (16, 16, 9), 11: (8, 64, 9), 12: (1, 64, 9)}
-PROF_SLOTS = 27
+PROF_SLOTS = 36
 PASS_NAMES = ["gc_first", "gc_mid", "gc_last", "visc_mid", "visc_last_rhs",
               "visc_last_rk3_filter", "filter", "gc_last_grad", "gc_last_conv"]
+# the profile slots after the passes: the upwind-biased convection's kernels
+PROF_NAMES = PASS_NAMES + ["uw_split", "uw_face", "uw_sensor"]
 
 _lib = None
 
@@ -341,7 +343,7 @@ class Flow:
         out = {}
         for k in range(PROF_SLOTS):
             if cnt[k]:
-                out[(PASS_NAMES[k // 3], "xyz"[k % 3])] = (float(ms[k]), int(cnt[k]))
+                out[(PROF_NAMES[k // 3], "xyz"[k % 3])] = (float(ms[k]), int(cnt[k]))
         return out
 
     def sync(self):

@@ -669,6 +669,21 @@ dim3 dense_grid(const FlowArgs& a, int hm) {
 }
 
 // launch one pass; with profiling on, bracket it with events on the stream
+// one launch (or launch sequence) timed into profile slot `slot` when the
+// context is profiling
+template <class F>
+int timed(astr_hip_ctx* c, int slot, F&& launch) {
+  const bool rec = c->prof && c->ev_used < (int)c->ev_kind.size();
+  if (rec) HIP_OK(hipEventRecord(c->ev[2 * c->ev_used], c->stream));
+  HIP_OK(launch());
+  if (rec) {
+    HIP_OK(hipEventRecord(c->ev[2 * c->ev_used + 1], c->stream));
+    c->ev_kind[c->ev_used++] = slot;
+  }
+  return 0;
+}
+constexpr int kSlotUwSplit = 27, kSlotUwFace = 30, kSlotUwSensor = 33;
+
 int pass(astr_hip_ctx* c, int prog, int d) {
   const bool rec = c->prof && c->ev_used < (int)c->ev_kind.size();
   if (rec) HIP_OK(hipEventRecord(c->ev[2 * c->ev_used], c->stream));
@@ -702,8 +717,12 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* 
   if (a.uwd) {
     // rhscal's upwind branch (solver.F90:219-226): ducrossensor when
     // recon_schem = 5 or lchardecomp, then convrsduwd along i, j, k
-    if (sensor) HIP_OK(uw_launch_sensor(a, c->uw, c->stream));
-    for (int d = 0; d < nd; d++) HIP_OK(uw_launch_dir(a, c->uw, d, d == 0, d == nd - 1, c->stream));
+    if (sensor && timed(c, kSlotUwSensor, [&] { return uw_launch_sensor(a, c->uw, c->stream); })) return -1;
+    for (int d = 0; d < nd; d++) {
+      if (timed(c, kSlotUwSplit + d, [&] { return uw_launch_split(a, c->uw, d, c->stream); })) return -1;
+      if (timed(c, kSlotUwFace + d, [&] { return uw_launch_face(a, c->uw, d, d == 0, d == nd - 1, c->stream); }))
+        return -1;
+    }
   }
   if (diss_out) {
     const long long npts = (long long)a.n[0] * a.n[1] * (a.n[2] > 0 ? a.n[2] : 1);

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "lines.hpp"
 
@@ -469,6 +470,15 @@ __device__ __forceinline__ double visc_col_full(const FlowArgs& a, int dir, int 
   return (s0 * M.r[0] + s1 * M.r[1] + s2 * M.r[2]) * M.j;
 }
 
+// element m of a 5-vector by selects: in a loop over m that the compiler
+// keeps rolled, indexing a register array by m would move the array to
+// scratch memory
+__device__ __forceinline__ double pick5(double a0, double a1, double a2, double a3, double a4, int m) {
+  return m == 0 ? a0 : (m == 1 ? a1 : (m == 2 ? a2 : (m == 3 ? a3 : a4)));
+}
+#define ASTR_QV(m, t) pick5(qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t], (m))
+#define ASTR_QH(m) pick5(qh[0], qh[1], qh[2], qh[3], qh[4], (m))
+
 // Main path: q of the halo copy of slot t.  qswap fills the halo of a line
 // from the filtered q; the lines on boundary planes that a later qswap step
 // averages take the values kept before that step (hx / hy), the others the
@@ -716,38 +726,52 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         ud[t] = (dir == 0 ? qv[1][t] : (dir == 1 ? qv[2][t] : qv[3][t])) / qv[0][t];
       }
     }
-#pragma unroll
-    for (int m = 0; m < 5; m++) {
+    // the main path's flux inputs of component MM: one copy per component
+    // behind a wave-uniform switch, so that q is indexed by constants while
+    // the loop over components (and its solve) stays rolled
+    auto stage_main = [&](auto mc) __attribute__((always_inline)) {
+      constexpr int MM = decltype(mc)::value;
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if constexpr (MAIN) {
-          // a periodic halo copy carries the source point's metrics, which are this point's
-          if (a.met) {
-            const unsigned o = S.g[t];
-            const MetRow M = met_row(a, dir, o);
-            const double u = ldg(a.pr + fs, o), v = ldg(a.pr + 2 * fs, o), w = ldg(a.pr + 3 * fs, o);
-            T[kLineOff + S.s[t]] = conv_flux_full(M, m, qv[m][t], qv[4][t], u, v, w, pp[t]);
-            if (S.h[t] != kDummy) {
-              double qh[5];
-              halo_q<C>(a, S, t, dir, qv, qh);
-              const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
-              T[kLineOff + S.h[t]] =
-                  conv_flux_full(M, m, qh[m], qh[4], qh[1] / qh[0], qh[2] / qh[0], qh[3] / qh[0], ph);
-            }
-          } else {
-            const Met M = met_of(a, dir);
-            T[kLineOff + S.s[t]] = conv_flux_main(M.j, M.d, m, dir, qv[m][t], qv[4][t], ud[t], pp[t]);
-            if (S.h[t] != kDummy) {
-              double qh[5];
-              halo_q<C>(a, S, t, dir, qv, qh);
-              const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
-              const double uh = qh[1 + dir] / qh[0];
-              T[kLineOff + S.h[t]] = conv_flux_main(M.j, M.d, m, dir, qh[m], qh[4], uh, ph);
-            }
+        // a periodic halo copy carries the source point's metrics, which are this point's
+        if (a.met) {
+          const unsigned o = S.g[t];
+          const MetRow M = met_row(a, dir, o);
+          const double u = ldg(a.pr + fs, o), v = ldg(a.pr + 2 * fs, o), w = ldg(a.pr + 3 * fs, o);
+          T[kLineOff + S.s[t]] = conv_flux_full(M, MM, qv[MM][t], qv[4][t], u, v, w, pp[t]);
+          if (S.h[t] != kDummy) {
+            double qh[5];
+            halo_q<C>(a, S, t, dir, qv, qh);
+            const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+            T[kLineOff + S.h[t]] =
+                conv_flux_full(M, MM, qh[MM], qh[4], qh[1] / qh[0], qh[2] / qh[0], qh[3] / qh[0], ph);
           }
         } else {
-          put<C>(T, S, t, conv_flux(m, dir, qv[m][t], qv[4][t], ud[t], pp[t]));
+          const Met M = met_of(a, dir);
+          T[kLineOff + S.s[t]] = conv_flux_main(M.j, M.d, MM, dir, qv[MM][t], qv[4][t], ud[t], pp[t]);
+          if (S.h[t] != kDummy) {
+            double qh[5];
+            halo_q<C>(a, S, t, dir, qv, qh);
+            const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+            const double uh = pick5(0.0, qh[1], qh[2], qh[3], 0.0, 1 + dir) / qh[0];
+            T[kLineOff + S.h[t]] = conv_flux_main(M.j, M.d, MM, dir, qh[MM], qh[4], uh, ph);
+          }
         }
+      }
+    };
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      if constexpr (MAIN) {
+        switch (m) {
+          case 0: stage_main(std::integral_constant<int, 0>{}); break;
+          case 1: stage_main(std::integral_constant<int, 1>{}); break;
+          case 2: stage_main(std::integral_constant<int, 2>{}); break;
+          case 3: stage_main(std::integral_constant<int, 3>{}); break;
+          default: stage_main(std::integral_constant<int, 4>{}); break;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < NPT; t++) put<C>(T, S, t, conv_flux(m, dir, ASTR_QV(m, t), qv[4][t], ud[t], pp[t]));
       }
       double* dst = a.qrhs + m * fs;
       double qr[NPT];

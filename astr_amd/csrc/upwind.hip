@@ -41,7 +41,9 @@ constexpr int kRun = 16;  // points per lane of a y- or z-line
 // pointwise pieces
 // ---------------------------------------------------------------------------
 
-// flux_steger_warming at one point (riemann.F90:45-162, nondimen)
+// flux_steger_warming at one point (riemann.F90:45-162, nondimen).  The
+// branch results are chosen value by value so that fp and fm stay in
+// registers.
 __device__ __forceinline__ void sw_split(double gamma, double mach, double jac, const double dx[3],
                                          const double q[5], const double pv[6], double fp[5],
                                          double fm[5]) {
@@ -54,16 +56,19 @@ __device__ __forceinline__ void sw_split(double gamma, double mach, double jac, 
   const double css = sqrt(tmp) / mach;
   const double csa = css / var0;
   const double lmach = uu / csa;
-  if (lmach >= 1.0 || lmach <= -1.0) {
-    double* f = lmach >= 1.0 ? fp : fm;
-    double* z = lmach >= 1.0 ? fm : fp;
-    f[0] = jac * q[0] * uu;
-    f[1] = jac * (q[1] * uu + dx[0] * prs);
-    f[2] = jac * (q[2] * uu + dx[1] * prs);
-    f[3] = jac * (q[3] * uu + dx[2] * prs);
-    f[4] = jac * (q[4] + prs) * uu;
+  const bool sup = lmach >= 1.0, sdown = lmach <= -1.0;
+  if (sup || sdown) {
+    double full[5];
+    full[0] = jac * q[0] * uu;
+    full[1] = jac * (q[1] * uu + dx[0] * prs);
+    full[2] = jac * (q[2] * uu + dx[1] * prs);
+    full[3] = jac * (q[3] * uu + dx[2] * prs);
+    full[4] = jac * (q[4] + prs) * uu;
 #pragma unroll
-    for (int m = 0; m < 5; m++) z[m] = 0.0;
+    for (int m = 0; m < 5; m++) {
+      fp[m] = sup ? full[m] : 0.0;
+      fm[m] = sup ? 0.0 : full[m];
+    }
     return;
   }
   const double l0 = uu, l3 = uu + csa, l4 = uu - csa;
@@ -77,17 +82,18 @@ __device__ __forceinline__ void sw_split(double gamma, double mach, double jac, 
     const double e0 = side == 0 ? p0 : l0 - p0;
     const double e3 = side == 0 ? p3 : l3 - p3;
     const double e4 = side == 0 ? p4 : l4 - p4;
-    double* f = side == 0 ? fp : fm;
     const double var1 = e0;
     const double var2 = e3 - e4;
     const double var3 = 2.0 * e0 - e3 - e4;
     const double var4 = var1 - var3 * gm2;
-    f[0] = jro * var4;
-    f[1] = jro * (var4 * u + var2 * css * gpd0 * gm2);
-    f[2] = jro * (var4 * v + var2 * css * gpd1 * gm2);
-    f[3] = jro * (var4 * w + var2 * css * gpd2 * gm2);
-    f[4] = jac * (var1 * q[4] +
-                  rho * (var2 * uu * var0 * css * gm2 - var3 * (fhi + css * css) * gm2 / (gamma - 1.0)));
+    const double f0 = jro * var4;
+    const double f1 = jro * (var4 * u + var2 * css * gpd0 * gm2);
+    const double f2 = jro * (var4 * v + var2 * css * gpd1 * gm2);
+    const double f3 = jro * (var4 * w + var2 * css * gpd2 * gm2);
+    const double f4 = jac * (var1 * q[4] + rho * (var2 * uu * var0 * css * gm2 -
+                                                  var3 * (fhi + css * css) * gm2 / (gamma - 1.0)));
+    if (side == 0) { fp[0] = f0; fp[1] = f1; fp[2] = f2; fp[3] = f3; fp[4] = f4; }
+    else { fm[0] = f0; fm[1] = f1; fm[2] = f2; fm[3] = f3; fm[4] = f4; }
   }
 }
 
@@ -215,18 +221,18 @@ __device__ __forceinline__ double round3(const double* f) {
   return g * (u3 - u1) + u1;
 }
 
-// recons_exp's periodic branch (flux.F90:305-335); the scheme is checked on
-// the host
-__device__ __forceinline__ double recons(int scheme, const double* f, double bfac, bool shock) {
-  switch (scheme) {
-    case -1: return f[3];
-    case 0: return suw7(f);
-    case 1: return weno7(f, false);
-    case 2: return weno7(f, true);
-    case 3: return mp7(f);
-    case 5: return mp7ld(f, bfac, shock);
-    default: return round3(f);
-  }
+// recons_exp's periodic branch (flux.F90:305-335) for scheme SCH (checked
+// on the host), one instantiation per scheme so that each kernel holds one
+// scheme's code
+template <int SCH>
+__device__ __forceinline__ double recons(const double* f, double bfac, bool shock) {
+  if constexpr (SCH == -1) return f[3];
+  else if constexpr (SCH == 0) return suw7(f);
+  else if constexpr (SCH == 1) return weno7(f, false);
+  else if constexpr (SCH == 2) return weno7(f, true);
+  else if constexpr (SCH == 3) return mp7(f);
+  else if constexpr (SCH == 5) return mp7ld(f, bfac, shock);
+  else return round3(f);
 }
 
 // chardecomp (solver.F90:1958-2162, no species); l, r = rho, u, v, w, p, E,
@@ -347,7 +353,9 @@ __device__ __forceinline__ void halo_q_at(const FlowArgs& a, int dir, const int*
 // kernels
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u, int dir) {
+template <int DIR>
+__global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
+  constexpr int dir = DIR;
   const int n = a.n[dir];
   const int NR = n + 9;
   const long long NL = uw_lines(a.n, dir);
@@ -416,7 +424,7 @@ __device__ __forceinline__ bool face_shock(const UwArgs& u, long long o0, long l
 
 // Fh of the interface between rows i and i+1; c0 = the split-buffer index
 // of row i, rs = the buffer stride between rows
-template <bool CD>
+template <int SCH, bool CD>
 __device__ __forceinline__ void face_flux(const UwArgs& u, long long c0, long long rs, bool lsh, double* Fh) {
   const double* sw = u.sw;
   const long long S = u.sws;
@@ -442,8 +450,8 @@ __device__ __forceinline__ void face_flux(const UwArgs& u, long long c0, long lo
           wm[k] = lev[5 * m] * sw[5 * S + im] + lev[5 * m + 1] * sw[6 * S + im] + lev[5 * m + 2] * sw[7 * S + im] +
                   lev[5 * m + 3] * sw[8 * S + im] + lev[5 * m + 4] * sw[9 * S + im];
         }
-        const double v1 = recons(u.recon, wp, u.bfac, lsh);
-        const double v2 = recons(u.recon, wm, u.bfac, lsh);
+        const double v1 = recons<SCH>(wp, u.bfac, lsh);
+        const double v2 = recons<SCH>(wm, u.bfac, lsh);
         Fhc[m] = v1 + v2;
       }
 #pragma unroll
@@ -461,28 +469,25 @@ __device__ __forceinline__ void face_flux(const UwArgs& u, long long c0, long lo
       wp[k] = sw[m * S + c0 + (long long)(k - 3) * rs];
       wm[k] = sw[(5 + m) * S + c0 + (long long)(4 - k) * rs];
     }
-    const double v1 = recons(u.recon, wp, u.bfac, lsh);
-    const double v2 = recons(u.recon, wm, u.bfac, lsh);
+    const double v1 = recons<SCH>(wp, u.bfac, lsh);
+    const double v2 = recons<SCH>(wm, u.bfac, lsh);
     Fh[m] = v1 + v2;
   }
 }
 
-// qrhs(t) = qrhs(t) + Fh(t) - Fh(t-1) (solver.F90:789-795); the first
-// direction starts from rhscal's zero, the last one negates (:242)
-__device__ __forceinline__ void face_update(const FlowArgs& a, long long o, const double* cur, const double* prev,
-                                            bool first, bool last) {
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    double* dst = a.qrhs + m * a.fs + o;
-    const double old = first ? 0.0 : *dst;
-    const double v = old + cur[m] - prev[m];
-    *dst = last ? -v : v;
-  }
+// qrhs(t) = qrhs(t) + Fh(t) - Fh(t-1) (solver.F90:789-795) of component m;
+// the first direction starts from rhscal's zero, the last one negates (:242)
+__device__ __forceinline__ void face_update1(const FlowArgs& a, int m, long long o, double cur, double prev,
+                                             bool first, bool last) {
+  double* dst = a.qrhs + m * a.fs + o;
+  const double old = first ? 0.0 : *dst;
+  const double v = old + cur - prev;
+  *dst = last ? -v : v;
 }
 
 // x-lines: lane l of a wavefront takes interface i0 - 1 + l of segment
 // (L, sg); lanes 1..63 update points i0..i0+62
-template <bool CD>
+template <int SCH, bool CD>
 __global__ __launch_bounds__(256) void k_uw_face_x(FlowArgs a, UwArgs u, int first, int last) {
   const int n = a.n[0];
   const int NR = n + 9;
@@ -498,16 +503,73 @@ __global__ __launch_bounds__(256) void k_uw_face_x(FlowArgs a, UwArgs u, int fir
   int c3[3];
   const long long o0 = line_base(a, 0, L, c3);
   double Fh[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-  if (ok) face_flux<CD>(u, L * NR + (i + 4), 1, face_shock(u, o0, 1, i, n), Fh);
-  double prev[5];
+  if (ok) face_flux<SCH, CD>(u, L * NR + (i + 4), 1, face_shock(u, o0, 1, i, n), Fh);
 #pragma unroll
-  for (int m = 0; m < 5; m++) prev[m] = __shfl_up(Fh[m], 1, 64);
-  if (ok && lane >= 1) face_update(a, o0 + i, Fh, prev, first, last);
+  for (int m = 0; m < 5; m++) {
+    const double prev = __shfl_up(Fh[m], 1, 64);
+    if (ok && lane >= 1) face_update1(a, m, o0 + i, Fh[m], prev, first, last);
+  }
 }
 
-// y- and z-lines: one lane per line and a run of kRun points
-template <bool CD>
-__global__ __launch_bounds__(256) void k_uw_face_yz(FlowArgs a, UwArgs u, int dir, int first, int last) {
+// y- and z-lines without the characteristic projection: one lane per line
+// and a run of kRun points; per component the 8-row windows of F+ and F-
+// slide along the run in registers (one new row of each per point)
+template <int SCH, int DIR>
+__global__ __launch_bounds__(256) void k_uw_face_yz(FlowArgs a, UwArgs u, int first, int last) {
+  constexpr int dir = DIR;
+  const int n = a.n[dir];
+  const long long NL = uw_lines(a.n, dir);
+  const int nruns = (n + 1 + kRun - 1) / kRun;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= NL * nruns) return;
+  const int run = (int)(idx / NL);
+  const long long L = idx - (long long)run * NL;
+  const int r0 = run * kRun, r1 = min(r0 + kRun - 1, n);
+  int c3[3];
+  const long long o0 = line_base(a, dir, L, c3);
+  const long long st = dir_stride(a, dir);
+  const double* sw = u.sw;
+  const long long S = u.sws;
+  bool shp = face_shock(u, o0, st, r0 - 1, n);
+#pragma unroll 1
+  for (int m = 0; m < 5; m++) {
+    // rows r0-4 .. r0+3: the window of interface r0-1
+    double wp[8], wr[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const long long c = (long long)(r0 - 4 + k + 4) * NL + L;
+      wp[k] = sw[m * S + c];
+      wr[k] = sw[(5 + m) * S + c];
+    }
+    double wm[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) wm[k] = wr[7 - k];
+    double prev = recons<SCH>(wp, u.bfac, shp) + recons<SCH>(wm, u.bfac, shp);
+#pragma unroll 1
+    for (int t = r0; t <= r1; t++) {
+#pragma unroll
+      for (int k = 0; k < 7; k++) {
+        wp[k] = wp[k + 1];
+        wr[k] = wr[k + 1];
+      }
+      const long long c = (long long)(t + 4 + 4) * NL + L;
+      wp[7] = sw[m * S + c];
+      wr[7] = sw[(5 + m) * S + c];
+#pragma unroll
+      for (int k = 0; k < 8; k++) wm[k] = wr[7 - k];
+      const bool sh = face_shock(u, o0, st, t, n);
+      const double cur = recons<SCH>(wp, u.bfac, sh) + recons<SCH>(wm, u.bfac, sh);
+      face_update1(a, m, o0 + (long long)t * st, cur, prev, first, last);
+      prev = cur;
+    }
+  }
+}
+
+// y- and z-lines with the characteristic projection: one lane per line
+// and a run of kRun points, the five components of each interface together
+template <int SCH, int DIR>
+__global__ __launch_bounds__(256) void k_uw_face_yz_cd(FlowArgs a, UwArgs u, int first, int last) {
+  constexpr int dir = DIR;
   const int n = a.n[dir];
   const long long NL = uw_lines(a.n, dir);
   const int nruns = (n + 1 + kRun - 1) / kRun;
@@ -520,12 +582,15 @@ __global__ __launch_bounds__(256) void k_uw_face_yz(FlowArgs a, UwArgs u, int di
   const long long o0 = line_base(a, dir, L, c3);
   const long long st = dir_stride(a, dir);
   double prev[5], cur[5];
-  face_flux<CD>(u, (long long)(r0 - 1 + 4) * NL + L, NL, face_shock(u, o0, st, r0 - 1, n), prev);
+  face_flux<SCH, true>(u, (long long)(r0 - 1 + 4) * NL + L, NL, face_shock(u, o0, st, r0 - 1, n), prev);
+#pragma unroll 1
   for (int t = r0; t <= r1; t++) {
-    face_flux<CD>(u, (long long)(t + 4) * NL + L, NL, face_shock(u, o0, st, t, n), cur);
-    face_update(a, o0 + (long long)t * st, cur, prev, first, last);
+    face_flux<SCH, true>(u, (long long)(t + 4) * NL + L, NL, face_shock(u, o0, st, t, n), cur);
 #pragma unroll
-    for (int m = 0; m < 5; m++) prev[m] = cur[m];
+    for (int m = 0; m < 5; m++) {
+      face_update1(a, m, o0 + (long long)t * st, cur[m], prev[m], first, last);
+      prev[m] = cur[m];
+    }
   }
 }
 
@@ -599,26 +664,46 @@ hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st) 
   return hipGetLastError();
 }
 
-hipError_t uw_launch_dir(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last, hipStream_t st) {
-  const long long np = uw_points(a.n, dir);
-  hipLaunchKernelGGL(k_uw_split, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, st, a, u, dir);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
+hipError_t uw_launch_split(const FlowArgs& a, const UwArgs& u, int dir, hipStream_t st) {
+  const dim3 g((unsigned)((uw_points(a.n, dir) + 255) / 256));
+  if (dir == 0) hipLaunchKernelGGL(k_uw_split<0>, g, dim3(256), 0, st, a, u);
+  else if (dir == 1) hipLaunchKernelGGL(k_uw_split<1>, g, dim3(256), 0, st, a, u);
+  else hipLaunchKernelGGL(k_uw_split<2>, g, dim3(256), 0, st, a, u);
+  return hipGetLastError();
+}
+
+template <int SCH>
+hipError_t launch_face_s(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last, hipStream_t st) {
   const long long NL = uw_lines(a.n, dir);
   if (dir == 0) {
     const long long waves = NL * ((a.n[0] + 1 + 62) / 63);
     const dim3 g((unsigned)((waves + 3) / 4));
-    if (u.chardec) hipLaunchKernelGGL(k_uw_face_x<true>, g, dim3(256), 0, st, a, u, (int)first, (int)last);
-    else hipLaunchKernelGGL(k_uw_face_x<false>, g, dim3(256), 0, st, a, u, (int)first, (int)last);
+    if (u.chardec)
+      hipLaunchKernelGGL((k_uw_face_x<SCH, true>), g, dim3(256), 0, st, a, u, (int)first, (int)last);
+    else
+      hipLaunchKernelGGL((k_uw_face_x<SCH, false>), g, dim3(256), 0, st, a, u, (int)first, (int)last);
   } else {
     const long long nt = NL * ((a.n[dir] + 1 + kRun - 1) / kRun);
     const dim3 g((unsigned)((nt + 255) / 256));
-    if (u.chardec)
-      hipLaunchKernelGGL(k_uw_face_yz<true>, g, dim3(256), 0, st, a, u, dir, (int)first, (int)last);
-    else
-      hipLaunchKernelGGL(k_uw_face_yz<false>, g, dim3(256), 0, st, a, u, dir, (int)first, (int)last);
+    const int f = first, l = last;
+    if (u.chardec && dir == 1) hipLaunchKernelGGL((k_uw_face_yz_cd<SCH, 1>), g, dim3(256), 0, st, a, u, f, l);
+    else if (u.chardec) hipLaunchKernelGGL((k_uw_face_yz_cd<SCH, 2>), g, dim3(256), 0, st, a, u, f, l);
+    else if (dir == 1) hipLaunchKernelGGL((k_uw_face_yz<SCH, 1>), g, dim3(256), 0, st, a, u, f, l);
+    else hipLaunchKernelGGL((k_uw_face_yz<SCH, 2>), g, dim3(256), 0, st, a, u, f, l);
   }
   return hipGetLastError();
+}
+
+hipError_t uw_launch_face(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last, hipStream_t st) {
+  switch (u.recon) {
+    case -1: return launch_face_s<-1>(a, u, dir, first, last, st);
+    case 0: return launch_face_s<0>(a, u, dir, first, last, st);
+    case 1: return launch_face_s<1>(a, u, dir, first, last, st);
+    case 2: return launch_face_s<2>(a, u, dir, first, last, st);
+    case 3: return launch_face_s<3>(a, u, dir, first, last, st);
+    case 5: return launch_face_s<5>(a, u, dir, first, last, st);
+    default: return launch_face_s<6>(a, u, dir, first, last, st);
+  }
 }
 
 }  // namespace astr

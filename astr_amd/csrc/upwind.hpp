@@ -43,10 +43,12 @@ __host__ __device__ inline long long uw_points(const int n[3], int dir) {
 // the Ducros sensor and the shock flags (after the gradient passes stored
 // the whole velocity gradient)
 hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st);
-// the upwind convective term of direction dir into qrhs: first direction
-// stores, later ones accumulate, the last one also negates (rhscal's
-// qrhs = -qrhs)
-hipError_t uw_launch_dir(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last,
-                         hipStream_t st);
+// the Steger-Warming split of every line of direction dir into u.sw
+hipError_t uw_launch_split(const FlowArgs& a, const UwArgs& u, int dir, hipStream_t st);
+// the interface fluxes of direction dir from u.sw, into qrhs: the first
+// direction stores, later ones accumulate, the last one also negates
+// (rhscal's qrhs = -qrhs)
+hipError_t uw_launch_face(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last,
+                          hipStream_t st);
 
 }  // namespace astr

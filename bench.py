@@ -358,6 +358,52 @@ def main_path_step(n, dt, steps, warmup):
            "value": round(pts * 3 * steps / wall / 1e6, 3), "unit": "Mpt-updates/s",
            "step_roofline": {"achieved_GBs": round(step_alg, 1), "frac": round(step_alg / HBM_PEAK_GBS, 4)},
            "kernels_ms_per_launch": {f"{k[0]}_{k[1]}": round(v[0] / v[1], 4) for k, v in prof.items()},
+           "launch": {f"{k[0]}_{k[1]}": {"shape": A.SHAPES[v[0]], "minb": v[1]}
+                      for k, v in f.pass_configs().items()},
+           "ke_last": float(ke[-1])}
+    f.close()
+    return out
+
+
+def upwind_path_step(n, dt, steps, warmup, recon, chardec):
+    """The main solver's stage with the explicit upwind-biased convection
+    (conschm 753e, recon_schem `recon`, lchardecomp `chardec`, bfacmpld 0.3,
+    shkcrt 0.05; convrsduwd, src/solver.F90:548-1201) on the same TGV
+    workload and field: ms per step, Mpt-updates/s and per-kernel times of
+    the timed steps; for the interface-flux kernel, its HBM rate from the
+    algorithmic bytes per point of one direction (the split buffer's ten
+    fields read once, qrhs read and written: 10 + 5 + 5 doubles, the first
+    direction 15)."""
+    import astr_amd as A
+
+    f = A.Flow(A.main_config(n, deltat=dt))
+    f.init_tgv(1e-3, 2024)
+    f.autotune(reps=3)
+    f.set_conschm("753e", recon, chardec, 0.3, 0.05)
+    if warmup:
+        f.rk3(warmup, stats=True)
+    f.sync()
+    f.profile_begin(3 * 16 * steps + 8)
+    f.sync()
+    t0 = time.perf_counter()
+    ke, en = f.rk3(steps, stats=True)
+    f.sync()
+    wall = time.perf_counter() - t0
+    prof = f.profile_end()
+    pts = (n + 1) ** 3
+    kern = {f"{k[0]}_{k[1]}": round(v[0] / v[1], 4) for k, v in prof.items()}
+    # algorithmic bytes of one interface-flux launch along y (10 split
+    # fields in, qrhs in and out) over its measured time
+    face_bytes = (10 + 10) * 8 * pts
+    face_ms = kern.get("uw_face_y")
+    out = {"workload": f"TGV {n}^3 RK3 step, main stage order, conschm 753e, recon_schem {recon}, "
+                       f"lchardecomp {'t' if chardec else 'f'}",
+           "ms_per_step": round(wall / steps * 1e3, 4),
+           "value": round(pts * 3 * steps / wall / 1e6, 3), "unit": "Mpt-updates/s",
+           "kernels_ms_per_launch": kern,
+           "face_y_hbm": ({"achieved_GBs": round(face_bytes / (face_ms * 1e-3) / 1e9, 1),
+                           "frac": round(face_bytes / (face_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                           "alg_bytes_per_point": 160} if face_ms else None),
            "ke_last": float(ke[-1])}
     f.close()
     return out
@@ -395,6 +441,10 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the batching-form measurement")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving-form measurement")
     ap.add_argument("--no-main", action="store_true", help="skip the main-path step measurement")
+    ap.add_argument("--no-upwind", action="store_true",
+                    help="skip the upwind-biased convection measurement")
+    ap.add_argument("--upwind-only", action="store_true",
+                    help="only the upwind-biased convection measurement (for profiling)")
     ap.add_argument("--chem-only", action="store_true",
                     help="only the chemistry batch (for counter runs of its kernel)")
     ap.add_argument("--no-autotune", action="store_true",
@@ -418,6 +468,11 @@ def main():
         return
     n = args.n
     dt = 5e-4 * 256.0 / n
+    if args.upwind_only:
+        print(json.dumps({"upwind_path": [upwind_path_step(n, dt, args.steps, args.warmup, 3, False),
+                                          upwind_path_step(n, dt, args.steps, args.warmup, 3, True)]}),
+              flush=True)
+        return
     f = A.Flow(A.tgv_config(n, deltat=dt))
     f.init_tgv(1e-3, 2024)
     tune_s = None
@@ -497,6 +552,11 @@ def main():
     f.close()
     if not args.no_main:
         out["main_path"] = main_path_step(n, dt, args.steps, args.warmup)
+    if not args.no_upwind:
+        # MP7 as the reference's shock examples use it, without and with the
+        # characteristic decomposition (which brings the sensor in)
+        out["upwind_path"] = [upwind_path_step(n, dt, args.steps, args.warmup, 3, False),
+                              upwind_path_step(n, dt, args.steps, args.warmup, 3, True)]
     if not args.no_chem:
         out["chem_batch"] = chem_batch()
     if not args.no_batch:

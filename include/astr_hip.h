@@ -218,8 +218,9 @@ int astr_hip_set_convection(astr_hip_ctx *ctx, int upwind, int recon_schem, int 
  * are no-ops).  Slot = pass*3 + direction, pass 0..8 = convective+gradient
  * first/middle/last, viscous middle/last(rhs)/last(RK3+filter), filter, and
  * the two halves of the split last convective+gradient pass (gradients and
- * stress, convective). */
-#define ASTR_PROF_SLOTS 27
+ * stress, convective); 9..11 = the upwind-biased convection's split,
+ * interface-flux and shock-sensor kernels (the sensor in slot 33). */
+#define ASTR_PROF_SLOTS 36
 int astr_hip_profile_begin(astr_hip_ctx *ctx, int max_launches);
 int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
 
