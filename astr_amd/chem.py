@@ -52,6 +52,8 @@ def _bind():
     L.astr_hip_reactor_get.argtypes = [C.c_void_p, C.c_int, _dp, _dp, C.POINTER(C.c_int),
                                        C.POINTER(C.c_int)]
     L.astr_hip_reactor_release.argtypes = [C.c_void_p, C.c_int]
+    L.astr_hip_reactor_rhs.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
+    L.astr_hip_reactor_method.argtypes = [_dp, C.c_int]
     L._chem_bound = True
     return L
 
@@ -87,6 +89,20 @@ class Mechanism:
             self.close()
         except Exception:
             pass
+
+    def reactor_rhs(self, p, y, jacobian=False):
+        """The reactor's d(T, Y)/dt at states y (n, ns+1) and pressures p (n,),
+        and with jacobian=True the integrator's forward-difference Jacobian
+        (n, ns+1, ns+1)."""
+        p = np.ascontiguousarray(np.atleast_1d(p), dtype=np.float64)
+        y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)
+        n, m = y.shape
+        if m != self.ns + 1 or p.shape != (n,):
+            raise AstrError("shapes: p (n,), y (n, ns+1)")
+        f = np.empty((n, m))
+        J = np.empty((n, m, m)) if jacobian else None
+        _chk(_bind().astr_hip_reactor_rhs(self.h, n, _p(p), _p(y), _p(f), _p(J)))
+        return (f, J) if jacobian else f
 
     def rates(self, T, rho_or_p, Y, input_is_p=True, species_h=False):
         """T, rho|p: (n,), Y: (ns, n) -> wdot (ns, n) kg/m^3/s, h_mix, cp_mix[, h_k]."""

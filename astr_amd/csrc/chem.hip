@@ -269,12 +269,36 @@ __global__ __launch_bounds__(256) void k_temperature(const Mech* __restrict__ M,
 // reference's 0dreactor is constant-volume).  State y = (T, Y_1..Y_ns):
 //   dY_k/dt = wdot_k W_k / rho,  dT/dt = -sum_k h_k dY_k/dt / cp,
 // rho = p / (R T sum Y_k/W_k), h_k = (h/RT)_k R T / W_k, cp = R sum Y_k cpr_k/W_k,
-// so the mixture enthalpy is conserved.  Integrated by ROS2, the two-stage
-// L-stable Rosenbrock method of Verwer, Spee, Blom and Hundsdorfer (1999):
-//   (I - g h J) k1 = f(y),  (I - g h J) k2 = f(y + h k1) - 2 k1,  g = 1 + 1/sqrt(2)
-//   y+ = y + 3/2 h k1 + 1/2 h k2,  error estimate h/2 (k1 + k2)
-// with a forward-difference Jacobian and step-size control.
+// so the mixture enthalpy is conserved.  Integrated by RODAS4, the stiffly
+// accurate, L-stable Rosenbrock method of order 4 with an embedded order-3
+// solution (Hairer and Wanner, Solving Ordinary Differential Equations II,
+// 2nd ed. 1996, Sect. VI.4), in the transformed form of their RODAS code:
+// with A = I - gamma h J (J by forward differences, column by column),
+//   U_i = gamma h A^-1 ( f(y + sum_{j<i} a_ij U_j) + sum_{j<i} (c_ij/h) U_j ),
+// stage 6's argument Y_6 = Y_5 + U_5, y_new = Y_6 + U_6, error estimate U_6;
+// step control h_new = h min(6, max(0.2, 0.9 err^(-1/4))) on a scaled
+// max-norm.
 // ---------------------------------------------------------------------------
+constexpr double kRoGamma = 0.25;
+// a_ij of stages 2..5 (row i-2 holds a_i1..a_i(i-1)) and c_ij of stages 2..6
+constexpr double kRoA[4][4] = {
+    {1.544, 0.0, 0.0, 0.0},
+    {0.9466785280815826, 0.2557011698983284, 0.0, 0.0},
+    {3.314825187068521, 2.896124015972201, 0.9986419139977817, 0.0},
+    {1.221224509226641, 6.019134481288629, 12.53708332932087, -0.6878860361058950}};
+constexpr double kRoC[5][5] = {
+    {-5.6688, 0.0, 0.0, 0.0, 0.0},
+    {-2.430093356833875, -0.2063599157091915, 0.0, 0.0, 0.0},
+    {-0.1073529058151375, -9.594562251023355, -20.47028614809616, 0.0, 0.0},
+    {7.496443313967647, -10.24680431464352, -33.99990352819905, 11.70890893206160, 0.0},
+    {8.083246795921522, -7.981132988064893, -31.52159432874371, 16.31930543123136, -6.058818238834054}};
+
+// the step-size factor after a step of scaled error err
+__device__ __forceinline__ double ro_factor(double err, bool accepted) {
+  if (accepted) return fmin(6.0, fmax(0.2, 0.9 / sqrt(sqrt(fmax(err, 1e-16)))));
+  return err < 1e300 ? fmax(0.2, fmin(0.9, 0.9 / sqrt(sqrt(err)))) : 0.2;
+}
+
 constexpr int NYMAX = NSMAX + 1;
 
 __device__ void reactor_rhs(const Mech* __restrict__ M, double p, const double* yv, double* f) {
@@ -323,6 +347,9 @@ __device__ bool lu_factor(double* a, int* piv, int n) {
   return true;
 }
 
+// The factorization swaps whole rows (its stored multipliers included), so
+// the solve applies every interchange to b first, then the unit lower and the
+// upper triangular solves (as LAPACK's getrs does).
 __device__ void lu_solve(const double* a, const int* piv, int n, double* b) {
   for (int k = 0; k < n; k++) {
     if (piv[k] != k) {
@@ -330,8 +357,9 @@ __device__ void lu_solve(const double* a, const int* piv, int n, double* b) {
       b[k] = b[piv[k]];
       b[piv[k]] = t;
     }
-    for (int i = k + 1; i < n; i++) b[i] -= a[i * NYMAX + k] * b[k];
   }
+  for (int k = 0; k < n; k++)
+    for (int i = k + 1; i < n; i++) b[i] -= a[i * NYMAX + k] * b[k];
   for (int k = n - 1; k >= 0; k--) {
     double s = b[k];
     for (int j = k + 1; j < n; j++) s -= a[k * NYMAX + j] * b[j];
@@ -362,8 +390,8 @@ __global__ __launch_bounds__(64) void k_reactor(const Mech* __restrict__ M, Reac
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P.cap || P.status[i] != RX_RUN) return;
   const int n = M->ns + 1;
-  const double gam = 1.0 + 0.70710678118654752440;
-  double y[NYMAX], f0[NYMAX], f1[NYMAX], k1[NYMAX], k2[NYMAX], yt[NYMAX];
+  const double gam = kRoGamma;
+  double y[NYMAX], f0[NYMAX], f1[NYMAX], yt[NYMAX], y5[NYMAX], U[6][NYMAX];
   double a[NYMAX * NYMAX];
   int piv[NYMAX];
   for (int k = 0; k < n; k++) y[k] = P.y[(size_t)k * P.cap + i];
@@ -378,7 +406,9 @@ __global__ __launch_bounds__(64) void k_reactor(const Mech* __restrict__ M, Reac
     // forward-difference Jacobian, column by column
     for (int j = 0; j < n; j++) {
       for (int k = 0; k < n; k++) yt[k] = y[k];
-      const double d = 1.4901161193847656e-08 * fmax(fabs(y[j]), j == 0 ? 1.0 : 1e-8);
+      // floor 1e-6 for mass fractions: a smaller step on a trace species
+      // leaves the difference quotient to rounding
+      const double d = 1.4901161193847656e-08 * fmax(fabs(y[j]), j == 0 ? 1.0 : 1e-6);
       yt[j] = y[j] + d;
       reactor_rhs(M, p, yt, f1);
       for (int k = 0; k < n; k++) a[k * NYMAX + j] = -gam * h * (f1[k] - f0[k]) / d;
@@ -387,17 +417,35 @@ __global__ __launch_bounds__(64) void k_reactor(const Mech* __restrict__ M, Reac
     bool ok = lu_factor(a, piv, n);
     double err = 2.0;
     if (ok) {
-      for (int k = 0; k < n; k++) k1[k] = f0[k];
-      lu_solve(a, piv, n, k1);
-      for (int k = 0; k < n; k++) yt[k] = y[k] + h * k1[k];
-      reactor_rhs(M, p, yt, f1);
-      for (int k = 0; k < n; k++) k2[k] = f1[k] - 2.0 * k1[k];
-      lu_solve(a, piv, n, k2);
+      const double gh = gam * h;
+      for (int k = 0; k < n; k++) U[0][k] = f0[k];
+      lu_solve(a, piv, n, U[0]);
+      for (int k = 0; k < n; k++) U[0][k] = gh * U[0][k];
+      for (int s = 1; s < 6; s++) {
+        for (int k = 0; k < n; k++) {
+          double v = y[k];
+          if (s < 5) {
+            for (int j = 0; j < s; j++) v += kRoA[s - 1][j] * U[j][k];
+            if (s == 4) y5[k] = v;
+          } else {
+            v = y5[k] + U[4][k];
+          }
+          yt[k] = v;
+        }
+        reactor_rhs(M, p, yt, f1);
+        for (int k = 0; k < n; k++) {
+          double c = 0.0;
+          for (int j = 0; j < s; j++) c += (kRoC[s - 1][j] / h) * U[j][k];
+          U[s][k] = f1[k] + c;
+        }
+        lu_solve(a, piv, n, U[s]);
+        for (int k = 0; k < n; k++) U[s][k] = gh * U[s][k];
+      }
       err = 0.0;
       for (int k = 0; k < n; k++) {
-        const double yn = y[k] + 1.5 * h * k1[k] + 0.5 * h * k2[k];
+        const double yn = yt[k] + U[5][k];
         const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
-        const double e = fabs(0.5 * h * (k1[k] + k2[k])) / sc;
+        const double e = fabs(U[5][k]) / sc;
         if (!(e <= err)) err = e;  // a NaN makes err NaN and rejects the step
         yt[k] = yn;
       }
@@ -406,12 +454,10 @@ __global__ __launch_bounds__(64) void k_reactor(const Mech* __restrict__ M, Reac
       t += h;
       for (int k = 0; k < n; k++) y[k] = yt[k];
       nstep++;
-      const double fac = 0.9 / sqrt(fmax(err, 1e-10));
-      h *= fmin(5.0, fmax(0.2, fac));
+      h *= ro_factor(err, true);
     } else {
       nrej++;
-      const double fac = err < 1e300 ? 0.9 / sqrt(err) : 0.2;
-      h *= fmax(0.2, fmin(0.9, fac));
+      h *= ro_factor(err, false);
     }
     if (t >= tend) { st = RX_DONE; break; }
   }
@@ -512,13 +558,130 @@ __device__ void reactor_rhs_n(const Mech* __restrict__ M, double p, const double
   f[0] = -dh / cp;
 }
 
+// c^(nu-1) nu: the derivative of c^nu
+__device__ __forceinline__ double conc_pow_d(double c, double nu) {
+  return nu == 1.0 ? 1.0 : (nu == 2.0 ? 2.0 * c : nu * conc_pow_general(c, nu - 1.0));
+}
+
+// Column j (0: T, 1 + m: Y_m) of the reactor's Jacobian J = d f / d(T, Y),
+// analytic, by the chain rule through the concentrations c_k = rho Y_k / W_k:
+//   dc_k/dY_m = rho/W_m delta_km - c_k / (W_m sum_i Y_i/W_i),  dc_k/dT = -c_k/T,
+// so per reaction, with s_r = sum_i c_i dq_r/dc_i (mass action is
+// homogeneous: s_r = o_f fwd - o_r rev, times [M] and plus [M] (fwd - rev)
+// for a three-body reaction, o = the sum of the orders),
+//   dq_r/dY_m = dq_r/dc_m rho/W_m - s_r / (W_m sum Y/W),
+//   dq_r/dT   = [M] (fwd dln kf/dT - rev dln kr/dT) - s_r / T,
+//   dln kf/dT = b/T + E/(R T^2),  dln kr/dT = dln kf/dT - dH_r/(R T^2) + dnu_r/T;
+// then f_Y = w W / rho and f_T = -sum_k h_k f_Yk / cp give the column.  f0 is
+// f(y).  Every column is computed alone, so a team evaluates them on separate
+// lanes with the arithmetic of the one-thread kernel's sequential loop.
+template <int NS>
+__device__ void reactor_jac_col(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
+                                const double (&f0)[NS + 1], int j, double (&col)[NS + 1]) {
+#pragma clang fp contract(off)
+  const double T = yv[0];
+  double c[NS], g[NS], cpr[NS], hrt[NS], dw[NS];
+  double winv = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; k++) winv += yv[1 + k] / M->W[k];
+  const double rho = p / (kRgas * T * winv);
+  const double logT = log(T), invT = 1.0 / T;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  double dcpm = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    c[k] = rho * yv[1 + k] / M->W[k];
+    const double* a = T <= M->tmid[k] ? M->lo[k] : M->hi[k];
+    cpr[k] = a[0] + a[1] * T + a[2] * T2 + a[3] * T3 + a[4] * T4;
+    hrt[k] = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 + a[4] * T4 / 5.0 + a[5] * invT;
+    const double sr = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 + a[4] * T4 / 4.0 + a[6];
+    g[k] = hrt[k] - sr;
+    dw[k] = 0.0;
+    if (j == 0) dcpm += yv[1 + k] * (a[1] + 2.0 * a[2] * T + 3.0 * a[3] * T2 + 4.0 * a[4] * T3) / M->W[k];
+  }
+  const int m = j - 1;
+  const double cm_ = m >= 0 ? pick<NS>(c, m) : 0.0;
+  const double Wm = m >= 0 ? M->W[m] : 1.0;
+  const double lc0 = log(M->p0 / (kRgas * T));
+  for (int r = 0; r < M->nr; r++) {
+    const double kf = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT);
+    double fwd = kf, ofw = 0.0, dfw = 0.0, dg = 0.0, dh = 0.0;
+    for (int q = 0; q < M->nre[r]; q++) {
+      const int sp = M->rsp[r][q];
+      const double nu = M->rnu[r][q];
+      fwd *= conc_pow(pick<NS>(c, sp), nu);
+      ofw += nu;
+      dg -= nu * pick<NS>(g, sp);
+      dh -= nu * pick<NS>(hrt, sp);
+      if (sp == m) {
+        double d = kf * conc_pow_d(cm_, nu);
+        for (int q2 = 0; q2 < M->nre[r]; q2++)
+          if (q2 != q) d *= conc_pow(pick<NS>(c, M->rsp[r][q2]), M->rnu[r][q2]);
+        dfw += d;
+      }
+    }
+    double rv = 0.0, orv = 0.0, drv = 0.0, dlkr = 0.0;
+    const double dlkf = M->b[r] * invT + M->EaR[r] * invT * invT;
+    if (M->rev[r]) {
+      for (int q = 0; q < M->npr[r]; q++) {
+        const int sp = M->psp[r][q];
+        dg += M->pnu[r][q] * pick<NS>(g, sp);
+        dh += M->pnu[r][q] * pick<NS>(hrt, sp);
+      }
+      const double kr = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT + dg - M->dnu[r] * lc0);
+      rv = kr;
+      for (int q = 0; q < M->npr[r]; q++) {
+        const int sp = M->psp[r][q];
+        const double nu = M->pnu[r][q];
+        rv *= conc_pow(pick<NS>(c, sp), nu);
+        orv += nu;
+        if (sp == m) {
+          double d = kr * conc_pow_d(cm_, nu);
+          for (int q2 = 0; q2 < M->npr[r]; q2++)
+            if (q2 != q) d *= conc_pow(pick<NS>(c, M->psp[r][q2]), M->pnu[r][q2]);
+          drv += d;
+        }
+      }
+      dlkr = dlkf - dh * invT + M->dnu[r] * invT;
+    }
+    double s, dq;
+    if (M->tb[r]) {
+      double cmb = 0.0;
+#pragma unroll
+      for (int k = 0; k < NS; k++) cmb += M->eff[r][k] * c[k];
+      s = cmb * ((ofw + 1.0) * fwd - (orv + 1.0) * rv);
+      dq = m >= 0 ? cmb * (dfw - drv) + (fwd - rv) * M->eff[r][m] : cmb * (fwd * dlkf - rv * dlkr);
+    } else {
+      s = ofw * fwd - orv * rv;
+      dq = m >= 0 ? dfw - drv : fwd * dlkf - rv * dlkr;
+    }
+    const double dqy = m >= 0 ? dq * rho / Wm - s / (Wm * winv) : dq - s * invT;
+    for (int q = 0; q < M->nre[r]; q++) add_at<NS>(dw, M->rsp[r][q], -M->rnu[r][q] * dqy);
+    for (int q = 0; q < M->npr[r]; q++) add_at<NS>(dw, M->psp[r][q], M->pnu[r][q] * dqy);
+  }
+  double cp = 0.0, dH = 0.0;
+#pragma unroll
+  for (int k = 0; k < NS; k++) {
+    const double fY = f0[1 + k];
+    const double v = M->W[k] / rho * dw[k] + (m >= 0 ? fY / (Wm * winv) : fY * invT);
+    col[1 + k] = v;
+    cp += yv[1 + k] * cpr[k] / M->W[k];
+    dH += hrt[k] * T / M->W[k] * v;
+    if (j == 0) dH += cpr[k] / M->W[k] * fY;
+  }
+  const double dcp = m >= 0 ? pick<NS>(cpr, m) / Wm : dcpm;
+  col[0] = -(dH + f0[0] * dcp) / cp;
+}
+
 // solve with LU factors in LDS, element (r,c) at a[(r*N + c)*S]: S = 64 for
 // the one-thread kernel's interleaved matrices, 1 for a team's own matrix
+// The factorization swaps whole rows (its stored multipliers included), so
+// every interchange is applied to b before the unit lower triangular solve.
 template <int N, int S = 64>
 __device__ __forceinline__ void lu_solve_n(const double* a, const int (&piv)[N], double (&b)[N]) {
 #pragma unroll
   for (int k = 0; k < N; k++) {
-    // apply the row swap of step k
+    // the row swap of step k
     const double bk = b[k];
     double bp = bk;
 #pragma unroll
@@ -527,6 +690,9 @@ __device__ __forceinline__ void lu_solve_n(const double* a, const int (&piv)[N],
     for (int r = k + 1; r < N; r++)
       if (piv[k] == r) b[r] = bk;
     b[k] = bp;
+  }
+#pragma unroll
+  for (int k = 0; k < N; k++) {
 #pragma unroll
     for (int r = k + 1; r < N; r++) b[r] -= a[(r * N + k) * S] * b[k];
   }
@@ -539,6 +705,59 @@ __device__ __forceinline__ void lu_solve_n(const double* a, const int (&piv)[N],
   }
 }
 
+// RODAS4's six stages from the factored A = I - gamma h J (in LDS, stride S)
+// and f0 = f(y): the new state into ynew, the scaled error as the result
+template <int NS, int S>
+__device__ __forceinline__ double ro_stages(const Mech* __restrict__ M, double p, double h, const double* a,
+                                            const int (&piv)[NS + 1], const double (&y)[NS + 1],
+                                            const double (&f0)[NS + 1], double (&ynew)[NS + 1], double rtol,
+                                            double atol) {
+  constexpr int N = NS + 1;
+  double U[6][N], yt[N], y5[N], f1[N];
+  const double gh = kRoGamma * h;
+#pragma unroll
+  for (int k = 0; k < N; k++) U[0][k] = f0[k];
+  lu_solve_n<N, S>(a, piv, U[0]);
+#pragma unroll
+  for (int k = 0; k < N; k++) U[0][k] = gh * U[0][k];
+#pragma unroll
+  for (int s = 1; s < 6; s++) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double v = y[k];
+      if (s < 5) {
+#pragma unroll
+        for (int j = 0; j < s; j++) v += kRoA[s - 1][j] * U[j][k];
+        if (s == 4) y5[k] = v;
+      } else {
+        v = y5[k] + U[4][k];
+      }
+      yt[k] = v;
+    }
+    reactor_rhs_n<NS>(M, p, yt, f1);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double c = 0.0;
+#pragma unroll
+      for (int j = 0; j < s; j++) c += (kRoC[s - 1][j] / h) * U[j][k];
+      U[s][k] = f1[k] + c;
+    }
+    lu_solve_n<N, S>(a, piv, U[s]);
+#pragma unroll
+    for (int k = 0; k < N; k++) U[s][k] = gh * U[s][k];
+  }
+  double err = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const double yn = yt[k] + U[5][k];
+    const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
+    const double e = fabs(U[5][k]) / sc;
+    if (!(e <= err)) err = e;  // a NaN makes err NaN and rejects the step
+    ynew[k] = yn;
+  }
+  return err;
+}
+
 template <int NS>
 __global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, ReactorPool P,
                                                   int max_tries, double rtol, double atol) {
@@ -548,8 +767,8 @@ __global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, Re
   if (i >= P.cap || P.status[i] != RX_RUN) return;
   double* a = sA + threadIdx.x;
 #define AIJ(r, c) a[((r) * N + (c)) * 64]
-  const double gam = 1.0 + 0.70710678118654752440;
-  double y[N], f0[N], f1[N], k1[N], k2[N], yt[N];
+  const double gam = kRoGamma;
+  double y[N], f0[N], f1[N], yt[N];
   int piv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) y[k] = P.y[(size_t)k * P.cap + i];
@@ -561,15 +780,12 @@ __global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, Re
     if (h > tend - t) h = tend - t;
     if (!(h > 1e-18)) { st = RX_FAIL; break; }
     reactor_rhs_n<NS>(M, p, y, f0);
-#pragma unroll
+    // the analytic Jacobian, column by column: A = I - gamma h J
+#pragma unroll 1
     for (int j = 0; j < N; j++) {
+      reactor_jac_col<NS>(M, p, y, f0, j, f1);
 #pragma unroll
-      for (int k = 0; k < N; k++) yt[k] = y[k];
-      const double d = 1.4901161193847656e-08 * fmax(fabs(y[j]), j == 0 ? 1.0 : 1e-8);
-      yt[j] = y[j] + d;
-      reactor_rhs_n<NS>(M, p, yt, f1);
-#pragma unroll
-      for (int k = 0; k < N; k++) AIJ(k, j) = -gam * h * (f1[k] - f0[k]) / d;
+      for (int k = 0; k < N; k++) AIJ(k, j) = -gam * h * f1[k];
     }
 #pragma unroll
     for (int k = 0; k < N; k++) AIJ(k, k) += 1.0;
@@ -603,38 +819,16 @@ __global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, Re
       }
     }
     double err = 2.0;
-    if (ok) {
-      // k1 = A^-1 f0, k2 = A^-1 (f(y + h k1) - 2 k1)
-#pragma unroll
-      for (int k = 0; k < N; k++) k1[k] = f0[k];
-      lu_solve_n<N>(a, piv, k1);
-#pragma unroll
-      for (int k = 0; k < N; k++) yt[k] = y[k] + h * k1[k];
-      reactor_rhs_n<NS>(M, p, yt, f1);
-#pragma unroll
-      for (int k = 0; k < N; k++) k2[k] = f1[k] - 2.0 * k1[k];
-      lu_solve_n<N>(a, piv, k2);
-      err = 0.0;
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-        const double yn = y[k] + 1.5 * h * k1[k] + 0.5 * h * k2[k];
-        const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
-        const double e = fabs(0.5 * h * (k1[k] + k2[k])) / sc;
-        if (!(e <= err)) err = e;
-        yt[k] = yn;
-      }
-    }
+    if (ok) err = ro_stages<NS, 64>(M, p, h, a, piv, y, f0, yt, rtol, atol);
     if (err <= 1.0) {
       t += h;
 #pragma unroll
       for (int k = 0; k < N; k++) y[k] = yt[k];
       nstep++;
-      const double fac = 0.9 / sqrt(fmax(err, 1e-10));
-      h *= fmin(5.0, fmax(0.2, fac));
+      h *= ro_factor(err, true);
     } else {
       nrej++;
-      const double fac = err < 1e300 ? 0.9 / sqrt(err) : 0.2;
-      h *= fmax(0.2, fmin(0.9, fac));
+      h *= ro_factor(err, false);
     }
     if (t >= tend) { st = RX_DONE; break; }
   }
@@ -648,6 +842,31 @@ __global__ __launch_bounds__(64) void k_reactor_n(const Mech* __restrict__ M, Re
   P.status[i] = st;
 }
 
+
+// The reactor's right-hand side and its analytic Jacobian (the integrator's,
+// before the -gamma h scaling) at n states, one thread per state: y, f
+// row-major N per state, jac N x N per state (row k, column j)
+template <int NS>
+__global__ __launch_bounds__(64) void k_reactor_eval(const Mech* __restrict__ M, int n, const double* p,
+                                                     const double* y, double* f, double* jac) {
+  constexpr int N = NS + 1;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double yv[N], f0[N], yt[N], f1[N];
+#pragma unroll
+  for (int k = 0; k < N; k++) yv[k] = y[(size_t)i * N + k];
+  reactor_rhs_n<NS>(M, p[i], yv, f0);
+#pragma unroll
+  for (int k = 0; k < N; k++) f[(size_t)i * N + k] = f0[k];
+  if (!jac) return;
+#pragma unroll 1
+  for (int j = 0; j < N; j++) {
+    reactor_jac_col<NS>(M, p[i], yv, f0, j, f1);
+#pragma unroll
+    for (int k = 0; k < N; k++) jac[((size_t)i * N + k) * N + j] = f1[k];
+  }
+  (void)yt;
+}
 
 // ---------------------------------------------------------------------------
 // The same reactor integrated by a team of 16 lanes (one DPP row), four
@@ -684,8 +903,8 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
   if (i >= P.cap || P.status[i] != RX_RUN) return;  // the whole team leaves
   double* a = sA[team];
 #define AT(r, c) a[(r) * N + (c)]
-  const double gam = 1.0 + 0.70710678118654752440;
-  double y[N], f0[N], f1[N], k1[N], k2[N], yt[N];
+  const double gam = kRoGamma;
+  double y[N], f0[N], f1[N], yt[N];
   int piv[N];
 #pragma unroll
   for (int k = 0; k < N; k++) y[k] = P.y[(size_t)k * P.cap + i];
@@ -696,17 +915,12 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
     if (t >= tend) { st = RX_DONE; break; }
     if (h > tend - t) h = tend - t;
     if (!(h > 1e-18)) { st = RX_FAIL; break; }
-    // lane j < N: the state with component j perturbed; lane N (and the
-    // spare lanes): the state itself
-    const double d = 1.4901161193847656e-08 * fmax(fabs(pick<N>(y, lane)), lane == 0 ? 1.0 : 1e-8);
-#pragma unroll
-    for (int k = 0; k < N; k++) yt[k] = k == lane ? y[k] + d : y[k];
-    reactor_rhs_n<NS>(M, p, yt, f1);
-#pragma unroll
-    for (int k = 0; k < N; k++) f0[k] = __shfl(f1[k], N, kTeam);
+    // f(y) on every lane, then lane j < N: column j of the analytic Jacobian
+    reactor_rhs_n<NS>(M, p, y, f0);
     if (lane < N) {
+      reactor_jac_col<NS>(M, p, y, f0, lane, f1);
 #pragma unroll
-      for (int k = 0; k < N; k++) AT(k, lane) = -gam * h * (f1[k] - f0[k]) / d;
+      for (int k = 0; k < N; k++) AT(k, lane) = -gam * h * f1[k];
       AT(lane, lane) += 1.0;
     }
     team_sync();
@@ -741,26 +955,7 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
       team_sync();
     }
     double err = 2.0;
-    if (ok) {
-#pragma unroll
-      for (int k = 0; k < N; k++) k1[k] = f0[k];
-      lu_solve_n<N, 1>(a, piv, k1);
-#pragma unroll
-      for (int k = 0; k < N; k++) yt[k] = y[k] + h * k1[k];
-      reactor_rhs_n<NS>(M, p, yt, f1);
-#pragma unroll
-      for (int k = 0; k < N; k++) k2[k] = f1[k] - 2.0 * k1[k];
-      lu_solve_n<N, 1>(a, piv, k2);
-      err = 0.0;
-#pragma unroll
-      for (int k = 0; k < N; k++) {
-        const double yn = y[k] + 1.5 * h * k1[k] + 0.5 * h * k2[k];
-        const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
-        const double e = fabs(0.5 * h * (k1[k] + k2[k])) / sc;
-        if (!(e <= err)) err = e;
-        yt[k] = yn;
-      }
-    }
+    if (ok) err = ro_stages<NS, 1>(M, p, h, a, piv, y, f0, yt, rtol, atol);
     // the next step's matrix writes must follow every lane's solve reads
     team_sync();
     if (err <= 1.0) {
@@ -768,12 +963,10 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
 #pragma unroll
       for (int k = 0; k < N; k++) y[k] = yt[k];
       nstep++;
-      const double fac = 0.9 / sqrt(fmax(err, 1e-10));
-      h *= fmin(5.0, fmax(0.2, fac));
+      h *= ro_factor(err, true);
     } else {
       nrej++;
-      const double fac = err < 1e300 ? 0.9 / sqrt(err) : 0.2;
-      h *= fmax(0.2, fmin(0.9, fac));
+      h *= ro_factor(err, false);
     }
     if (t >= tend) { st = RX_DONE; break; }
   }
@@ -1149,6 +1342,41 @@ int astr_hip_chem_rates(astr_chem* c, long long n, const double* T, const double
   }
   (void)hipFree(d);
   return rc;
+}
+
+int astr_hip_reactor_rhs(astr_chem* chem, int n, const double* p, const double* y, double* f, double* jac) {
+  if (!chem || !p || !y || !f || n < 1) return cfail("reactor_rhs: null argument or no states");
+  if (chem->host.ns != 9) return cfail("reactor_rhs: 9-species mechanisms only (the unrolled kernel)");
+  CHIP_OK(hipSetDevice(chem->device));
+  constexpr int N = 10;
+  double* d = nullptr;
+  const size_t ny = (size_t)n * N, nj = jac ? (size_t)n * N * N : 0;
+  CHIP_OK(hipMalloc(&d, (n + 2 * ny + nj) * sizeof(double)));
+  double *dp = d, *dy = d + n, *df = dy + ny, *dj = jac ? df + ny : nullptr;
+  int rc = 0;
+  if (hipMemcpy(dp, p, n * sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dy, y, ny * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) {
+    rc = cfail("reactor_rhs: copy in failed");
+  } else {
+    hipLaunchKernelGGL(k_reactor_eval<9>, dim3((n + 63) / 64), dim3(64), 0, nullptr, chem->dev, n, dp, dy, df, dj);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess ||
+        hipMemcpy(f, df, ny * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess ||
+        (jac && hipMemcpy(jac, dj, nj * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+      rc = cfail("reactor_rhs: kernel or copy out failed");
+  }
+  (void)hipFree(d);
+  return rc;
+}
+
+int astr_hip_reactor_method(double* coef, int n) {
+  if (!coef || n < 26) return cfail("reactor_method needs 26 values");
+  int k = 0;
+  coef[k++] = kRoGamma;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j <= i; j++) coef[k++] = kRoA[i][j];
+  for (int i = 0; i < 5; i++)
+    for (int j = 0; j <= i; j++) coef[k++] = kRoC[i][j];
+  return 0;
 }
 
 int astr_hip_reactor_create(astr_chem* chem, int capacity, double rtol, double atol,

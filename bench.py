@@ -278,7 +278,7 @@ def batch_form(ncases=64, steps=100, seed=2026):
 
 def serve_form(rates=(10.0, 100.0, 1000.0), counts=(40, 150, 300), seed=7, rtol=1e-5, atol=1e-10):
     """Serving form: Poisson request streams, 70% 0-D constant-pressure H2-air
-    ignition to 1 ms (ROS2 at rtol/atol), 30% 2-D vortex (side U{64..256},
+    ignition to 1 ms (RODAS4 at rtol/atol), 30% 2-D vortex (side U{64..256},
     200 steps), under a 200 GB HBM budget.  Latency = completion - arrival.
     The sequential reference serves a sample one request at a time through
     the plain path; its cases/s is 1 / mean service time."""
@@ -297,11 +297,14 @@ def serve_form(rates=(10.0, 100.0, 1000.0), counts=(40, 150, 300), seed=7, rtol=
         st = S.latency_stats(res)
         ign = [r.latency for r in res if r.kind == "ignition"]
         vor = [r.latency for r in res if r.kind == "vortex"]
+        isteps = [r.data["steps"] for r in res if r.kind == "ignition"]
         out["rates"].append({"rate_req_per_s": rate, "requests": count,
                              "cases_per_s": round(st["cases_per_s"], 3),
                              "p50_s": round(st["p50_s"], 4), "p99_s": round(st["p99_s"], 4),
                              "p50_ignition_s": round(float(np.percentile(ign, 50)), 4) if ign else None,
                              "p50_vortex_s": round(float(np.percentile(vor, 50)), 4) if vor else None,
+                             "ignition_steps_mean": round(float(np.mean(isteps)), 1) if isteps else None,
+                             "ignition_steps_max": int(max(isteps)) if isteps else None,
                              "max_in_flight": srv.max_active})
     # sequential plain path on a sample; equality of results with the served runs
     sample = S.poisson_requests(6, 1.0, seed + 99)
@@ -443,6 +446,7 @@ def main():
     ap.add_argument("--no-main", action="store_true", help="skip the main-path step measurement")
     ap.add_argument("--no-upwind", action="store_true",
                     help="skip the upwind-biased convection measurement")
+    ap.add_argument("--serve-only", action="store_true", help="only the serving-form measurement")
     ap.add_argument("--upwind-only", action="store_true",
                     help="only the upwind-biased convection measurement (for profiling)")
     ap.add_argument("--chem-only", action="store_true",
@@ -465,6 +469,9 @@ def main():
 
     if args.chem_only:
         print(json.dumps({"chem_batch": chem_batch()}), flush=True)
+        return
+    if args.serve_only:
+        print(json.dumps({"serving": serve_form()}), flush=True)
         return
     n = args.n
     dt = 5e-4 * 256.0 / n

@@ -334,9 +334,12 @@ int astr_hip_chem_temperature_device(astr_chem *chem, long long n, const double 
  * The model: state (T, Y_k), dY_k/dt = wdot_k W_k / rho,
  * dT/dt = -sum_k h_k dY_k/dt / cp.
  *
- * The integrator: ROS2, the two-stage L-stable Rosenbrock method (Verwer et
- * al. 1999), with a finite-difference Jacobian and error control at rtol,
- * atol (mass fractions; temperature uses atol 1e-6 K).
+ * The integrator: RODAS4, the stiffly accurate L-stable Rosenbrock method of
+ * order 4 with an embedded order-3 solution (Hairer and Wanner 1996,
+ * Sect. VI.4), with a finite-difference Jacobian and error control at rtol,
+ * atol (mass fractions; temperature uses atol 1e-6 K).  reactor_method
+ * writes its table: gamma, a21, a31, a32, a41..a43, a51..a54, c21, c31,
+ * c32, c41..c43, c51..c54, c61..c65 (26 values; n must be at least 26).
  *
  * Lanes per reactor (set_lanes / get_lanes): 1, one thread per reactor and 64
  * reactors per wavefront, for throughput over many thousands of reactors; or
@@ -369,6 +372,12 @@ int astr_hip_reactor_sync(astr_reactor_pool *pool);
 int astr_hip_reactor_status(astr_reactor_pool *pool, int *status);
 int astr_hip_reactor_get(astr_reactor_pool *pool, int slot, double *t, double *TY, int *nstep, int *nrej);
 int astr_hip_reactor_release(astr_reactor_pool *pool, int slot);
+int astr_hip_reactor_method(double *coef, int n);
+/* The reactor's right-hand side d(T, Y)/dt at n states (p[n], y and f n x
+ * (ns+1) row-major) and, if jac is not null, the integrator's
+ * forward-difference Jacobian (n x (ns+1) x (ns+1), row k = component,
+ * column j = perturbed variable); 9-species mechanisms. */
+int astr_hip_reactor_rhs(astr_chem *chem, int n, const double *p, const double *y, double *f, double *jac);
 
 #ifdef __cplusplus
 }

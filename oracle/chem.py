@@ -182,6 +182,86 @@ class Mechanism:
         hk = hrt[0] * R_GAS * T[0] / self.W
         return np.concatenate([[-(hk * dY).sum() / cp[0]], dY])
 
+    def reactor_jacobian(self, yv, p):
+        """The analytic Jacobian d f / d(T, Y) of reactor_rhs (mass-action
+        kinetics, NASA-7 thermodynamics, constant p), by the chain rule through
+        the molar concentrations c_k = rho Y_k / W_k:
+          dq_r/dc_m  from the forward and reverse products and [M],
+          dq_r/dT|_c = [M](fwd dln kf/dT - rev dln kr/dT),
+            dln kf/dT = b/T + E/(R T^2), dln kr/dT = dln kf/dT - dH_r/(R T^2) + dnu_r/T,
+          dc_k/dY_j = rho/W_k delta_kj - c_k / (W_j sum_i Y_i/W_i),  dc_k/dT = -c_k/T,
+        then f_Y = w W / rho and f_T = -sum_k h_k f_Yk / cp."""
+        ns = self.ns
+        T = float(yv[0])
+        Y = np.asarray(yv[1:], dtype=float)
+        winv = (Y / self.W).sum()
+        rho = p / (R_GAS * T * winv)
+        c = rho * Y / self.W
+        cpr, hrt, sr = (a[0] for a in self.thermo(np.array([T])))
+        g = hrt - sr
+        a = self.coeffs(np.array([T]))[0]
+        dcpr = a[:, 1] + 2 * a[:, 2] * T + 3 * a[:, 3] * T**2 + 4 * a[:, 4] * T**3
+        logT = np.log(T)
+        lc0 = np.log(self.p0 / (R_GAS * T))
+        w = np.zeros(ns)
+        Jc = np.zeros((ns, ns))   # d w_k / d c_m
+        wT = np.zeros(ns)         # d w_k / dT at fixed c
+        for j, r in enumerate(self.rx):
+            kf = np.exp(np.log(r["A"]) + r["b"] * logT - r["EaR"] / T)
+            dlkf = r["b"] / T + r["EaR"] / T**2
+            fwd, dfwd = kf, np.zeros(ns)
+            for i, (k, v) in enumerate(r["reac"]):
+                fwd = fwd * c[k] ** v
+            for i, (k, v) in enumerate(r["reac"]):
+                other = kf
+                for i2, (k2, v2) in enumerate(r["reac"]):
+                    if i2 != i:
+                        other = other * c[k2] ** v2
+                dfwd[k] += other * v * c[k] ** (v - 1)
+            rev, drev, dlkr = 0.0, np.zeros(ns), 0.0
+            if r["rev"]:
+                dg = sum(v * g[k] for k, v in r["prod"]) - sum(v * g[k] for k, v in r["reac"])
+                dh = sum(v * hrt[k] for k, v in r["prod"]) - sum(v * hrt[k] for k, v in r["reac"])
+                dnu = sum(v for _, v in r["prod"]) - sum(v for _, v in r["reac"])
+                kr = np.exp(np.log(r["A"]) + r["b"] * logT - r["EaR"] / T + dg - dnu * lc0)
+                dlkr = dlkf - dh / T + dnu / T
+                rev = kr
+                for k, v in r["prod"]:
+                    rev = rev * c[k] ** v
+                for i, (k, v) in enumerate(r["prod"]):
+                    other = kr
+                    for i2, (k2, v2) in enumerate(r["prod"]):
+                        if i2 != i:
+                            other = other * c[k2] ** v2
+                    drev[k] += other * v * c[k] ** (v - 1)
+            if r["tb"]:
+                cm = (r["eff"] * c).sum()
+                dq = cm * (dfwd - drev) + (fwd - rev) * r["eff"]
+                q = cm * (fwd - rev)
+                qT = cm * (fwd * dlkf - rev * dlkr)
+            else:
+                dq = dfwd - drev
+                q = fwd - rev
+                qT = fwd * dlkf - rev * dlkr
+            w += self.nu[:, j] * q
+            Jc += np.outer(self.nu[:, j], dq)
+            wT += self.nu[:, j] * qT
+        s = Jc @ c
+        dwdY = Jc * (rho / self.W)[None, :] - np.outer(s, 1.0 / (self.W * winv))
+        dwdT = wT - s / T
+        fY = w * self.W / rho
+        J = np.zeros((ns + 1, ns + 1))
+        J[1:, 1:] = (self.W / rho)[:, None] * dwdY + np.outer(fY, 1.0 / (self.W * winv))
+        J[1:, 0] = self.W / rho * dwdT + fY / T
+        e = hrt * T / self.W              # h_k / R
+        cpm = (Y * cpr / self.W).sum()    # cp / R
+        fT = -(e * fY).sum() / cpm
+        dcpm = np.concatenate([[(Y * dcpr / self.W).sum()], cpr / self.W])
+        dH = e @ J[1:, :]
+        dH[0] += (cpr / self.W * fY).sum()
+        J[0, :] = -(dH + fT * dcpm) / cpm
+        return J
+
     def ignite(self, T0, p, Y0, t_end, rtol=1e-11, atol=1e-16):
         """Reference solution of the reactor by scipy's Radau (implicit
         Runge-Kutta, order 5) at tight tolerances; returns (T, Y) at t_end."""

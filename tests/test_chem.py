@@ -205,6 +205,77 @@ def test_reactor_rhs_conserves_enthalpy_and_elements(mech):
 IGN = os.path.join(os.path.dirname(__file__), "golden", "ignition_radau.npz")
 
 
+def _trajectory(mech, T0=1100.0, p=3 * 101325.0, phi=1.0, times=(0.0, 1e-6, 2e-5, 5e-5, 1e-4)):
+    """States of one ignition along scipy's Radau solution."""
+    from scipy.integrate import solve_ivp
+
+    y0 = np.concatenate([[T0], _mix(mech, phi)])
+    sol = solve_ivp(lambda t, y: mech.reactor_rhs(y, p), (0.0, max(times)), y0, method="Radau",
+                    rtol=1e-9, atol=1e-14, dense_output=True)
+    return p, np.array([sol.sol(t) for t in times])
+
+
+def test_reactor_jacobian_against_richardson_differences(mech):
+    """The restatement's analytic Jacobian of the reactor (reactor_jacobian)
+    against Richardson-extrapolated central differences of reactor_rhs along
+    an ignition: every column to 1e-5 of its norm (the differences are good to
+    about 1e-9 in most columns and 2e-6 in the water column, whose entries
+    cancel; a dropped chain-rule term is an O(1) error)."""
+    p, Ys = _trajectory(mech)
+    for v in Ys:
+        J = mech.reactor_jacobian(v, p)
+        n = v.size
+        for j in range(n):
+            h = 1e-5 * max(abs(v[j]), 1e-4 if j else 1.0)
+
+            def cd(hh):
+                a = v.copy()
+                b = v.copy()
+                a[j] += hh
+                b[j] -= hh
+                return (mech.reactor_rhs(a, p) - mech.reactor_rhs(b, p)) / (2 * hh)
+
+            ref = (4 * cd(h / 2) - cd(h)) / 3
+            assert np.abs(J[:, j] - ref).max() <= 1e-5 * np.abs(ref).max(), (j, v[0])
+
+
+@pytest.mark.gpu
+def test_gpu_reactor_rhs_and_jacobian_match_oracle(mech):
+    """The device reactor's right-hand side and analytic Jacobian
+    (astr_hip_reactor_rhs) against the restatement along an ignition: f to
+    1e-12 and every Jacobian column to 1e-10 of their norms."""
+    gm = AC.Mechanism(MECH)
+    p, Ys = _trajectory(mech)
+    f, J = gm.reactor_rhs(np.full(len(Ys), p), Ys, jacobian=True)
+    for i, v in enumerate(Ys):
+        f0 = mech.reactor_rhs(v, p)
+        assert np.abs(f[i] - f0).max() <= 1e-12 * np.abs(f0).max(), i
+        Jo = mech.reactor_jacobian(v, p)
+        for j in range(v.size):
+            assert np.abs(J[i][:, j] - Jo[:, j]).max() <= 1e-10 * np.abs(Jo[:, j]).max(), (i, j)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("lanes", [1, 16])
+def test_gpu_reactor_ignites_in_few_steps(mech, lanes):
+    """RODAS4 with the analytic Jacobian at the serving tolerance (rtol 1e-5,
+    atol 1e-10): a 1 ms ignition in a few hundred steps with few rejections,
+    ending within 1e-4 of scipy's Radau temperature."""
+    gm = AC.Mechanism(MECH)
+    cases = [(1100.0, 3 * 101325.0, 1.0), (1000.0, 101325.0, 0.8), (1300.0, 5 * 101325.0, 1.5),
+             (960.0, 101325.0, 0.6)]
+    pool = AC.ReactorPool(gm, len(cases), rtol=1e-5, atol=1e-10, lanes=lanes)
+    for k, (T0, p, phi) in enumerate(cases):
+        pool.set(k, T0, p, _mix(mech, phi), 1e-3)
+    st = pool.run(max_tries=5000)
+    assert np.all(st == AC.ReactorPool.DONE)
+    for k, (T0, p, phi) in enumerate(cases):
+        t, T, Y, nstep, nrej = pool.get(k)
+        ref = mech.ignite(T0, p, _mix(mech, phi), 1e-3, rtol=1e-9, atol=1e-14)
+        assert nstep < 600 and nrej < nstep // 4, (k, nstep, nrej)
+        assert abs(T / ref[0] - 1) < 1e-4, (k, T, ref[0])
+
+
 def test_ignition_fixture_matches_oracle(mech):
     """The stored Radau reference (tests/golden/make_ignition.py) is what the
     current restatement gives: case 1 recomputed at rtol 1e-9."""
@@ -218,7 +289,7 @@ def test_ignition_fixture_matches_oracle(mech):
 @pytest.mark.gpu
 @pytest.mark.parametrize("lanes", [1, 16])
 def test_gpu_reactor_converges_to_radau(mech, lanes):
-    """ROS2 on the GPU at rtol 1e-7 against scipy's Radau at 1e-10 (stored)
+    """RODAS4 on the GPU at rtol 1e-7 against scipy's Radau at 1e-10 (stored)
     for lean, stoichiometric and rich mixtures at 1 and 5 atm, to t = 1 ms,
     with one thread and with a team of 16 lanes per reactor."""
     z = np.load(IGN)
@@ -257,7 +328,7 @@ def test_gpu_reactor_team_equals_one_thread(mech):
         out[lanes] = [pool.get(k) for k in range(6)]
         pool.close()
     for a, b in zip(out[1], out[16]):
-        assert a[3] > 20
+        assert a[3] > 5
         assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) and a[3:] == b[3:]
 
 
