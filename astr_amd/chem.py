@@ -54,6 +54,8 @@ def _bind():
     L.astr_hip_reactor_release.argtypes = [C.c_void_p, C.c_int]
     L.astr_hip_reactor_rhs.argtypes = [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp]
     L.astr_hip_reactor_method.argtypes = [_dp, C.c_int]
+    L.astr_hip_srccomb.argtypes = [C.c_void_p] + [C.c_int] * 5 + [_dp] * 5
+    L.astr_hip_srccomb_device.argtypes = [C.c_void_p] + [C.c_int] * 5 + [C.c_void_p] * 6
     L._chem_bound = True
     return L
 
@@ -121,6 +123,35 @@ class Mechanism:
         return (wd, hm, cp, hk) if species_h else (wd, hm, cp)
 
     chemrate = rates
+
+    def srccomb(self, rho, tmp, spc, jacob, qrhs, hm, stream=0):
+        """srccomb (src/solver.F90:500-537): qrhs[5 + k] += wdot_k W_k * jacob at
+        every computed point, in place.  The arrays are the reference's
+        Fortran ones with their index order reversed (C order): rho, tmp,
+        jacob (km+1+2hm, jm+1+2hm, im+1+2hm), spc (ns, ...same), qrhs
+        (5 + ns, km+1, jm+1, im+1).  numpy arrays go through the host entry
+        point; astr_amd.hip.DeviceArray buffers in HBM through the device
+        one, on stream."""
+        nz, ny, nx = (d - 2 * hm for d in rho.shape)
+        want = {"tmp": tmp.shape, "jacob": jacob.shape}
+        for name, shp in want.items():
+            if tuple(shp) != tuple(rho.shape):
+                raise AstrError(f"srccomb: {name} shape {tuple(shp)} != rho's {tuple(rho.shape)}")
+        if tuple(spc.shape) != (self.ns,) + tuple(rho.shape):
+            raise AstrError("srccomb: spc must be (ns,) + rho's shape")
+        if tuple(qrhs.shape) != (5 + self.ns, nz, ny, nx):
+            raise AstrError("srccomb: qrhs must be (5 + ns, km+1, jm+1, im+1)")
+        dims = (nx - 1, ny - 1, nz - 1, hm, 5 + self.ns)
+        L = _bind()
+        if isinstance(rho, np.ndarray):
+            arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, tmp, spc, jacob)]
+            if not (qrhs.flags.c_contiguous and qrhs.dtype == np.float64):
+                raise AstrError("srccomb: qrhs must be a C-contiguous float64 array")
+            _chk(L.astr_hip_srccomb(self.h, *dims, *(_p(a) for a in arrs), _p(qrhs)))
+            return qrhs
+        _chk(L.astr_hip_srccomb_device(self.h, *dims, *(a.ptr for a in (rho, tmp, spc, jacob, qrhs)),
+                                       stream))
+        return qrhs
 
 
 def mixture(names, W, phi, c):

@@ -130,31 +130,22 @@ __device__ __forceinline__ void kinetics(const Mech* __restrict__ M, double T, d
   }
 }
 
-// NSA: the per-cell arrays' size.  NSA = NSMAX serves any mechanism (loops
-// to M->ns); an instance with NSA = the mechanism's species count keeps four
-// arrays of that size in registers instead of NSMAX, with the same arithmetic
-// in the same order.
+// The chemistry of one cell at (T, rho, y): net molar production rates w
+// (kmol/m^3/s), and the mixture cp (per R) and h (J/kg).  h_k is written to
+// hk[k * n + i] when hk is not null.  NSA: the per-cell arrays' size.
+// NSA = NSMAX serves any mechanism (loops to M->ns); an instance with NSA =
+// the mechanism's species count keeps four arrays of that size in registers
+// instead of NSMAX, with the same arithmetic in the same order.
 template <int NSA>
-__global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
-                                              const double* __restrict__ Tin,
-                                              const double* __restrict__ RPin, int input_is_p,
-                                              const double* __restrict__ Y, double* __restrict__ wdot,
-                                              double* __restrict__ hmix, double* __restrict__ cpmix,
-                                              double* __restrict__ hk) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void cell_rates(const Mech* __restrict__ M, double T, double rho,
+                                           const double* y, double* w, double& cp, double& h,
+                                           double* __restrict__ hk, long long n, long long i) {
   const int ns = NSA == NSMAX ? M->ns : NSA, nr = M->nr;
-  const double T = Tin[i];
-  double y[NSA], c[NSA], g[NSA], w[NSA];
-  double winv = 0.0;
-  for (int k = 0; k < ns; k++) {
-    y[k] = Y[k * n + i];
-    winv += y[k] / M->W[k];
-  }
-  const double rho = input_is_p ? RPin[i] / (kRgas * T * winv) : RPin[i];
+  double c[NSA], g[NSA];
   const double logT = log(T), invT = 1.0 / T;
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  double cp = 0.0, h = 0.0;
+  cp = 0.0;
+  h = 0.0;
   for (int k = 0; k < ns; k++) {
     c[k] = rho * y[k] / M->W[k];
     // NASA 7-coefficient polynomials, evaluated with both ranges' coefficients
@@ -211,9 +202,63 @@ __global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long l
     for (int j = 0; j < M->nre[r]; j++) w[M->rsp[r][j]] -= M->rnu[r][j] * q;
     for (int j = 0; j < M->npr[r]; j++) w[M->psp[r][j]] += M->pnu[r][j] * q;
   }
+}
+
+template <int NSA>
+__global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
+                                              const double* __restrict__ Tin,
+                                              const double* __restrict__ RPin, int input_is_p,
+                                              const double* __restrict__ Y, double* __restrict__ wdot,
+                                              double* __restrict__ hmix, double* __restrict__ cpmix,
+                                              double* __restrict__ hk) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ns = NSA == NSMAX ? M->ns : NSA;
+  const double T = Tin[i];
+  double y[NSA], w[NSA];
+  double winv = 0.0;
+  for (int k = 0; k < ns; k++) {
+    y[k] = Y[k * n + i];
+    winv += y[k] / M->W[k];
+  }
+  const double rho = input_is_p ? RPin[i] / (kRgas * T * winv) : RPin[i];
+  double cp, h;
+  cell_rates<NSA>(M, T, rho, y, w, cp, h, hk, n, i);
   for (int k = 0; k < ns; k++) wdot[k * n + i] = w[k] * M->W[k];
   if (hmix) hmix[i] = h;
   if (cpmix) cpmix[i] = cp * kRgas;
+}
+
+// srccomb (solver.F90:500-537): at every computed point (i, j, k) of
+// 0:im x 0:jm x 0:km, chemrate of (rho, tmp, spc) (wirate = wdot_k W_k,
+// thermchem.F90:1553-1575) times jacob added to qrhs(i,j,k,6:5+ns).  rho,
+// tmp, jacob: (-hm:im+hm, -hm:jm+hm, -hm:km+hm), spc: the same with a last
+// species index, qrhs: (0:im, 0:jm, 0:km, 1:numq), all column-major as the
+// reference allocates them (commarray.F90).
+template <int NSA>
+__global__ __launch_bounds__(256) void k_srccomb(const Mech* __restrict__ M, int im, int jm, int km,
+                                                 int hm, const double* __restrict__ rho,
+                                                 const double* __restrict__ tmp,
+                                                 const double* __restrict__ spc,
+                                                 const double* __restrict__ jacob,
+                                                 double* __restrict__ qrhs) {
+  const long long nx = im + 1, ny = jm + 1, ncell = nx * ny * (km + 1);
+  const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= ncell) return;
+  const int ns = NSA == NSMAX ? M->ns : NSA;
+  const long long i = c % nx, j = (c / nx) % ny, k = c / (nx * ny);
+  const long long sx = im + 1 + 2 * hm, sy = jm + 1 + 2 * hm, sz = km + 1 + 2 * hm;
+  const long long hp = ((k + hm) * sy + (j + hm)) * sx + (i + hm);
+  const long long hs = sx * sy * sz;
+  double y[NSA], w[NSA];
+  for (int s = 0; s < ns; s++) y[s] = spc[s * hs + hp];
+  double cp, h;
+  cell_rates<NSA>(M, tmp[hp], rho[hp], y, w, cp, h, nullptr, 0, 0);
+  const double jac = jacob[hp];
+  for (int s = 0; s < ns; s++) {
+    double* q = qrhs + (5 + s) * ncell + c;
+    *q = *q + (w[s] * M->W[s]) * jac;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1340,6 +1385,60 @@ int astr_hip_chem_rates(astr_chem* c, long long n, const double* T, const double
         (h_k && hipMemcpy(h_k, dK, b1 * ns, hipMemcpyDeviceToHost) != hipSuccess))
       rc = cfail("copy from device failed");
   }
+  (void)hipFree(d);
+  return rc;
+}
+
+static int srccomb_check(astr_chem* c, int im, int jm, int km, int hm, int numq, const void* rho,
+                         const void* tmp, const void* spc, const void* jacob, const void* qrhs) {
+  if (!c) return cfail("srccomb: null mechanism");
+  if (im < 0 || jm < 0 || km < 0 || hm < 0) return cfail("srccomb: negative extent or halo");
+  if (numq != 5 + c->host.ns)
+    return cfail("srccomb: numq must be 5 + the species count (qrhs(6:numq) takes the rates)");
+  if (!rho || !tmp || !spc || !jacob || !qrhs) return cfail("srccomb: null buffer");
+  return 0;
+}
+
+int astr_hip_srccomb_device(astr_chem* c, int im, int jm, int km, int hm, int numq, const double* rho,
+                            const double* tmp, const double* spc, const double* jacob, double* qrhs,
+                            void* stream) {
+  if (int rc = srccomb_check(c, im, jm, km, hm, numq, rho, tmp, spc, jacob, qrhs)) return rc;
+  CHIP_OK(hipSetDevice(c->device));
+  const long long ncell = (long long)(im + 1) * (jm + 1) * (km + 1);
+  const int bs = 256;
+  const dim3 grid((unsigned)((ncell + bs - 1) / bs));
+  if (c->host.ns == 9)
+    hipLaunchKernelGGL(k_srccomb<9>, grid, dim3(bs), 0, (hipStream_t)stream, c->dev, im, jm, km, hm,
+                       rho, tmp, spc, jacob, qrhs);
+  else
+    hipLaunchKernelGGL(k_srccomb<NSMAX>, grid, dim3(bs), 0, (hipStream_t)stream, c->dev, im, jm, km,
+                       hm, rho, tmp, spc, jacob, qrhs);
+  CHIP_OK(hipGetLastError());
+  return 0;
+}
+
+int astr_hip_srccomb(astr_chem* c, int im, int jm, int km, int hm, int numq, const double* rho,
+                     const double* tmp, const double* spc, const double* jacob, double* qrhs) {
+  if (int rc = srccomb_check(c, im, jm, km, hm, numq, rho, tmp, spc, jacob, qrhs)) return rc;
+  CHIP_OK(hipSetDevice(c->device));
+  const size_t nh = (size_t)(im + 1 + 2 * hm) * (jm + 1 + 2 * hm) * (km + 1 + 2 * hm);
+  const size_t nq = (size_t)(im + 1) * (jm + 1) * (km + 1) * numq;
+  const size_t ns = c->host.ns;
+  double* d = nullptr;
+  CHIP_OK(hipMalloc(&d, ((3 + ns) * nh + nq) * sizeof(double)));
+  double *dr = d, *dt = dr + nh, *dj = dt + nh, *ds = dj + nh, *dq = ds + ns * nh;
+  const size_t b = nh * sizeof(double);
+  int rc = 0;
+  if (hipMemcpy(dr, rho, b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dt, tmp, b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dj, jacob, b, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(ds, spc, b * ns, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(dq, qrhs, nq * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+    rc = cfail("srccomb: copy to device failed");
+  if (!rc) rc = astr_hip_srccomb_device(c, im, jm, km, hm, numq, dr, dt, ds, dj, dq, nullptr);
+  if (!rc && hipDeviceSynchronize() != hipSuccess) rc = cfail("srccomb: kernel failed");
+  if (!rc && hipMemcpy(qrhs, dq, nq * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = cfail("srccomb: copy from device failed");
   (void)hipFree(d);
   return rc;
 }

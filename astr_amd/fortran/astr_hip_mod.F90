@@ -229,6 +229,14 @@ module astr_hip_capi
       type(c_ptr), value :: y, wdot, hmix, cpmix, hk
       integer(c_int) :: astr_hip_chem_rates
     end function
+    function astr_hip_srccomb(chem, im, jm, km, hm, numq, rho, tmp, spc, jacob, qrhs) &
+        bind(C, name='astr_hip_srccomb')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: chem
+      integer(c_int), value :: im, jm, km, hm, numq
+      type(c_ptr), value :: rho, tmp, spc, jacob, qrhs
+      integer(c_int) :: astr_hip_srccomb
+    end function
   end interface
 
 contains
@@ -507,6 +515,21 @@ contains
                                        c_loc(spc), c_loc(wi), c_null_ptr, c_null_ptr, c_null_ptr), &
                    'chemrate')
   end subroutine chemrate_cells
+
+  ! srccomb (solver.F90:500-537) over the reference's arrays, passed
+  ! explicitly: qrhs(i,j,k,6:numq) += chemrate(rho, tmp, spc) * jacob at every
+  ! point of 0:im x 0:jm x 0:km, in one launch
+  subroutine srccomb(im, jm, km, hm, numq, rho, tmp, spc, jacob, qrhs)
+    integer, intent(in) :: im, jm, km, hm, numq
+    real(8), intent(in), target :: rho(-hm:im+hm, -hm:jm+hm, -hm:km+hm), &
+                                   tmp(-hm:im+hm, -hm:jm+hm, -hm:km+hm), &
+                                   spc(-hm:im+hm, -hm:jm+hm, -hm:km+hm, 1:num_species), &
+                                   jacob(-hm:im+hm, -hm:jm+hm, -hm:km+hm)
+    real(8), intent(inout), target :: qrhs(0:im, 0:jm, 0:km, 1:numq)
+    call chemcheck(astr_hip_srccomb(mixture, int(im, c_int), int(jm, c_int), int(km, c_int), &
+                                    int(hm, c_int), int(numq, c_int), c_loc(rho), c_loc(tmp), &
+                                    c_loc(spc), c_loc(jacob), c_loc(qrhs)), 'srccomb')
+  end subroutine srccomb
 
   ! cpeval (thermchem.F90:1342-1428): cp_mass or intEnergy_mass at (tmp,
   ! prefgb, spc), exactly one of cp, eng present

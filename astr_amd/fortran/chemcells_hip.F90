@@ -11,6 +11,11 @@
 !              from per-cell cpeval / enthpy calls; T_e the temperature that
 !              temperature_calc recovers from eng starting at 1000 K; rgc
 !              from rgcmix_scar; Xmole from convertxiyi(Y2X)
+!
+!        chemcells_hip MECHANISM.xml CELLS.txt OUT.txt GRID.txt QRHS.txt
+!   also srccomb over a block: GRID.txt holds "im jm km hm", then rho, tmp,
+!   spc, jacob and qrhs in Fortran array order (the reference's shapes);
+!   QRHS.txt receives qrhs after srccomb, one value a line, in array order
 !-----------------------------------------------------------------------
 program chemcells_hip
   use astr_hip_thermchem
@@ -20,6 +25,9 @@ program chemcells_hip
   real(8), allocatable :: den(:), tmp(:), spc(:,:), wi(:,:), hi(:), row(:)
   real(8) :: cp, eng, te
   real(8), allocatable :: xm(:)
+  character(len=512) :: fgrid, fqrhs
+  integer :: im, jm, km, hm
+  real(8), allocatable :: grho(:,:,:), gtmp(:,:,:), gspc(:,:,:,:), gjac(:,:,:), gq(:,:,:,:)
 
   call get_command_argument(1, mech)
   call get_command_argument(2, fin)
@@ -53,5 +61,21 @@ program chemcells_hip
         te, rgcmix_scar(spc(i, :)), (xm(k), k = 1, num_species)
   end do
   close(22)
+
+  if (command_argument_count() >= 5) then
+    call get_command_argument(4, fgrid)
+    call get_command_argument(5, fqrhs)
+    open(23, file=trim(fgrid), status='old')
+    read(23, *) im, jm, km, hm
+    allocate(grho(-hm:im+hm, -hm:jm+hm, -hm:km+hm), gtmp(-hm:im+hm, -hm:jm+hm, -hm:km+hm), &
+             gspc(-hm:im+hm, -hm:jm+hm, -hm:km+hm, num_species), &
+             gjac(-hm:im+hm, -hm:jm+hm, -hm:km+hm), gq(0:im, 0:jm, 0:km, 5 + num_species))
+    read(23, *) grho, gtmp, gspc, gjac, gq
+    close(23)
+    call srccomb(im, jm, km, hm, 5 + num_species, grho, gtmp, gspc, gjac, gq)
+    open(24, file=trim(fqrhs), status='replace')
+    write(24, '(ES25.17E3)') gq
+    close(24)
+  end if
   call chemend
 end program chemcells_hip

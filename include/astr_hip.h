@@ -319,6 +319,20 @@ int astr_hip_chem_rates_device(astr_chem *chem, long long n, const double *T,
                                double *wdot, double *h_mix, double *cp_mix, double *h_k,
                                void *stream);
 
+/* srccomb (solver.F90:500-537): the chemical source of every computed point
+ * added to the species equations, qrhs(i,j,k,6:numq) += wirate * jacob(i,j,k),
+ * wirate = chemrate(rho, tmp, spc) = wdot_k W_k (kg/m^3/s), for i, j, k in
+ * 0:im, 0:jm, 0:km.  The arrays are the reference's (commarray.F90),
+ * column-major: rho, tmp, jacob (-hm:im+hm, -hm:jm+hm, -hm:km+hm); spc the
+ * same with a last index 1:nsp; qrhs (0:im, 0:jm, 0:km, 1:numq).  numq must
+ * be 5 + nsp.  The first form takes host buffers (qrhs updated in place); the
+ * _device form takes buffers in HBM and is stream-ordered on stream. */
+int astr_hip_srccomb(astr_chem *chem, int im, int jm, int km, int hm, int numq, const double *rho,
+                     const double *tmp, const double *spc, const double *jacob, double *qrhs);
+int astr_hip_srccomb_device(astr_chem *chem, int im, int jm, int km, int hm, int numq,
+                            const double *rho, const double *tmp, const double *spc,
+                            const double *jacob, double *qrhs, void *stream);
+
 /* temperature_calc (thermchem.F90:1775-1786; Cantera setState_UV): T from
  * the mass-specific internal energy e (J/kg) and Y (nsp x n, SoA); T holds
  * the initial guesses on entry and the temperatures on return.  Newton on

@@ -63,7 +63,8 @@ def test_tgvsolver_hip_reproduces_state_ref_128(tmp_path):
 
 @pytest.mark.gpu
 def test_chemcells_hip_matches_oracle(tmp_path):
-    """chemread / chemrate / cpeval / enthpy through the Fortran module."""
+    """chemread / chemrate / cpeval / enthpy / srccomb through the Fortran
+    module."""
     m = OC.Mechanism(MECH)
     n = 2000
     T, p, Y = AC.sample_cells(m.names, m.W, n, 21)
@@ -72,7 +73,22 @@ def test_chemcells_hip_matches_oracle(tmp_path):
         fh.write(f"{n}\n")
         for i in range(n):
             fh.write(" ".join(repr(float(x)) for x in [rho[i], T[i], *Y[:, i]]) + "\n")
-    _run([os.path.join(FDIR, "chemcells_hip"), MECH, "cells.txt", "out.txt"], tmp_path)
+    # srccomb over a block in the reference's array shapes
+    from test_chem import srccomb_block, srccomb_expected
+    im, jm, km, hm = 7, 6, 5, 2
+    grho, gT, gY, gjac, gq = srccomb_block(m, im, jm, km, hm, 41)
+    with open(tmp_path / "grid.txt", "w") as fh:
+        fh.write(f"{im} {jm} {km} {hm}\n")
+        for a in (grho, gT, gY, gjac, gq):
+            fh.write("\n".join(repr(float(x)) for x in a.ravel()) + "\n")
+    _run([os.path.join(FDIR, "chemcells_hip"), MECH, "cells.txt", "out.txt", "grid.txt", "qrhs.txt"],
+         tmp_path)
+    gq1 = np.loadtxt(tmp_path / "qrhs.txt").reshape(gq.shape)
+    want = srccomb_expected(m, grho, gT, gY, gjac, gq, hm)
+    assert np.array_equal(gq1[:5], gq[:5])
+    for s in range(m.ns):
+        scale = max(np.abs(want[5 + s] - gq[5 + s]).max(), 1e-300)
+        assert np.abs(gq1[5 + s] - want[5 + s]).max() <= 1e-12 * scale + 4e-16 * np.abs(gq[5 + s]).max()
     out = np.loadtxt(tmp_path / "out.txt")
     ns = m.ns
     wi, cp, eng, hi = out[:, :ns].T, out[:, ns], out[:, ns + 1], out[:, ns + 2:2 * ns + 2].T
