@@ -12,7 +12,7 @@ import sys
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(PKG, "csrc")
-SOURCES = ["flow.hip", "chem.hip", "upwind.hip"]
+SOURCES = ["flow.hip", "chem.hip", "upwind.hip", "probe.hip"]
 # passes.hip is compiled once per launch shape (-DASTR_SHAPE_ID=k)
 PASS_SHAPES = 13
 OUT = os.path.join(PKG, "libastr_hip.so")

@@ -94,7 +94,7 @@ class Mechanism:
 
     def reactor_rhs(self, p, y, jacobian=False):
         """The reactor's d(T, Y)/dt at states y (n, ns+1) and pressures p (n,),
-        and with jacobian=True the integrator's forward-difference Jacobian
+        and with jacobian=True the analytic Jacobian the integrator uses
         (n, ns+1, ns+1)."""
         p = np.ascontiguousarray(np.atleast_1d(p), dtype=np.float64)
         y = np.ascontiguousarray(np.atleast_2d(y), dtype=np.float64)

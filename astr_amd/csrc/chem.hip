@@ -130,22 +130,93 @@ __device__ __forceinline__ void kinetics(const Mech* __restrict__ M, double T, d
   }
 }
 
-// The chemistry of one cell at (T, rho, y): net molar production rates w
-// (kmol/m^3/s), and the mixture cp (per R) and h (J/kg).  h_k is written to
-// hk[k * n + i] when hk is not null.  NSA: the per-cell arrays' size.
-// NSA = NSMAX serves any mechanism (loops to M->ns); an instance with NSA =
-// the mechanism's species count keeps four arrays of that size in registers
-// instead of NSMAX, with the same arithmetic in the same order.
+// The net molar production rates w (kmol/m^3/s) of one cell at (T, rho, y):
+// k_chem's arithmetic, in the same order, without the mixture cp and h.
+// k_chem keeps its own copy: with this function inlined in it, its register
+// peak rose from 164 VGPRs to 256 and its occupancy fell from three waves per
+// SIMD to one.  NSA as in k_chem.
 template <int NSA>
 __device__ __forceinline__ void cell_rates(const Mech* __restrict__ M, double T, double rho,
-                                           const double* y, double* w, double& cp, double& h,
-                                           double* __restrict__ hk, long long n, long long i) {
+                                           const double* y, double* w) {
   const int ns = NSA == NSMAX ? M->ns : NSA, nr = M->nr;
   double c[NSA], g[NSA];
   const double logT = log(T), invT = 1.0 / T;
   const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
-  cp = 0.0;
-  h = 0.0;
+  for (int k = 0; k < ns; k++) {
+    c[k] = rho * y[k] / M->W[k];
+    double hrt, sr;
+    {
+      double hr[2], srr[2];
+#pragma unroll
+      for (int q = 0; q < 2; q++) {
+        const double* a = q == 0 ? M->lo[k] : M->hi[k];
+        hr[q] = a[0] + a[1] * T / 2.0 + a[2] * T2 / 3.0 + a[3] * T3 / 4.0 + a[4] * T4 / 5.0 +
+                a[5] * invT;
+        srr[q] = a[0] * logT + a[1] * T + a[2] * T2 / 2.0 + a[3] * T3 / 3.0 + a[4] * T4 / 4.0 + a[6];
+      }
+      const int q = T <= M->tmid[k] ? 0 : 1;
+      hrt = q == 0 ? hr[0] : hr[1];
+      sr = q == 0 ? srr[0] : srr[1];
+    }
+    g[k] = hrt - sr;
+    w[k] = 0.0;
+  }
+  const double lc0 = log(M->p0 / (kRgas * T));  // ln of the standard concentration
+  for (int r = 0; r < nr; r++) {
+    const double kf = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT);
+    double fwd = kf;
+    double dg = 0.0;
+    for (int j = 0; j < M->nre[r]; j++) {
+      fwd *= conc_pow(c[M->rsp[r][j]], M->rnu[r][j]);
+      dg -= M->rnu[r][j] * g[M->rsp[r][j]];
+    }
+    double rv = 0.0;
+    if (M->rev[r]) {
+      double pp = 1.0;
+      for (int j = 0; j < M->npr[r]; j++) {
+        pp *= conc_pow(c[M->psp[r][j]], M->pnu[r][j]);
+        dg += M->pnu[r][j] * g[M->psp[r][j]];
+      }
+      // kr = kf / Kc, Kc = exp(-dG/RT) * (p0/RT)^dnu
+      const double kr = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT + dg - M->dnu[r] * lc0);
+      rv = kr * pp;
+    }
+    double q = fwd - rv;
+    if (M->tb[r]) {
+      double cm = 0.0;
+      for (int k = 0; k < ns; k++) cm += M->eff[r][k] * c[k];
+      q *= cm;
+    }
+    for (int j = 0; j < M->nre[r]; j++) w[M->rsp[r][j]] -= M->rnu[r][j] * q;
+    for (int j = 0; j < M->npr[r]; j++) w[M->psp[r][j]] += M->pnu[r][j] * q;
+  }
+}
+
+// NSA: the per-cell arrays' size.  NSA = NSMAX serves any mechanism (loops
+// to M->ns); an instance with NSA = the mechanism's species count keeps four
+// arrays of that size in registers instead of NSMAX, with the same arithmetic
+// in the same order.
+template <int NSA>
+__global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
+                                              const double* __restrict__ Tin,
+                                              const double* __restrict__ RPin, int input_is_p,
+                                              const double* __restrict__ Y, double* __restrict__ wdot,
+                                              double* __restrict__ hmix, double* __restrict__ cpmix,
+                                              double* __restrict__ hk) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int ns = NSA == NSMAX ? M->ns : NSA, nr = M->nr;
+  const double T = Tin[i];
+  double y[NSA], c[NSA], g[NSA], w[NSA];
+  double winv = 0.0;
+  for (int k = 0; k < ns; k++) {
+    y[k] = Y[k * n + i];
+    winv += y[k] / M->W[k];
+  }
+  const double rho = input_is_p ? RPin[i] / (kRgas * T * winv) : RPin[i];
+  const double logT = log(T), invT = 1.0 / T;
+  const double T2 = T * T, T3 = T2 * T, T4 = T3 * T;
+  double cp = 0.0, h = 0.0;
   for (int k = 0; k < ns; k++) {
     c[k] = rho * y[k] / M->W[k];
     // NASA 7-coefficient polynomials, evaluated with both ranges' coefficients
@@ -202,28 +273,6 @@ __device__ __forceinline__ void cell_rates(const Mech* __restrict__ M, double T,
     for (int j = 0; j < M->nre[r]; j++) w[M->rsp[r][j]] -= M->rnu[r][j] * q;
     for (int j = 0; j < M->npr[r]; j++) w[M->psp[r][j]] += M->pnu[r][j] * q;
   }
-}
-
-template <int NSA>
-__global__ __launch_bounds__(256) void k_chem(const Mech* __restrict__ M, long long n,
-                                              const double* __restrict__ Tin,
-                                              const double* __restrict__ RPin, int input_is_p,
-                                              const double* __restrict__ Y, double* __restrict__ wdot,
-                                              double* __restrict__ hmix, double* __restrict__ cpmix,
-                                              double* __restrict__ hk) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int ns = NSA == NSMAX ? M->ns : NSA;
-  const double T = Tin[i];
-  double y[NSA], w[NSA];
-  double winv = 0.0;
-  for (int k = 0; k < ns; k++) {
-    y[k] = Y[k * n + i];
-    winv += y[k] / M->W[k];
-  }
-  const double rho = input_is_p ? RPin[i] / (kRgas * T * winv) : RPin[i];
-  double cp, h;
-  cell_rates<NSA>(M, T, rho, y, w, cp, h, hk, n, i);
   for (int k = 0; k < ns; k++) wdot[k * n + i] = w[k] * M->W[k];
   if (hmix) hmix[i] = h;
   if (cpmix) cpmix[i] = cp * kRgas;
@@ -252,8 +301,7 @@ __global__ __launch_bounds__(256) void k_srccomb(const Mech* __restrict__ M, int
   const long long hs = sx * sy * sz;
   double y[NSA], w[NSA];
   for (int s = 0; s < ns; s++) y[s] = spc[s * hs + hp];
-  double cp, h;
-  cell_rates<NSA>(M, tmp[hp], rho[hp], y, w, cp, h, nullptr, 0, 0);
+  cell_rates<NSA>(M, tmp[hp], rho[hp], y, w);
   const double jac = jacob[hp];
   for (int s = 0; s < ns; s++) {
     double* q = qrhs + (5 + s) * ncell + c;
@@ -915,11 +963,11 @@ __global__ __launch_bounds__(64) void k_reactor_eval(const Mech* __restrict__ M,
 
 // ---------------------------------------------------------------------------
 // The same reactor integrated by a team of 16 lanes (one DPP row), four
-// reactors per wavefront, for latency: the step's N+1 right-hand sides that
-// do not depend on each other (the unperturbed state and the N
-// forward-difference columns) are evaluated by separate lanes at once, and
+// reactors per wavefront, for latency: the step's Jacobian columns, which
+// do not depend on each other, are evaluated by separate lanes at once (lane
+// j the analytic column j, after every lane has the right-hand side), and
 // the LU elimination runs one row per lane.  The pivot search, both
-// triangular solves, the second stage and the step control are done by every
+// triangular solves, the later stages and the step control are done by every
 // lane of the team, in the one-thread kernel's order, so each matrix entry,
 // factor and solution component goes through the same operations as there.
 // A team's lanes hold identical scalars (t, h, err), so its control flow is

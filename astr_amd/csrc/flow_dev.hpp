@@ -550,6 +550,27 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
       for (int m = 0; m < 5; m++) qv[m][t] = ldg(a.q + m * fs, S.g[t]);
 
+    // main path: the stored primitives the gradient solves take (pv) and
+    // the convective part's u_dir and p are loaded with q, at the start, so
+    // that every load of the pass is in flight at once (these passes run at
+    // two waves per SIMD, whose register budget holds them).  The split last
+    // pass's gradient half loads each just before its solve: the registers
+    // would cost it its third wave per SIMD.
+    constexpr bool PF = MAIN && DO_GRAD && PROG != GL_GRAD;
+    double pv[4][NPT], ud[NPT], pp[NPT];
+    if constexpr (PF) {
+#pragma unroll
+      for (int t = 0; t < NPT; t++)
+#pragma unroll
+        for (int c = 0; c < 4; c++) pv[c][t] = ldg(a.pr + (c < 3 ? 1 + c : 5) * fs, S.g[t]);
+      if (DO_CONV && !a.uwd) {
+#pragma unroll
+        for (int t = 0; t < NPT; t++) {
+          pp[t] = ldg(a.pr + 4 * fs, S.g[t]);
+          ud[t] = ldg(a.pr + (1 + dir) * fs, S.g[t]);
+        }
+      }
+    }
     if constexpr (DO_GRAD) {
     // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
     double gz[4][NPT];
@@ -560,7 +581,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         if constexpr (MAIN) {
           // the stored u, v, w or T (gradcal, src/comsolver.F90:280-300);
           // the halo copy from the primitives of the halo's q (qswap's q2fvar)
-          T[kLineOff + S.s[t]] = ldg(a.pr + (c < 3 ? 1 + c : 5) * fs, S.g[t]);
+          T[kLineOff + S.s[t]] = PF ? pv[c][t] : ldg(a.pr + (c < 3 ? 1 + c : 5) * fs, S.g[t]);
           if (S.h[t] != kDummy) {
             double qh[5];
             halo_q<C>(a, S, t, dir, qv, qh);
@@ -714,13 +735,15 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     if (MAIN && a.uwd) return;
     if constexpr (DO_CONV) {
     // convective flux derivatives along dir (convection, tgvsolver.F90:539-633)
-    double ud[NPT], pp[NPT];
 #pragma unroll
     for (int t = 0; t < NPT; t++) {
       if constexpr (MAIN) {
-        // convrsdcal6 takes the stored vel and prs (src/solver.F90:2196-2207)
-        pp[t] = ldg(a.pr + 4 * fs, S.g[t]);
-        ud[t] = ldg(a.pr + (1 + dir) * fs, S.g[t]);
+        // convrsdcal6 takes the stored vel and prs (src/solver.F90:2196-2207),
+        // loaded at the start where the pass has gradient solves
+        if constexpr (!PF) {
+          pp[t] = ldg(a.pr + 4 * fs, S.g[t]);
+          ud[t] = ldg(a.pr + (1 + dir) * fs, S.g[t]);
+        }
       } else {
         pp[t] = pressure_of(a, qv[0][t], qv[1][t], qv[2][t], qv[3][t], qv[4][t]);
         ud[t] = (dir == 0 ? qv[1][t] : (dir == 1 ? qv[2][t] : qv[3][t])) / qv[0][t];

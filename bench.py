@@ -173,23 +173,51 @@ def chem_batch(ncells=1 << 20, reps=20):
             "mechanism": "Jachimowski88_H2-9-19 (9 species, 19 reactions)"}
 
 
-def copy_probe(nbytes, reps=3):
-    """A plain device-to-device copy of nbytes, reps times: the calibration
-    kernel for the FETCH_SIZE / WRITE_SIZE counters (it reads and writes
-    exactly nbytes per launch)."""
+def copy_probe(nbytes, reps=5):
+    """The achievable HBM rate of a plain FP64 copy of nbytes: the library's
+    streaming probe (astr_hip_stream_probe, 1, 2 and 4 doubles per lane per
+    access, nontemporal) and the runtime's own device-to-device copy, each
+    timed over reps launches after one untimed launch.  The probe reads and
+    writes exactly nbytes per launch, so it also calibrates the FETCH_SIZE /
+    WRITE_SIZE counters."""
+    import ctypes as C
+
+    import astr_amd as A
     from astr_amd import hip
 
-    a = hip.DeviceArray((nbytes // 8,))
-    b = hip.DeviceArray((nbytes // 8,))
+    L = A.lib()
+    L.astr_hip_stream_probe.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_int, C.c_void_p]
+    L.astr_hip_probe_last_error.restype = C.c_char_p
+    n = nbytes // 16 * 2
+    a = hip.DeviceArray((n,))
+    b = hip.DeviceArray((n,))
     hip.synchronize()
-    e0, e1 = hip.Event(), hip.Event()
-    e0.record()
-    for _ in range(reps):
-        hip.copy_device(b, a, nbytes)
-    e1.record()
-    ms = e0.elapsed_ms(e1) / reps
-    return {"bytes": nbytes, "launches": reps, "ms_per_copy": round(ms, 4),
-            "GBs_read_plus_write": round(2 * nbytes / (ms * 1e-3) / 1e9, 1)}
+    out = {"bytes": 8 * n, "launches": reps}
+
+    def timed(launch):
+        launch()
+        hip.synchronize()
+        e0, e1 = hip.Event(), hip.Event()
+        e0.record()
+        for _ in range(reps):
+            launch()
+        e1.record()
+        ms = e0.elapsed_ms(e1) / reps
+        return {"ms": round(ms, 4), "GBs_read_plus_write": round(2 * 8 * n / (ms * 1e-3) / 1e9, 1)}
+
+    def probe(w):
+        def go():
+            if L.astr_hip_stream_probe(a.ptr, b.ptr, n, w, None) != 0:
+                raise RuntimeError(L.astr_hip_probe_last_error().decode())
+        return go
+
+    for w in (1, 2, 4):
+        out[f"stream_probe_w{w}"] = timed(probe(w))
+    out["hipMemcpy"] = timed(lambda: hip.copy_device(b, a, 8 * n))
+    best = max(v["GBs_read_plus_write"] for k, v in out.items() if isinstance(v, dict))
+    out["best_GBs"] = best
+    out["best_frac_of_peak"] = round(best / HBM_PEAK_GBS, 4)
+    return out
 
 
 def batch_form(ncases=64, steps=100, seed=2026):
@@ -457,8 +485,10 @@ def main():
                     help="apply the launch selection of an earlier bench line (a file holding "
                          "that JSON line) instead of autotuning; counter runs use it so that "
                          "they measure the kernels the timed run chose")
-    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
-                    help="also time a device copy of BYTES (counter calibration)")
+    ap.add_argument("--copy-probe", type=int, default=1 << 31, metavar="BYTES",
+                    help="bytes of the HBM copy probe (the achievable-rate reference and the "
+                         "counter calibration); 0 skips it")
+    ap.add_argument("--probe-only", action="store_true", help="only the HBM copy probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: the whole {args.n}^3 domain sits in one MI355X's HBM; "
@@ -472,6 +502,9 @@ def main():
         return
     if args.serve_only:
         print(json.dumps({"serving": serve_form()}), flush=True)
+        return
+    if args.probe_only:
+        print(json.dumps({"hbm_copy_probe": copy_probe(args.copy_probe or 1 << 31)}), flush=True)
         return
     n = args.n
     dt = 5e-4 * 256.0 / n
@@ -571,7 +604,12 @@ def main():
     if not args.no_serve:
         out["serving"] = serve_form()
     if args.copy_probe:
-        out["copy_probe"] = copy_probe(args.copy_probe)
+        probe = copy_probe(args.copy_probe)
+        out["hbm_copy_probe"] = probe
+        # the dominant kernel and the step against what a plain copy reaches
+        out["roofline"]["frac_of_copy_probe"] = round(out["roofline"]["achieved"] / probe["best_GBs"], 4)
+        out["step_roofline"]["frac_of_copy_probe"] = round(out["step_roofline"]["achieved_GBs"]
+                                                           / probe["best_GBs"], 4)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)

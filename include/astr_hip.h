@@ -38,7 +38,8 @@ extern "C" {
  * gradients and fluxes from the primitives the last updatefvar stored
  * (src/comsolver.F90:244-497, src/solver.F90:2173-2873), fluxes carrying
  * the metrics J and dxi, the RK3 update (q J-weighted), then updatefvar.
- * The metrics are those of a uniform Cartesian grid (astr_hip_set_metrics).
+ * The metrics are any grid's J and dxi (astr_hip_set_metrics); periodic or
+ * closed directions per astr_hip_set_boundary.
  * Batches take ASTR_SEM_MINIAPP cases only. */
 #define ASTR_SEM_MINIAPP 0
 #define ASTR_SEM_MAIN 1
@@ -388,10 +389,21 @@ int astr_hip_reactor_get(astr_reactor_pool *pool, int slot, double *t, double *T
 int astr_hip_reactor_release(astr_reactor_pool *pool, int slot);
 int astr_hip_reactor_method(double *coef, int n);
 /* The reactor's right-hand side d(T, Y)/dt at n states (p[n], y and f n x
- * (ns+1) row-major) and, if jac is not null, the integrator's
- * forward-difference Jacobian (n x (ns+1) x (ns+1), row k = component,
- * column j = perturbed variable); 9-species mechanisms. */
+ * (ns+1) row-major) and, if jac is not null, the analytic Jacobian the
+ * integrator uses (n x (ns+1) x (ns+1), row k = component, column j =
+ * variable); 9-species mechanisms. */
 int astr_hip_reactor_rhs(astr_chem *chem, int n, const double *p, const double *y, double *f, double *jac);
+
+/* ------------------------------------------------------------------------
+ * HBM streaming probe (no reference counterpart): dst[0:n] = src[0:n] in
+ * FP64 with width doubles per lane per access (1, or 2 and 4 as 16-byte
+ * loads; even n and 16-byte aligned buffers), nontemporal, stream-ordered.
+ * It reads and writes exactly 8n bytes each: the achievable-bandwidth
+ * reference of the GPU and the calibration of the FETCH_SIZE / WRITE_SIZE
+ * counters.  Errors: astr_hip_probe_last_error().
+ * ------------------------------------------------------------------------ */
+const char *astr_hip_probe_last_error(void);
+int astr_hip_stream_probe(const double *src, double *dst, long long n, int width, void *stream);
 
 #ifdef __cplusplus
 }
