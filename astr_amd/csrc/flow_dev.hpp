@@ -224,13 +224,13 @@ __device__ __forceinline__ void lds_chunk_coef_f(ChunkCoef<C::R, true>& CF, cons
   CF.c3last = c3last;
 }
 
-// A line solve of a pass: the periodic form, or (main path) the closed form
-// of a direction with physical ends
-template <class C, bool FILT, bool MAIN>
+// A line solve of a pass: the periodic form, or (main path, general-grid
+// kernels only) the closed form of a direction with physical ends
+template <class C, bool FILT, bool GEN>
 __device__ __forceinline__ void solve_pass_line(const double* Tl, double* Xl, int n, int p,
                                                 const ChunkCoef<C::R, FILT>& CC, const FilterCoef& fc,
                                                 int bnd) {
-  if constexpr (MAIN) {
+  if constexpr (GEN) {
     if (bnd) {
       solve_line<C::P, C::R, FILT, true>(Tl, Xl, n, p, CC, fc, bnd);
       return;
@@ -441,9 +441,11 @@ __device__ __forceinline__ MetRow met_row(const FlowArgs& a, int dir, unsigned g
   for (int nn = 0; nn < 3; nn++) M.r[nn] = ldg(a.met + (1 + dir + 3 * nn) * a.fs, g);
   return M;
 }
-// the Jacobian of a point, either grid
+// the Jacobian of a point, either grid (GEN: a general-grid kernel, which
+// alone reads per-point metrics)
+template <bool GEN>
 __device__ __forceinline__ double jac_of(const FlowArgs& a, unsigned g) {
-  return a.met ? ldg(a.met, g) : a.jac;
+  return GEN && a.met ? ldg(a.met, g) : a.jac;
 }
 
 // convrsdcal6 (src/solver.F90:2196-2207) with the full metric row:
@@ -516,7 +518,7 @@ __device__ __forceinline__ double rk_update_main(const StageArgs& sa, double dt,
 // and their halo copies from the filtered q (qswap), the fluxes carry the
 // metrics J and dxi (convrsdcal6, diffrsdcal6), and the RK pass ends with
 // updatefvar instead of the filter.
-template <class C, int PROG, bool MAIN = false>
+template <class C, int PROG, bool MAIN = false, bool GEN = false>
 __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa, int dir, int tile,
                                           double* smem) {
   constexpr int NPT = C::NPT;
@@ -611,15 +613,15 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       if constexpr (LAST) {
         ChunkCoef<C::R, false> CD;
         lds_chunk_coef<C>(CD, cs, p * C::R);
-        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
+        solve_pass_line<C, false, GEN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
       } else {
-        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CDr, a.fc, a.bnd[dir]);
+        solve_pass_line<C, false, GEN>(Tl, Xl, n, p, CDr, a.fc, a.bnd[dir]);
       }
       __syncthreads();
       const int fld = c < 3 ? c + 3 * dir : 9 + dir;
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
-        if (MAIN && a.met) {
+        if (GEN && a.met) {
           // gradcal's accumulation dvel(m,n) += D_d(u_m) dxi(d,n) over every
           // n (src/comsolver.F90:300-321); the last pass adds its terms with
           // the stress below
@@ -656,7 +658,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         double g[3][3], gt[3];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-          if (MAIN && a.met) {
+          if (GEN && a.met) {
 #pragma clang fp contract(off)
             // the earlier passes' terms plus this direction's, D_dir(u_m) dxi(dir,d)
             if (d < a.ndir) {
@@ -757,7 +759,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
       for (int t = 0; t < NPT; t++) {
         // a periodic halo copy carries the source point's metrics, which are this point's
-        if (a.met) {
+        if (GEN && a.met) {
           const unsigned o = S.g[t];
           const MetRow M = met_row(a, dir, o);
           const double u = ldg(a.pr + fs, o), v = ldg(a.pr + 2 * fs, o), w = ldg(a.pr + 3 * fs, o);
@@ -807,9 +809,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       if constexpr (LAST) {
         ChunkCoef<C::R, false> CD;
         lds_chunk_coef<C>(CD, cs, p * C::R);
-        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
+        solve_pass_line<C, false, GEN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
       } else {
-        solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CDr, a.fc, a.bnd[dir]);
+        solve_pass_line<C, false, GEN>(Tl, Xl, n, p, CDr, a.fc, a.bnd[dir]);
       }
       __syncthreads();
 #pragma unroll
@@ -850,7 +852,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
         for (int t = 0; t < NPT; t++) {
           const unsigned o = S.g[t];
-          const double jb = jac_of(a, o);
+          const double jb = jac_of<GEN>(a, o);
           const double q0 = ldg(a.q, o);
           double s0;
           if (sa.stage == 0) {
@@ -870,7 +872,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 #pragma unroll
         for (int t = 0; t < NPT; t++) {
           if constexpr (MAIN) {
-            if (a.met) {
+            if (GEN && a.met) {
               put<C>(T, S, t, visc_col_full(a, dir, c, S.g[t]));
             } else {
               const Met M = met_of(a, dir);
@@ -901,14 +903,14 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         {
           ChunkCoef<C::R, false> CD;
           lds_chunk_coef<C>(CD, cs, p * C::R);
-          solve_pass_line<C, false, MAIN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
+          solve_pass_line<C, false, GEN>(Tl, Xl, n, p, CD, a.fc, a.bnd[dir]);
         }
         __syncthreads();
 #pragma unroll
         for (int t = 0; t < NPT; t++) {
           const double v = qr[t] + (MAIN ? X[S.s[t]] : X[S.s[t]] * idn);
           if constexpr (RKM) {
-            const double jb = jac_of(a, S.g[t]);
+            const double jb = jac_of<GEN>(a, S.g[t]);
             double s0 = qs[t];
             if (sa.stage == 0) {
               s0 = qo[t] * jb;
@@ -1027,7 +1029,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       {
         ChunkCoef<C::R, true> CF;
         lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
-        solve_pass_line<C, true, MAIN>(Tl, Xl, n, p, CF, fl, a.bnd[dir]);
+        solve_pass_line<C, true, GEN>(Tl, Xl, n, p, CF, fl, a.bnd[dir]);
       }
       __syncthreads();
       double* dst = a.q + m * fs;
@@ -1039,10 +1041,14 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
 }
 
 // one pass of one case: block b handles tile xcd_tile(b)
-template <class C, int PROG, int MINB, bool MAIN = false>
+// GEN (main path only): the general-grid form, for a direction with
+// physical ends or a grid with per-point metrics; the periodic, uniform-grid
+// kernels leave that code out (it would multiply their code size several
+// times over, past what the instruction cache holds)
+template <class C, int PROG, int MINB, bool MAIN = false, bool GEN = false>
 __global__ __launch_bounds__(C::NT, MINB) void k_pass(FlowArgs a, int dir) {
   extern __shared__ double smem[];
-  pass_body<C, PROG, MAIN>(a, stage_of(a), dir, xcd_tile(blockIdx.x, gridDim.x), smem);
+  pass_body<C, PROG, MAIN, GEN>(a, stage_of(a), dir, xcd_tile(blockIdx.x, gridDim.x), smem);
 }
 
 // The tiles of one pass over several cases: case k owns global tiles
@@ -1093,29 +1099,30 @@ hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageA
   return hipGetLastError();
 }
 
-template <class C, int MINB, bool MAIN>
+template <class C, int MINB, bool MAIN, bool GEN>
 void launch_pass_m(int prog, const FlowArgs& a, int dir, hipStream_t st) {
   const int nt = tiles_of(a.n, dir, C::TL);
   const size_t lds = (size_t)lds_doubles<C>(prog) * sizeof(double);
   dim3 g(nt), b(C::NT);
   switch (prog) {
-    case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case GC_MID: hipLaunchKernelGGL((k_pass<C, GC_MID, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case GC_LAST: hipLaunchKernelGGL((k_pass<C, GC_LAST, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB, MAIN>), g, b, lds, st, a, dir); break;
-    case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    case GC_FIRST: hipLaunchKernelGGL((k_pass<C, GC_FIRST, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case GC_MID: hipLaunchKernelGGL((k_pass<C, GC_MID, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case GC_LAST: hipLaunchKernelGGL((k_pass<C, GC_LAST, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case V_MID: hipLaunchKernelGGL((k_pass<C, V_MID, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case V_LAST_RHS: hipLaunchKernelGGL((k_pass<C, V_LAST_RHS, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
     // the filter pass of the main path also takes physical boundaries
-    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB, MAIN>), g, b, lds, st, a, dir); break;
+    default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
   }
 }
 
 template <class C, int MINB>
 hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
-  if (a.main_sem) launch_pass_m<C, MINB, true>(prog, a, dir, st);
-  else launch_pass_m<C, MINB, false>(prog, a, dir, st);
+  if (a.main_sem && (a.met || a.bnd[dir])) launch_pass_m<C, MINB, true, true>(prog, a, dir, st);
+  else if (a.main_sem) launch_pass_m<C, MINB, true, false>(prog, a, dir, st);
+  else launch_pass_m<C, MINB, false, false>(prog, a, dir, st);
   return hipGetLastError();
 }
 
