@@ -573,6 +573,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         }
       }
     }
+    // main path, halo slots: the halo's pressure (hp) and u_dir (hu), kept
+    // from the gradient part for the convective part (set on halo lanes only)
+    double hp[NPT], hu[NPT];
     if constexpr (DO_GRAD) {
     // gradients of u, v, w, T along dir (gradcal, tgvsolver.F90:450-537)
     double gz[4][NPT];
@@ -590,8 +593,10 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
             double f;
             if (c < 3) {
               f = qh[c + 1] / qh[0];
+              if (c == dir) hu[t] = f;
             } else {
               const double pr = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+              hp[t] = pr;
               f = pr / qh[0] * a.const2;
             }
             T[kLineOff + S.h[t]] = f;
@@ -777,8 +782,14 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           if (S.h[t] != kDummy) {
             double qh[5];
             halo_q<C>(a, S, t, dir, qv, qh);
-            const double ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
-            const double uh = pick5(0.0, qh[1], qh[2], qh[3], 0.0, 1 + dir) / qh[0];
+            double ph, uh;
+            if constexpr (DO_GRAD) {
+              ph = hp[t];
+              uh = hu[t];
+            } else {
+              ph = pressure_of(a, qh[0], qh[1], qh[2], qh[3], qh[4]);
+              uh = pick5(0.0, qh[1], qh[2], qh[3], 0.0, 1 + dir) / qh[0];
+            }
             T[kLineOff + S.h[t]] = conv_flux_main(M.j, M.d, MM, dir, qh[MM], qh[4], uh, ph);
           }
         }
