@@ -311,7 +311,8 @@ class Flow:
         shkcrt; src/readwrite.F90:541-546).  An even first digit of conschm
         selects the central compact scheme (643c only); an odd first digit
         with suffix 'e' the explicit upwind-biased scheme (convrsduwd,
-        src/solver.F90:548-1201); the compact-upwind suffix 'c' is refused."""
+        src/solver.F90:548-1201); 543c the compact upwind-biased scheme
+        (convrsdcmp, src/solver.F90:1271-1937; recon_schem unused)."""
         conschm = conschm.strip()
         if len(conschm) != 4 or not conschm[0].isdigit() or conschm[3] not in "ce":
             raise AstrError(f"conschm {conschm!r}: expected three digits and 'c' or 'e'")
@@ -320,10 +321,11 @@ class Flow:
                 raise AstrError(f"central scheme {conschm!r}: only 643c is provided")
             _chk(lib().astr_hip_set_convection(self.h, 0, 0, 0, 0.0, 0.0))
             return
-        if conschm[3] != "e":
-            raise AstrError(f"conschm {conschm!r}: the compact upwind-biased scheme is not provided")
-        _chk(lib().astr_hip_set_convection(self.h, 1, int(recon_schem), int(bool(lchardecomp)),
-                                           float(bfacmpld), float(shkcrt)))
+        if conschm[3] == "c" and conschm != "543c":
+            raise AstrError(f"conschm {conschm!r}: the compact upwind-biased scheme is defined for 543c only "
+                            "(compact_flux_initiate, src/flux.F90:93-110)")
+        _chk(lib().astr_hip_set_convection(self.h, 2 if conschm[3] == "c" else 1, int(recon_schem),
+                                           int(bool(lchardecomp)), float(bfacmpld), float(shkcrt)))
 
     def set_metrics(self, jac, dxi):
         """Use the metrics (as gridgeom returns them): those of a Cartesian

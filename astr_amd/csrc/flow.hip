@@ -715,8 +715,9 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* 
     if (pass(c, prog, d)) return -1;
   }
   if (a.uwd) {
-    // rhscal's upwind branch (solver.F90:219-226): ducrossensor when
-    // recon_schem = 5 or lchardecomp, then convrsduwd along i, j, k
+    // rhscal's upwind branches (solver.F90:219-233): ducrossensor when
+    // recon_schem = 5 or lchardecomp (explicit) or lchardecomp (compact),
+    // then convrsduwd or convrsdcmp along i, j, k
     if (sensor && timed(c, kSlotUwSensor, [&] { return uw_launch_sensor(a, c->uw, c->stream); })) return -1;
     for (int d = 0; d < nd; d++) {
       if (timed(c, kSlotUwSplit + d, [&] { return uw_launch_split(a, c->uw, d, c->stream); })) return -1;
@@ -2002,9 +2003,12 @@ int astr_hip_set_convection(astr_hip_ctx* c, int upwind, int recon_schem, int lc
     a.uwd = 0;
     return 0;
   }
+  if (upwind != 1 && upwind != 2)
+    return fail("upwind must be 0 (central), 1 (explicit upwind-biased) or 2 (compact upwind-biased)");
   if (!a.main_sem) return fail("the upwind-biased convection is a main-path (ASTR_SEM_MAIN) feature");
-  if (!(recon_schem == -1 || recon_schem == 0 || recon_schem == 1 || recon_schem == 2 || recon_schem == 3 ||
-        recon_schem == 5 || recon_schem == 6))
+  const bool compact = upwind == 2;
+  if (!compact && !(recon_schem == -1 || recon_schem == 0 || recon_schem == 1 || recon_schem == 2 ||
+                    recon_schem == 3 || recon_schem == 5 || recon_schem == 6))
     return fail("recon_schem must be -1, 0, 1, 2, 3, 5 or 6 (recons_exp, flux.F90:305-335)");
   for (int d = 0; d < a.ndir; d++)
     if (a.bnd[d]) return fail("the upwind-biased convection runs on periodic directions only");
@@ -2013,7 +2017,35 @@ int astr_hip_set_convection(astr_hip_ctx* c, int upwind, int recon_schem, int lc
   for (int d = 0; d < a.ndir; d++) sws = std::max(sws, uw_points(a.n, d));
   sws = (sws + 63) / 64 * 64;
   const long long lsh_doubles = (a.fs + 7) / 8;
-  const size_t bytes = (size_t)(nf * sws + a.fs + lsh_doubles) * sizeof(double);
+  // compact: the interface-flux buffer and the LU factors of the line
+  // systems (compact_flux_initiate, flux.F90:33-118, scheme 543, ntype 3)
+  const long long nfh = compact ? kUwFields * sws : 0;
+  const long long nac = compact ? (cuw_ac_doubles(a.n, a.ndir) + 63) / 64 * 64 : 0;
+  std::vector<double> ac(nac, 0.0);
+  if (compact) {
+    const double one6 = 1.0 / 6.0;
+    for (int d = 0; d < a.ndir; d++)
+      for (int side = 0; side < 2; side++) {
+        const bool plus = side == 0;
+        const double av = plus ? 0.5 - one6 * bfacmpld : one6 + one6 * bfacmpld;
+        const double cv = plus ? one6 + one6 * bfacmpld : 0.5 - one6 * bfacmpld;
+        double* t = ac.data() + cuw_ac_offset(a.n, d, plus);
+        const int N = a.n[d] + 4;
+        // tridiagonal_thomas_proprocess (commfunc.F90:752-774); a = c = 0 on
+        // the end rows, row 0 holds ac(1,1) = c(1) only
+        t[0] = 0.0;
+        t[1] = 1.0;
+        t[2] = 0.0;
+        for (int r = 1; r < N; r++) {
+          const bool end = r == N - 1;
+          const double aa = end ? 0.0 : av, cc = end ? 0.0 : cv;
+          t[3 * r] = cc / (1.0 - aa * t[3 * (r - 1)]);
+          t[3 * r + 1] = 1.0 / (1.0 - aa * t[3 * (r - 1)]);
+          t[3 * r + 2] = aa / (1.0 - aa * t[3 * (r - 1)]);
+        }
+      }
+  }
+  const size_t bytes = (size_t)(nf * sws + a.fs + lsh_doubles + nfh + nac) * sizeof(double);
   (void)hipStreamSynchronize(c->stream);
   if (c->uwbuf) (void)hipFree(c->uwbuf);
   c->uwbuf = nullptr;
@@ -2036,6 +2068,16 @@ int astr_hip_set_convection(astr_hip_ctx* c, int upwind, int recon_schem, int lc
   u.sws = sws;
   u.ssf = c->uwbuf + nf * sws;
   u.lsh = reinterpret_cast<unsigned char*>(u.ssf + a.fs);
+  u.compact = compact ? 1 : 0;
+  if (compact) {
+    // the explicit scheme's sensor runs for recon_schem 5 or lchardecomp,
+    // the compact one's for lchardecomp only (solver.F90:223, 229)
+    u.sensor = lchardecomp ? 1 : 0;
+    u.fhc = u.ssf + a.fs + lsh_doubles;
+    double* act = u.fhc + nfh;
+    HIP_OK(hipMemcpy(act, ac.data(), nac * sizeof(double), hipMemcpyHostToDevice));
+    u.cuw_ac = act;
+  }
   a.uwd = 1;
   return 0;
 }

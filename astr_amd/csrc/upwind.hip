@@ -653,6 +653,216 @@ __global__ __launch_bounds__(64) void k_lshock(FlowArgs a, UwArgs u) {
   u.lsh[o] = mx > u.shkcrt ? 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// The compact upwind-biased scheme (conschm 543c, convrsdcmp,
+// solver.F90:1271-1937) on periodic lines: the compact interface fluxes of
+// every split component (k_cuw_solve), the MP5-limited interface flux
+// (k_cuw_face) and its difference into qrhs (k_cuw_diff).
+// ---------------------------------------------------------------------------
+
+// split-buffer index of row r (-4..n+4) of line L of direction DIR
+template <int DIR>
+__device__ __forceinline__ long long cuw_at(long long L, int r, int NR, long long NL) {
+  return DIR == 0 ? L * NR + (r + 4) : (long long)(r + 4) * NL + L;
+}
+
+// MP5 (flux.F90:434-495) as mplimiter calls it on a periodic line
+// (flux.F90:363-379): u(1:5) with u(3) the upwind point, ul the compact
+// scheme's interface value
+__device__ __forceinline__ double mp5(const double* u, double ul, bool shock) {
+  const double ump = u[2] + minmod2(u[3] - u[2], 4.0 * (u[2] - u[1]));
+  if (shock && (ul - u[2]) * (ul - ump) >= 1.0e-10) {
+    const double w[6] = {0.0, u[0], u[1], u[2], u[3], u[4]};
+    return mp_limit(w, ul);
+  }
+  return ul;
+}
+
+// flux_compact (flux.F90:125-150, compact_flux_rhs :163-254, the Thomas
+// sweeps of tridiagonal_thomas_solver, commfunc.F90:790-813) of one line and
+// one split component (0..4 F+ with the upwind scheme, 5..9 F- with the
+// downwind one): rows -2..n+1, the end rows the explicit 6th-order interface
+// value, rows -1..n the 5th-order compact one.  The forward sweep leaves d'
+// in the output rows, the backward sweep overwrites them with the solution;
+// rows -1..n are the interface fluxes fh(-1:n).  One lane per (component,
+// line); for y and z the lanes of a wavefront take consecutive lines, so
+// every access coalesces.
+template <int DIR>
+__global__ __launch_bounds__(256) void k_cuw_solve(FlowArgs a, UwArgs u) {
+  const int n = a.n[DIR];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, DIR);
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= 10 * NL) return;
+  const int f = (int)(idx / NL);
+  const long long L = idx - (long long)f * NL;
+  const bool plus = f < 5;
+  const double* ac = u.cuw_ac + cuw_ac_offset(a.n, DIR, plus);
+  const double* src = u.sw + f * u.sws;
+  double* dst = u.fhc + f * u.sws;
+  const double b = u.bfac;
+  const double k118 = 1.0 / 18.0, k136 = 1.0 / 36.0, k1918 = 19.0 / 18.0, k59 = 5.0 / 9.0, k936 = 9.0 / 36.0;
+  const double w0 = k118 - k136 * b, w1 = k1918 - k936 * b, w2 = k59 + k936 * b, w3 = k136 * b;
+  const double c37 = 37.0 / 60.0, c2 = 2.0 / 15.0, c1 = 1.0 / 60.0;
+  auto F = [&](int r) { return src[cuw_at<DIR>(L, r, NR, NL)]; };
+  auto end_row = [&](int j) {
+    const double v1 = F(j) + F(j + 1), v2 = F(j - 1) + F(j + 2), v3 = F(j - 2) + F(j + 3);
+    return c37 * v1 - c2 * v2 + c1 * v3;
+  };
+  const int N = n + 4;
+  double dp = end_row(-2);
+  dst[cuw_at<DIR>(L, -2, NR, NL)] = dp;
+#pragma unroll 1
+  for (int r = 1; r < N; r++) {
+    const int j = r - 2;
+    double d;
+    if (r == N - 1) d = end_row(j);
+    else if (plus) d = w0 * F(j - 1) + w1 * F(j) + w2 * F(j + 1) + w3 * F(j + 2);
+    else d = w0 * F(j + 2) + w1 * F(j + 1) + w2 * F(j) + w3 * F(j - 1);
+    dp = d * ac[3 * r + 1] - dp * ac[3 * r + 2];
+    dst[cuw_at<DIR>(L, j, NR, NL)] = dp;
+  }
+  double x = dp;
+#pragma unroll 1
+  for (int r = N - 2; r >= 0; r--) {
+    const int j = r - 2;
+    const long long c = cuw_at<DIR>(L, j, NR, NL);
+    x = dst[c] - ac[3 * r] * x;
+    dst[c] = x;
+  }
+}
+
+// The interface flux Fh(i), i = -1..n, of convrsdcmp (solver.F90:1366-1490):
+// the windows Flcp(m, n) = fplus(i+n-3), Flcm(m, n) = fminus(i+4-n), n =
+// 1..5 (iwind6) and the compact values, with CD all of them projected on the
+// Roe-state characteristics of rows i, i+1, each limited by MP5 (no crinod:
+// hdiss false), summed, projected back.  Fh(i) replaces the compact value of
+// F+ component m at row i (this lane's own row, read before).
+template <int DIR, bool CD>
+__global__ __launch_bounds__(256) void k_cuw_face(FlowArgs a, UwArgs u) {
+  const int n = a.n[DIR];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, DIR);
+  const long long NI = n + 2;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= NI * NL) return;
+  long long L;
+  int i;
+  if (DIR == 0) {
+    L = idx / NI;
+    i = (int)(idx - L * NI) - 1;
+  } else {
+    i = (int)(idx / NL) - 1;
+    L = idx - (long long)(i + 1) * NL;
+  }
+  int c3[3];
+  const long long o0 = line_base(a, DIR, L, c3);
+  const long long st = dir_stride(a, DIR);
+  const bool lsh = face_shock(u, o0, st, i, n);
+  const double* sw = u.sw;
+  const long long S = u.sws;
+  const long long ci = cuw_at<DIR>(L, i, NR, NL);
+  auto at = [&](int r) { return cuw_at<DIR>(L, r, NR, NL); };
+  double hp[5], hm[5], Fh[5];
+#pragma unroll
+  for (int m = 0; m < 5; m++) {
+    hp[m] = u.fhc[m * S + ci];
+    hm[m] = u.fhc[(5 + m) * S + ci];
+  }
+  if constexpr (CD) {
+    double sl[9], sr[9], lev[25], rev[25];
+    const long long cr = at(i + 1);
+#pragma unroll
+    for (int m = 0; m < 9; m++) {
+      sl[m] = sw[(kUwFields + m) * S + ci];
+      sr[m] = sw[(kUwFields + m) * S + cr];
+    }
+    chardecomp(u.gamma, sl, sr, rev, lev);
+    double Fhc[5];
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      double wp[5], wm[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        const long long ip = at(i + k - 2), im = at(i + 3 - k);
+        wp[k] = lev[5 * m] * sw[ip] + lev[5 * m + 1] * sw[S + ip] + lev[5 * m + 2] * sw[2 * S + ip] +
+                lev[5 * m + 3] * sw[3 * S + ip] + lev[5 * m + 4] * sw[4 * S + ip];
+        wm[k] = lev[5 * m] * sw[5 * S + im] + lev[5 * m + 1] * sw[6 * S + im] + lev[5 * m + 2] * sw[7 * S + im] +
+                lev[5 * m + 3] * sw[8 * S + im] + lev[5 * m + 4] * sw[9 * S + im];
+      }
+      const double lp = lev[5 * m] * hp[0] + lev[5 * m + 1] * hp[1] + lev[5 * m + 2] * hp[2] +
+                        lev[5 * m + 3] * hp[3] + lev[5 * m + 4] * hp[4];
+      const double lm = lev[5 * m] * hm[0] + lev[5 * m + 1] * hm[1] + lev[5 * m + 2] * hm[2] +
+                        lev[5 * m + 3] * hm[3] + lev[5 * m + 4] * hm[4];
+      const double v1 = mp5(wp, lp, lsh);
+      const double v2 = mp5(wm, lm, lsh);
+      Fhc[m] = v1 + v2;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; m++)
+      Fh[m] = rev[5 * m] * Fhc[0] + rev[5 * m + 1] * Fhc[1] + rev[5 * m + 2] * Fhc[2] + rev[5 * m + 3] * Fhc[3] +
+              rev[5 * m + 4] * Fhc[4];
+  } else {
+#pragma unroll
+    for (int m = 0; m < 5; m++) {
+      double wp[5], wm[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        wp[k] = sw[m * S + at(i + k - 2)];
+        wm[k] = sw[(5 + m) * S + at(i + 3 - k)];
+      }
+      const double v1 = mp5(wp, hp[m], lsh);
+      const double v2 = mp5(wm, hm[m], lsh);
+      Fh[m] = v1 + v2;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < 5; m++) u.fhc[m * S + ci] = Fh[m];
+}
+
+// qrhs(t) = qrhs(t) + Fh(t) - Fh(t-1), t = 0..n (solver.F90:1492-1497)
+template <int DIR>
+__global__ __launch_bounds__(256) void k_cuw_diff(FlowArgs a, UwArgs u, int first, int last) {
+  const int n = a.n[DIR];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, DIR);
+  const long long NP = n + 1;
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= NP * NL) return;
+  long long L;
+  int t;
+  if (DIR == 0) {
+    L = idx / NP;
+    t = (int)(idx - L * NP);
+  } else {
+    t = (int)(idx / NL);
+    L = idx - (long long)t * NL;
+  }
+  int c3[3];
+  const long long o0 = line_base(a, DIR, L, c3);
+  const long long o = o0 + (long long)t * dir_stride(a, DIR);
+  const long long c = cuw_at<DIR>(L, t, NR, NL), cp = cuw_at<DIR>(L, t - 1, NR, NL);
+#pragma unroll
+  for (int m = 0; m < 5; m++)
+    face_update1(a, m, o, u.fhc[m * u.sws + c], u.fhc[m * u.sws + cp], first, last);
+}
+
+template <int DIR>
+hipError_t cuw_launch_dir(const FlowArgs& a, const UwArgs& u, bool first, bool last, hipStream_t st) {
+  const long long NL = uw_lines(a.n, DIR);
+  hipLaunchKernelGGL(k_cuw_solve<DIR>, dim3((unsigned)((10 * NL + 255) / 256)), dim3(256), 0, st, a, u);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 gf((unsigned)(((a.n[DIR] + 2) * NL + 255) / 256));
+  if (u.chardec) hipLaunchKernelGGL((k_cuw_face<DIR, true>), gf, dim3(256), 0, st, a, u);
+  else hipLaunchKernelGGL((k_cuw_face<DIR, false>), gf, dim3(256), 0, st, a, u);
+  e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  const dim3 gd((unsigned)(((a.n[DIR] + 1) * NL + 255) / 256));
+  hipLaunchKernelGGL(k_cuw_diff<DIR>, gd, dim3(256), 0, st, a, u, (int)first, (int)last);
+  return hipGetLastError();
+}
+
 }  // namespace
 
 hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st) {
@@ -695,6 +905,11 @@ hipError_t launch_face_s(const FlowArgs& a, const UwArgs& u, int dir, bool first
 }
 
 hipError_t uw_launch_face(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last, hipStream_t st) {
+  if (u.compact) {
+    if (dir == 0) return cuw_launch_dir<0>(a, u, first, last, st);
+    if (dir == 1) return cuw_launch_dir<1>(a, u, first, last, st);
+    return cuw_launch_dir<2>(a, u, first, last, st);
+  }
   switch (u.recon) {
     case -1: return launch_face_s<-1>(a, u, dir, first, last, st);
     case 0: return launch_face_s<0>(a, u, dir, first, last, st);

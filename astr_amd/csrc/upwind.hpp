@@ -23,7 +23,24 @@ struct UwArgs {
   long long sws;
   double* ssf;          // Ducros sensor, field layout of FlowArgs (computed points)
   unsigned char* lsh;   // shock flags, field layout of FlowArgs
+  // the compact upwind-biased scheme (conschm 543c): the LU factors of its
+  // line systems (cuw_ac_offset) and the interface-flux buffer, 10 fields of
+  // sws doubles in the split buffer's layout
+  int compact;
+  const double* cuw_ac;
+  double* fhc;
 };
+
+// LU factors ac(1:3, -2:n+1) of the compact flux systems: for each direction
+// the upwind ('+') then the downwind ('-') system, 3 (n+4) doubles each
+__host__ __device__ inline long long cuw_ac_offset(const int n[3], int dir, bool plus) {
+  long long o = 0;
+  for (int d = 0; d < dir; d++) o += 6ll * (n[d] + 4);
+  return o + (plus ? 0 : 3ll * (n[dir] + 4));
+}
+__host__ __device__ inline long long cuw_ac_doubles(const int n[3], int ndir) {
+  return cuw_ac_offset(n, ndir, true);
+}
 
 // fields of the split buffer: F+ (5), F- (5), then with the characteristic
 // decomposition rho, u, v, w, p, E and the metric row dxi(dir, 0..2)
@@ -47,7 +64,8 @@ hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st);
 hipError_t uw_launch_split(const FlowArgs& a, const UwArgs& u, int dir, hipStream_t st);
 // the interface fluxes of direction dir from u.sw, into qrhs: the first
 // direction stores, later ones accumulate, the last one also negates
-// (rhscal's qrhs = -qrhs)
+// (rhscal's qrhs = -qrhs); explicit (recons_exp) or, with u.compact, the
+// compact scheme's line solves, limited interface fluxes and difference
 hipError_t uw_launch_face(const FlowArgs& a, const UwArgs& u, int dir, bool first, bool last,
                           hipStream_t st);
 

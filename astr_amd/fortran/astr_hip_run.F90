@@ -2,9 +2,10 @@
 ! astr_hip_run.F90 -- the main solver's run ("astr run", src/astr.F90:39-93)
 ! on the device, for the cases this build's main-path semantics cover: a
 ! periodic box (flowtype tgv, 3-D or 2-D, and 2dvort), the convective
-! scheme 643c or an explicit upwind-biased one (odd first digit, suffix 'e',
-! with recon_schem, lchardecomp, bfacmpld, shkcrt), diffusion 643c, rk3, the
-! compact filter, no species, no turbulence model.
+! scheme 643c, an explicit upwind-biased one (odd first digit, suffix 'e',
+! with recon_schem, lchardecomp, bfacmpld, shkcrt) or the compact
+! upwind-biased 543c (with lchardecomp, bfacmpld, shkcrt), diffusion 643c,
+! rk3, the compact filter, no species, no turbulence model.
 !
 ! module astr_hip_main   the reference's module state (commvar, commarray)
 !                        and subroutines under their own names: readinput,
@@ -37,7 +38,7 @@ module astr_hip_main
   real(8) :: roinf, uinf, vinf, winf, tinf, pinf
   character(len=512) :: contfile = 'datin/controller'
   ! the convective scheme (readwrite.F90:541-546)
-  logical :: lupwind = .false., lchardecomp = .false.
+  logical :: lupwind = .false., lcompact = .false., lchardecomp = .false.
   integer :: recon_schem = 0
   real(8) :: bfacmpld = 0.3d0, shkcrt = 0.05d0
 
@@ -110,15 +111,21 @@ contains
     if (.not. lfilter) call input_stop('lfilter = f')
     if (lreadgrid .or. lfftk .or. limmbou) call input_stop('lreadgrid, lfftz or limmbou = t')
     ! rhscal's choice (solver.F90:210-236): an even first digit is the
-    ! central scheme, an odd one the upwind-biased scheme of suffix 'e'
+    ! central scheme, an odd one the upwind-biased scheme, explicit for
+    ! suffix 'e', compact for suffix 'c' (scheme 543 only,
+    ! compact_flux_initiate, flux.F90:93-110)
     read(conschm(1:1), '(I1)', iostat=ios) nconv
     if (ios /= 0 .or. len_trim(conschm) /= 4) call input_stop('convective scheme '//trim(conschm))
     if (mod(nconv, 2) == 0) then
       if (trim(conschm) /= '643c') call input_stop('central scheme '//trim(conschm))
       lupwind = .false.
-    else
-      if (conschm(4:4) /= 'e') call input_stop('compact upwind-biased scheme '//trim(conschm))
+    else if (conschm(4:4) == 'e') then
       lupwind = .true.
+    else if (trim(conschm) == '543c') then
+      lupwind = .true.
+      lcompact = .true.
+    else
+      call input_stop('upwind-biased scheme '//trim(conschm))
     end if
     if (trim(dschm) /= '643c') call input_stop('diffusion scheme '//trim(dschm))
     if (trim(rkscheme) /= 'rk3') call input_stop('time scheme '//trim(rkscheme))
@@ -226,10 +233,15 @@ contains
     cfg%deltat = deltat
     call hipcheck(astr_hip_create(cfg, ctx), 'solvrinit')
     if (lupwind) then
-      call hipcheck(astr_hip_set_convection(ctx, 1, recon_schem, merge(1, 0, lchardecomp), bfacmpld, &
-                                            shkcrt), 'solvrinit: convective scheme')
-      write(*, '(A,I0,A,L1)') '  ** upwind-biased convection, recon_schem ', recon_schem, &
-        ', lchardecomp ', lchardecomp
+      call hipcheck(astr_hip_set_convection(ctx, merge(2, 1, lcompact), recon_schem, &
+                                            merge(1, 0, lchardecomp), bfacmpld, shkcrt), &
+                    'solvrinit: convective scheme')
+      if (lcompact) then
+        write(*, '(A,L1)') '  ** compact upwind-biased convection 543c, lchardecomp ', lchardecomp
+      else
+        write(*, '(A,I0,A,L1)') '  ** upwind-biased convection, recon_schem ', recon_schem, &
+          ', lchardecomp ', lchardecomp
+      end if
     end if
     print *, ' ** solver initialised on the device'
   end subroutine solvrinit

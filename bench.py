@@ -396,21 +396,25 @@ def main_path_step(n, dt, steps, warmup):
     return out
 
 
-def upwind_path_step(n, dt, steps, warmup, recon, chardec):
-    """The main solver's stage with the explicit upwind-biased convection
-    (conschm 753e, recon_schem `recon`, lchardecomp `chardec`, bfacmpld 0.3,
-    shkcrt 0.05; convrsduwd, src/solver.F90:548-1201) on the same TGV
+def upwind_path_step(n, dt, steps, warmup, recon, chardec, conschm="753e"):
+    """The main solver's stage with the upwind-biased convection, explicit
+    (conschm 753e, recon_schem `recon`; convrsduwd, src/solver.F90:548-1201)
+    or compact (conschm 543c; convrsdcmp, src/solver.F90:1271-1937), with
+    lchardecomp `chardec`, bfacmpld 0.3, shkcrt 0.05, on the same TGV
     workload and field: ms per step, Mpt-updates/s and per-kernel times of
-    the timed steps; for the interface-flux kernel, its HBM rate from the
-    algorithmic bytes per point of one direction (the split buffer's ten
-    fields read once, qrhs read and written: 10 + 5 + 5 doubles, the first
-    direction 15)."""
+    the timed steps; for the explicit interface-flux kernel, its HBM rate
+    from the algorithmic bytes per point of one direction (the split
+    buffer's ten fields read once, qrhs read and written: 10 + 5 + 5
+    doubles, the first direction 15).  The compact scheme's uw_face slot
+    holds its three kernels (line solves, limited interface fluxes, flux
+    difference)."""
     import astr_amd as A
 
+    compact = conschm.endswith("c")
     f = A.Flow(A.main_config(n, deltat=dt))
     f.init_tgv(1e-3, 2024)
     f.autotune(reps=3)
-    f.set_conschm("753e", recon, chardec, 0.3, 0.05)
+    f.set_conschm(conschm, recon, chardec, 0.3, 0.05)
     if warmup:
         f.rk3(warmup, stats=True)
     f.sync()
@@ -426,8 +430,9 @@ def upwind_path_step(n, dt, steps, warmup, recon, chardec):
     # algorithmic bytes of one interface-flux launch along y (10 split
     # fields in, qrhs in and out) over its measured time
     face_bytes = (10 + 10) * 8 * pts
-    face_ms = kern.get("uw_face_y")
-    out = {"workload": f"TGV {n}^3 RK3 step, main stage order, conschm 753e, recon_schem {recon}, "
+    face_ms = None if compact else kern.get("uw_face_y")
+    scheme = "conschm 543c" if compact else f"conschm 753e, recon_schem {recon}"
+    out = {"workload": f"TGV {n}^3 RK3 step, main stage order, {scheme}, "
                        f"lchardecomp {'t' if chardec else 'f'}",
            "ms_per_step": round(wall / steps * 1e3, 4),
            "value": round(pts * 3 * steps / wall / 1e6, 3), "unit": "Mpt-updates/s",
@@ -510,7 +515,9 @@ def main():
     dt = 5e-4 * 256.0 / n
     if args.upwind_only:
         print(json.dumps({"upwind_path": [upwind_path_step(n, dt, args.steps, args.warmup, 3, False),
-                                          upwind_path_step(n, dt, args.steps, args.warmup, 3, True)]}),
+                                          upwind_path_step(n, dt, args.steps, args.warmup, 3, True),
+                                          upwind_path_step(n, dt, args.steps, args.warmup, 0, False, "543c"),
+                                          upwind_path_step(n, dt, args.steps, args.warmup, 0, True, "543c")]}),
               flush=True)
         return
     f = A.Flow(A.tgv_config(n, deltat=dt))
@@ -596,7 +603,9 @@ def main():
         # MP7 as the reference's shock examples use it, without and with the
         # characteristic decomposition (which brings the sensor in)
         out["upwind_path"] = [upwind_path_step(n, dt, args.steps, args.warmup, 3, False),
-                              upwind_path_step(n, dt, args.steps, args.warmup, 3, True)]
+                              upwind_path_step(n, dt, args.steps, args.warmup, 3, True),
+                              upwind_path_step(n, dt, args.steps, args.warmup, 0, False, "543c"),
+                              upwind_path_step(n, dt, args.steps, args.warmup, 0, True, "543c")]
     if not args.no_chem:
         out["chem_batch"] = chem_batch()
     if not args.no_batch:

@@ -208,9 +208,15 @@ int astr_hip_set_boundary(astr_hip_ctx *ctx, const int *npdc);
  * when recon_schem = 5 or lchardecomp (ducrossensor,
  * src/commcal.F90:196-360) -- the input line "recon_schem, lchardecomp,
  * bfacmpld, shkcrt" of the reference's input file (readwrite.F90:546).
+ * upwind = 2: the compact upwind-biased scheme (conschm 543c; convrsdcmp,
+ * src/solver.F90:1271-1937): the same splitting, the interface fluxes of the
+ * 5th-order compact upwind scheme with bandwidth factor bfacmpld
+ * (compact_flux_initiate / flux_compact, src/flux.F90:33-254), limited by
+ * MP5 on the 5-point windows (mplimiter, :363-495), characteristic
+ * decomposition at every interface when lchardecomp, and the Ducros sensor
+ * when lchardecomp (recon_schem is not used).
  * Every direction must be periodic (npdc 3); the scheme is kept until
- * changed.  The compact-upwind scheme (suffix 'c', convrsdcmp) is not
- * provided. */
+ * changed. */
 int astr_hip_set_convection(astr_hip_ctx *ctx, int upwind, int recon_schem, int lchardecomp,
                             double bfacmpld, double shkcrt);
 

@@ -111,6 +111,9 @@ void or_sw_split(double gamma, double mach, double jac, const double *dxi, const
                  const double *pv, double *fp, double *fm);
 double or_recons(int scheme, const double *f, double bfac, int shock);
 int or_chardecomp(double gamma, const double *l, const double *r, double *rev, double *lev);
+double or_mp5(const double *u, double ul, int discont);
+void or_cuw_lu(int n, double bfac, int plus, double *ac);
+void or_cuw_flux(int n, double bfac, int plus, const double *ac, const double *f, double *fh);
 
 /* counter-based noise shared with the device path (not a reference routine) */
 double or_gauss(uint64_t seed, uint32_t i, uint32_t j, uint32_t k, uint32_t c);

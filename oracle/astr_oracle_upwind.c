@@ -14,6 +14,13 @@
  *   or_chardecomp    chardecomp, src/solver.F90:1958-2162 (no species): the
  *                    left and right eigenvectors of the Roe-averaged state
  *                    between two points
+ * and of the compact upwind-biased path (suffix 'c': convrsdcmp,
+ * src/solver.F90:1271-1937), scheme 543 on periodic lines:
+ *   or_cuw_lu        compact_flux_initiate, src/flux.F90:33-118: the
+ *                    tridiagonal LHS a, c and its LU factors
+ *   or_cuw_flux      flux_compact + compact_flux_rhs, src/flux.F90:125-254:
+ *                    the linear 5th-order compact interface flux of a line
+ *   or_mp5           MP5, src/flux.F90:434-495, as mplimiter calls it
  * The numerical constants are the reference's own decimal literals (its
  * 16-digit roundings of 13/12, 1/36, ...), so that the restatement and the
  * HIP path round as the reference does.  Expressions keep the reference's
@@ -31,9 +38,15 @@
  * against their design order on a smooth periodic function and their
  * non-oscillation on a step; MP7-LD with its bandwidth factor 0 against the
  * centred 8-point interpolant; ROUND against its normalised-variable limits;
- * chardecomp against LEV REV = I and the Roe-state eigen-relations.
+ * chardecomp against LEV REV = I and the Roe-state eigen-relations.  The
+ * compact flux against a dense solve of its own system written from the
+ * scheme's definition, polynomial exactness of the interface value, its
+ * convergence order on a smooth periodic function and the Fourier symbol
+ * of the interior rows; MP5 against the linear value on smooth monotone
+ * data and against the monotonicity bounds on a step.
  */
 #include <math.h>
+#include <stdlib.h>
 
 #include "astr_oracle.h"
 
@@ -238,6 +251,76 @@ double or_recons(int scheme, const double *f, double bfac, int shock) {
     case 6: return round3(f + 2);
     default: return NAN;
   }
+}
+
+/* MP5 (flux.F90:434-495) as mplimiter calls it for a periodic direction
+ * (flux.F90:363-379): u = u(1:5) with u(3) the upwind point, ul the compact
+ * scheme's linear interface value, discont the shock flag */
+double or_mp5(const double *u, double ul, int discont) {
+  const double ump = u[2] + minmod2(u[3] - u[2], 4.0 * (u[2] - u[1]));
+  if (discont && (ul - u[2]) * (ul - ump) >= 1.0e-10) {
+    const double w[6] = {0.0, u[0], u[1], u[2], u[3], u[4]};  /* w(1:5) = u(1:5) */
+    return mp_limit(w, ul);
+  }
+  return ul;
+}
+
+/* compact_flux_initiate (flux.F90:33-118) for scheme 543 on a periodic line
+ * (ntype 3): rows i_0 = -2 .. i_m = n+1, a(:) = 0.5 - bfac/6 and
+ * c(:) = 1/6 + bfac/6 for the upwind ('+') flux, the two swapped for '-',
+ * a = c = 0 on both end rows; ac[3 r + 0..2] = ac(1..3, r+1), r = 0..n+3,
+ * by tridiagonal_thomas_proprocess (commfunc.F90:752-774).  Row 0 holds
+ * ac(1,1) = c(1) only (the solver never reads its ac(2:3, 1)). */
+void or_cuw_lu(int n, double bfac, int plus, double *ac) {
+  const double one6 = 1.0 / 6.0;
+  const double av = plus ? 0.5 - one6 * bfac : one6 + one6 * bfac;
+  const double cv = plus ? one6 + one6 * bfac : 0.5 - one6 * bfac;
+  const int N = n + 4;
+  for (int r = 0; r < N; r++) {
+    const int end = r == 0 || r == N - 1;
+    const double a = end ? 0.0 : av, c = end ? 0.0 : cv;
+    if (r == 0) {
+      ac[0] = c;
+      ac[1] = 1.0;
+      ac[2] = 0.0;
+      continue;
+    }
+    ac[3 * r] = c / (1.0 - a * ac[3 * (r - 1)]);
+    ac[3 * r + 1] = 1.0 / (1.0 - a * ac[3 * (r - 1)]);
+    ac[3 * r + 2] = a / (1.0 - a * ac[3 * (r - 1)]);
+  }
+}
+
+/* flux_compact (flux.F90:125-150) with compact_flux_rhs (flux.F90:163-254),
+ * periodic line: f = f(-4:n+4) at f[t + 4], fh = fh(-1:n) at fh[i + 1].
+ * End rows -2 and n+1: the explicit 6th-order interface value; rows -1..n:
+ * the 5th-order compact upwind right-hand side; then the Thomas sweeps of
+ * tridiagonal_thomas_solver (commfunc.F90:790-813). */
+void or_cuw_flux(int n, double bfac, int plus, const double *ac, const double *f, double *fh) {
+  const int N = n + 4;
+  double *d = malloc(sizeof(double) * 2 * N), *x = d + N;
+  const double c37 = 37.0 / 60.0, c2 = 2.0 / 15.0, c1 = 1.0 / 60.0;
+  const double k118 = 1.0 / 18.0, k136 = 1.0 / 36.0, k1918 = 19.0 / 18.0, k59 = 5.0 / 9.0, k936 = 9.0 / 36.0;
+#define F(j) f[(j) + 4]
+  for (int r = 0; r < N; r++) {
+    const int j = r - 2;
+    if (r == 0 || r == N - 1) {
+      const double v1 = F(j) + F(j + 1), v2 = F(j - 1) + F(j + 2), v3 = F(j - 2) + F(j + 3);
+      d[r] = c37 * v1 - c2 * v2 + c1 * v3;
+    } else if (plus) {
+      d[r] = (k118 - k136 * bfac) * F(j - 1) + (k1918 - k936 * bfac) * F(j) + (k59 + k936 * bfac) * F(j + 1) +
+             k136 * bfac * F(j + 2);
+    } else {
+      d[r] = (k118 - k136 * bfac) * F(j + 2) + (k1918 - k936 * bfac) * F(j + 1) + (k59 + k936 * bfac) * F(j) +
+             k136 * bfac * F(j - 1);
+    }
+  }
+#undef F
+  for (int r = 1; r < N; r++) d[r] = d[r] * ac[3 * r + 1] - d[r - 1] * ac[3 * r + 2];
+  x[N - 1] = d[N - 1];
+  for (int r = N - 2; r >= 0; r--) x[r] = d[r] - ac[3 * r] * x[r + 1];
+  for (int i = -1; i <= n; i++) fh[i + 1] = x[i + 2];
+  free(d);
 }
 
 /* chardecomp (solver.F90:1958-2162, no species): l, r = rho, u, v, w, p,

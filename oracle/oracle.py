@@ -107,6 +107,10 @@ def lib():
         L.or_recons.argtypes = [C.c_int, _dp, C.c_double, C.c_int]
         L.or_chardecomp.restype = C.c_int
         L.or_chardecomp.argtypes = [C.c_double, _dp, _dp, _dp, _dp]
+        L.or_mp5.restype = C.c_double
+        L.or_mp5.argtypes = [_dp, C.c_double, C.c_int]
+        L.or_cuw_lu.argtypes = [C.c_int, C.c_double, C.c_int, _dp]
+        L.or_cuw_flux.argtypes = [C.c_int, C.c_double, C.c_int, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -324,9 +328,11 @@ class MainState:
         return ke, en, di
 
     def set_conv(self, upwind, recon=0, chardecomp=False, bfacmpld=0.3, shkcrt=0.05):
-        """Convective scheme: central (upwind False) or the explicit
-        upwind-biased one with recon_schem, lchardecomp, bfacmpld, shkcrt."""
-        rc = lib().or_main_set_conv(self.h, int(bool(upwind)), int(recon), int(bool(chardecomp)),
+        """Convective scheme: central (upwind 0 / False), the explicit
+        upwind-biased one (1 / True) with recon_schem, lchardecomp, bfacmpld,
+        shkcrt, or the compact upwind-biased 543c (2) with lchardecomp,
+        bfacmpld, shkcrt."""
+        rc = lib().or_main_set_conv(self.h, int(upwind), int(recon), int(bool(chardecomp)),
                                     float(bfacmpld), float(shkcrt))
         if rc:
             raise ValueError("convection scheme not covered by the restatement")
@@ -369,6 +375,32 @@ def chardecomp(gamma, left, right):
     if lib().or_chardecomp(gamma, _p(l), _p(r), _p(rev), _p(lev)):
         raise ValueError("zero metric row")
     return rev.reshape(5, 5), lev.reshape(5, 5)
+
+
+def mp5(u, ul, shock=True):
+    """MP5 as mplimiter applies it on a periodic line: the window u(1:5)
+    (u(3) upwind) limited against the linear interface value ul."""
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    assert u.shape == (5,)
+    return lib().or_mp5(_p(u), float(ul), int(bool(shock)))
+
+
+def cuw_lu(n, bfac, plus):
+    """compact_flux_initiate, scheme 543, periodic line of n+1 points: the
+    LU factors ac(1:3, -2:n+1) as an (n+4, 3) array."""
+    ac = np.zeros(3 * (n + 4))
+    lib().or_cuw_lu(int(n), float(bfac), int(bool(plus)), _p(ac))
+    return ac.reshape(n + 4, 3)
+
+
+def cuw_flux(n, bfac, plus, f):
+    """flux_compact on a periodic line: f = f(-4:n+4) -> fh(-1:n)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    assert f.shape == (n + 9,)
+    ac = np.ascontiguousarray(cuw_lu(n, bfac, plus)).ravel()
+    fh = np.zeros(n + 2)
+    lib().or_cuw_flux(int(n), float(bfac), int(bool(plus)), _p(ac), _p(f), _p(fh))
+    return fh
 
 
 def qswap(q):

@@ -1,10 +1,11 @@
 """Parity of the HIP path's upwind-biased convection (main path, through the
 C ABI: astr_hip_set_convection) against the restatement of convrsduwd,
-ducrossensor and their pointwise pieces in oracle/astr_oracle_main.c and
-oracle/astr_oracle_upwind.c.
+convrsdcmp, ducrossensor and their pointwise pieces in
+oracle/astr_oracle_main.c and oracle/astr_oracle_upwind.c.
 
-Every reconstruction scheme (recon_schem -1, 0, 1, 2, 3, 5, 6), with and
-without the characteristic decomposition, in 3-D and 2-D, on fields whose
+Every reconstruction scheme of the explicit scheme (recon_schem -1, 0, 1,
+2, 3, 5, 6) and the compact scheme 543c (bandwidth factors 0.3 and 1), with
+and without the characteristic decomposition, in 3-D and 2-D, on fields whose
 duplicate planes differ (so qswap's averages, refreshed primitives and kept
 halo sources all take part) and that carry a compression bump (so the shock
 sensor flags some interfaces and not others); a full metric tensor; RK3
@@ -48,7 +49,7 @@ def bumped_field(cfg, seed, skew=1e-3):
     return q * (1.0 + skew * rng.standard_normal(q.shape))
 
 
-def make_pair(dims, scheme, chardec, shkcrt=0.02, bfac=0.3, dt=1e-3, metrics=None, seed=7):
+def make_pair(dims, scheme, chardec, shkcrt=0.02, bfac=0.3, dt=1e-3, metrics=None, seed=7, conschm="753e"):
     ocfg = O.tgv_cfg(*dims, dt=dt, suth_dim=110.3)
     q = bumped_field(ocfg, seed)
     shape = q.shape[1:]
@@ -67,11 +68,11 @@ def make_pair(dims, scheme, chardec, shkcrt=0.02, bfac=0.3, dt=1e-3, metrics=Non
             for b in range(3):
                 od[a + 3 * b] = dev_dxi[a, b].ravel()
     s = O.MainState(ocfg, jac, od)
-    s.set_conv(True, scheme, chardec, bfac, shkcrt)
+    s.set_conv(2 if conschm.endswith("c") else 1, scheme, chardec, bfac, shkcrt)
     s.set_q(q)
     f = A.Flow(A.main_config(*dims, deltat=dt))
     f.set_metrics(dev_jac, dev_dxi)
-    f.set_conschm("753e", scheme, chardec, bfac, shkcrt)
+    f.set_conschm(conschm, scheme, chardec, bfac, shkcrt)
     f.upload(q)
     return f, s
 
@@ -179,11 +180,58 @@ def test_upwind_launch_selection_and_central_switch_back():
     f.close()
 
 
+# the compact upwind-biased scheme (conschm 543c, convrsdcmp)
+
+@pytest.mark.parametrize("dims", [(24, 20, 16), (26, 20, 0)])
+@pytest.mark.parametrize("chardec", [False, True])
+@pytest.mark.parametrize("bfac", [0.3, 1.0])
+def test_compact_upwind_rhs_matches_oracle(dims, chardec, bfac):
+    f, s = make_pair(dims, 0, chardec, bfac=bfac, conschm="543c")
+    f.qswap()
+    s.qswap()
+    got = f.rhscal()
+    s.gradcal()
+    s.rhscal()
+    if chardec:
+        flags = s.lshock
+        assert 0 < flags.sum() < flags.size   # the sensor splits the box
+    for m in range(5):
+        if m == 3 and dims[2] == 0:
+            continue
+        assert relmax(got[m], s.qrhs[m]) < TOL, (m, relmax(got[m], s.qrhs[m]))
+    f.close()
+
+
+@pytest.mark.parametrize("chardec", [False, True])
+def test_compact_upwind_rk3_steps_match_oracle(chardec):
+    f, s = make_pair((24, 20, 16), 0, chardec, dt=2e-3, conschm="543c")
+    ke, en, di = f.rk3(2, dissipation=True)
+    ke0, en0, di0 = s.rk3(2)
+    qd, prd = f.download(prims=True)
+    qo, pro = s.get()
+    assert max(state_err(qd, qo)) < TOL
+    assert relmax(prd, pro) < TOL
+    assert relmax(ke, ke0) < 1e-9 and relmax(en, en0) < 1e-9 and relmax(di, di0) < 1e-9
+    f.close()
+
+
+def test_compact_upwind_full_metric_tensor_matches_oracle():
+    f, s = make_pair((20, 18, 16), 0, True, metrics=_affine_metrics, conschm="543c")
+    f.qswap()
+    s.qswap()
+    got = f.rhscal()
+    s.gradcal()
+    s.rhscal()
+    for m in range(5):
+        assert relmax(got[m], s.qrhs[m]) < TOL, (m, relmax(got[m], s.qrhs[m]))
+    f.close()
+
+
 def test_upwind_refusals():
     f = A.Flow(A.main_config(16, 16, 16, deltat=1e-3))
     with pytest.raises(A.AstrError, match="recon_schem"):
         f.set_conschm("753e", 4)
-    with pytest.raises(A.AstrError, match="compact upwind"):
+    with pytest.raises(A.AstrError, match="543c only"):
         f.set_conschm("753c", 0)
     with pytest.raises(A.AstrError, match="643c"):
         f.set_conschm("642c")
