@@ -363,10 +363,12 @@ __global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
   if (idx >= NL * NR) return;
   long long L;
   int rr;
-  if (dir == 0) {
+  if (dir == 0 && !u.compact) {
+    // explicit scheme: x-lines row-fastest, for k_uw_face_x's lanes
     L = idx / NR;
     rr = (int)(idx - L * NR);
   } else {
+    // y and z, and every direction of the compact scheme: lines fastest
     rr = (int)(idx / NL);
     L = idx - (long long)rr * NL;
   }
@@ -660,10 +662,12 @@ __global__ __launch_bounds__(64) void k_lshock(FlowArgs a, UwArgs u) {
 // (k_cuw_face) and its difference into qrhs (k_cuw_diff).
 // ---------------------------------------------------------------------------
 
-// split-buffer index of row r (-4..n+4) of line L of direction DIR
+// split-buffer index of row r (-4..n+4) of line L: the compact scheme keeps
+// every direction's lines fastest, so that one lane per line coalesces
 template <int DIR>
 __device__ __forceinline__ long long cuw_at(long long L, int r, int NR, long long NL) {
-  return DIR == 0 ? L * NR + (r + 4) : (long long)(r + 4) * NL + L;
+  (void)NR;
+  return (long long)(r + 4) * NL + L;
 }
 
 // MP5 (flux.F90:434-495) as mplimiter calls it on a periodic line
@@ -697,9 +701,9 @@ __global__ __launch_bounds__(256) void k_cuw_solve(FlowArgs a, UwArgs u) {
   const int f = (int)(idx / NL);
   const long long L = idx - (long long)f * NL;
   const bool plus = f < 5;
-  const double* ac = u.cuw_ac + cuw_ac_offset(a.n, DIR, plus);
-  const double* src = u.sw + f * u.sws;
-  double* dst = u.fhc + f * u.sws;
+  const double* __restrict__ ac = u.cuw_ac + cuw_ac_offset(a.n, DIR, plus);
+  const double* __restrict__ src = u.sw + f * u.sws;
+  double* __restrict__ dst = u.fhc + f * u.sws;
   const double b = u.bfac;
   const double k118 = 1.0 / 18.0, k136 = 1.0 / 36.0, k1918 = 19.0 / 18.0, k59 = 5.0 / 9.0, k936 = 9.0 / 36.0;
   const double w0 = k118 - k136 * b, w1 = k1918 - k936 * b, w2 = k59 + k936 * b, w3 = k136 * b;
@@ -712,18 +716,24 @@ __global__ __launch_bounds__(256) void k_cuw_solve(FlowArgs a, UwArgs u) {
   const int N = n + 4;
   double dp = end_row(-2);
   dst[cuw_at<DIR>(L, -2, NR, NL)] = dp;
-#pragma unroll 1
-  for (int r = 1; r < N; r++) {
+  // interior rows j = -1..n: the window F(j-1..j+2) slides one row a step;
+  // unrolled so that the loads of later rows issue ahead of the recurrence
+  double f0 = F(-2), f1 = F(-1), f2 = F(0);
+#pragma unroll 8
+  for (int r = 1; r < N - 1; r++) {
     const int j = r - 2;
-    double d;
-    if (r == N - 1) d = end_row(j);
-    else if (plus) d = w0 * F(j - 1) + w1 * F(j) + w2 * F(j + 1) + w3 * F(j + 2);
-    else d = w0 * F(j + 2) + w1 * F(j + 1) + w2 * F(j) + w3 * F(j - 1);
+    const double f3 = F(j + 2);
+    const double d = plus ? w0 * f0 + w1 * f1 + w2 * f2 + w3 * f3 : w0 * f3 + w1 * f2 + w2 * f1 + w3 * f0;
     dp = d * ac[3 * r + 1] - dp * ac[3 * r + 2];
     dst[cuw_at<DIR>(L, j, NR, NL)] = dp;
+    f0 = f1;
+    f1 = f2;
+    f2 = f3;
   }
+  dp = end_row(n + 1) * ac[3 * (N - 1) + 1] - dp * ac[3 * (N - 1) + 2];
+  dst[cuw_at<DIR>(L, n + 1, NR, NL)] = dp;
   double x = dp;
-#pragma unroll 1
+#pragma unroll 8
   for (int r = N - 2; r >= 0; r--) {
     const int j = r - 2;
     const long long c = cuw_at<DIR>(L, j, NR, NL);
@@ -746,15 +756,9 @@ __global__ __launch_bounds__(256) void k_cuw_face(FlowArgs a, UwArgs u) {
   const long long NI = n + 2;
   const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
   if (idx >= NI * NL) return;
-  long long L;
-  int i;
-  if (DIR == 0) {
-    L = idx / NI;
-    i = (int)(idx - L * NI) - 1;
-  } else {
-    i = (int)(idx / NL) - 1;
-    L = idx - (long long)(i + 1) * NL;
-  }
+  // consecutive lanes take consecutive lines, as the split buffer holds them
+  const int i = (int)(idx / NL) - 1;
+  const long long L = idx - (long long)(i + 1) * NL;
   int c3[3];
   const long long o0 = line_base(a, DIR, L, c3);
   const long long st = dir_stride(a, DIR);
@@ -832,6 +836,8 @@ __global__ __launch_bounds__(256) void k_cuw_diff(FlowArgs a, UwArgs u, int firs
   long long L;
   int t;
   if (DIR == 0) {
+    // x: consecutive lanes take consecutive points of a line, so that qrhs
+    // coalesces
     L = idx / NP;
     t = (int)(idx - L * NP);
   } else {

@@ -305,8 +305,8 @@ def _host_init_vort(im, jm, mach=0.3):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case,conv", [("tgv2d", None), ("2dvort", None), ("2dvort", (3, False)),
-                                       ("2dvort", (5, True))])
+@pytest.mark.parametrize("case,conv", [("tgv2d", None), ("2dvort", None), ("2dvort", ("753e", 3, False)),
+                                       ("2dvort", ("753e", 5, True)), ("2dvort", ("543c", 3, True))])
 def test_astr_hip_runs_2d_examples(tmp_path, case, conv):
     """The reference's 2-D examples (Taylor_Green_Vortex_2D; Vortex_Transport,
     inviscid with diffterm = f) cut to 64 x 48 / 80 x 40 and 5 steps with a
@@ -315,8 +315,9 @@ def test_astr_hip_runs_2d_examples(tmp_path, case, conv):
     1e-13: numpy's sin, cos and exp may differ from libm's in the last bit),
     and the main-path oracle to 1e-12 (1/Re = 0 for the inviscid case).
     conv: the vortex with the convective scheme changed to the explicit
-    upwind-biased 753e and (recon_schem, lchardecomp) on the input line
-    that the example already carries (bfacmpld 0.3, shkcrt 0.01)."""
+    upwind-biased 753e or the compact upwind-biased 543c, and (recon_schem,
+    lchardecomp) on the input line that the example already carries
+    (bfacmpld 0.3, shkcrt 0.01)."""
     if case == "tgv2d":
         datin, inp, grid, new = os.path.join(GOLD, "datin_tgv2d"), "input.tgv2d", "128,128,0", "64,48,0"
         cont = [("10,      10,     100,      50,      1,    50", "4,      4,     100,      50,      1,    50")]
@@ -327,12 +328,14 @@ def test_astr_hip_runs_2d_examples(tmp_path, case, conv):
         dt, mach, re = 5e-3, 0.3, float("inf")
     edits = [(grid, new)]
     if conv is not None:
-        edits += [("643c, 643c, rk3", "753e, 643c, rk3"),
-                  ("3, f, 0.3d0, 0.01d0", f"{conv[0]}, {'t' if conv[1] else 'f'}, 0.3d0, 0.01d0")]
+        edits += [("643c, 643c, rk3", f"{conv[0]}, 643c, rk3"),
+                  ("3, f, 0.3d0, 0.01d0", f"{conv[1]}, {'t' if conv[2] else 'f'}, 0.3d0, 0.01d0")]
     r = _astr_run(tmp_path, datin, inp, edits=edits, cont_edits=cont)
     assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-2000:]
-    if conv is not None:
-        assert f"upwind-biased convection, recon_schem {conv[0]}" in r.stdout
+    if conv is not None and conv[0] == "543c":
+        assert f"compact upwind-biased convection 543c, lchardecomp {'T' if conv[2] else 'F'}" in r.stdout
+    elif conv is not None:
+        assert f"upwind-biased convection, recon_schem {conv[1]}" in r.stdout
     im, jm = [int(v) for v in new.split(",")[:2]]
     if case == "tgv2d":
         X, q, rho, vel, prs, tmp = _host_init_2d(im, jm)
@@ -348,7 +351,7 @@ def test_astr_hip_runs_2d_examples(tmp_path, case, conv):
     f.upload(q)
     f.set_fvar(rho, vel, prs, tmp)
     if conv is not None:
-        f.set_conschm("753e", conv[0], conv[1], 0.3, 0.01)
+        f.set_conschm(conv[0], conv[1], conv[2], 0.3, 0.01)
     f.rk3(5, stats=False)
     qc, pc = f.download(prims=True)
     g = A.Flow(cfg)
@@ -365,7 +368,7 @@ def test_astr_hip_runs_2d_examples(tmp_path, case, conv):
         od[a + 3 * a] = float(np.mean(dxi[a, a], dtype=np.longdouble))
     s = O.MainState(ocfg, oj, od)
     if conv is not None:
-        s.set_conv(True, conv[0], conv[1], 0.3, 0.01)
+        s.set_conv(2 if conv[0] == "543c" else 1, conv[1], conv[2], 0.3, 0.01)
     s.set_q(q)
     s.rk3(5)
     qo, _ = s.get()
