@@ -156,8 +156,9 @@ module astr_hip_capi
       type(c_ptr), value :: ctx, jacob, dxi
       integer(c_int) :: astr_hip_set_metrics
     end function
-    ! the convective scheme: central 643c (upwind = 0) or the explicit
-    ! upwind-biased one (convrsduwd, src/solver.F90:548-1201)
+    ! the convective scheme: central 643c (upwind = 0), the explicit
+    ! upwind-biased one (1; convrsduwd, src/solver.F90:548-1201) or the
+    ! compact upwind-biased 543c (2; convrsdcmp, src/solver.F90:1271-1937)
     function astr_hip_set_convection(ctx, upwind, recon_schem, lchardecomp, bfacmpld, shkcrt) &
         bind(C, name='astr_hip_set_convection')
       import :: c_int, c_ptr, c_double
