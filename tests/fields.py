@@ -27,3 +27,20 @@ def periodic_field(cfg, seed, amp=0.05):
     q[1], q[2], q[3] = rho * u, rho * v, rho * w
     q[4] = p / 0.4 + 0.5 * rho * (u * u + v * v + w * w)
     return q
+
+
+def double_shock_tube(im, jm, km=0):
+    """A periodic double shock tube along x: Sod's states, rho = 1, p = 1 for
+    x in [pi/2, 3 pi/2) and rho = 0.125, p = 0.1 elsewhere, at rest, the
+    same on every y (and z) line.  Returns q (5, km+1, jm+1, im+1) for the
+    nondimensional E = p/(gamma-1) + rho |u|^2/2 with gamma = 1.4."""
+    ii = np.arange(im + 1) % im
+    x = 2 * np.pi * ii / im
+    inside = (x >= np.pi / 2) & (x < 3 * np.pi / 2)
+    shape = (km + 1, jm + 1, im + 1)
+    rho = np.broadcast_to(np.where(inside, 1.0, 0.125), shape)
+    p = np.broadcast_to(np.where(inside, 1.0, 0.1), shape)
+    q = np.zeros((5,) + shape)
+    q[0] = rho
+    q[4] = p / 0.4
+    return q

@@ -227,6 +227,44 @@ def test_compact_upwind_full_metric_tensor_matches_oracle():
     f.close()
 
 
+@pytest.mark.parametrize("dims", [(128, 12, 0), (64, 12, 12)])
+@pytest.mark.parametrize("conschm,recon,chardec", [("543c", 0, False), ("543c", 0, True), ("753e", 3, False),
+                                                   ("753e", 5, True)])
+def test_shock_tube_matches_oracle(dims, conschm, recon, chardec):
+    """A periodic double shock tube (Sod's states, Ma = 1, inviscid) through
+    20 RK3 steps: the limiters act at the shocks, contacts and rarefaction
+    edges, and the device stays on the oracle to 1e-12 and within the
+    states' density range."""
+    from fields import double_shock_tube
+    im, jm, km = dims
+    dt = 5e-3 * 128 / im
+    ocfg = O.tgv_cfg(im, jm, km, dt=dt, mach=1.0, reynolds=1e30, suth_dim=110.3)
+    q = double_shock_tube(im, jm, km)
+    shape = q.shape[1:]
+    jac, od = O.uniform_metrics(ocfg)
+    dev_dxi = np.zeros((3, 3) + shape)
+    for a in range(3):
+        for b in range(3):
+            dev_dxi[a, b] = od[a + 3 * b].reshape(shape)
+    s = O.MainState(ocfg, jac, od)
+    s.set_conv(2 if conschm == "543c" else 1, recon, chardec, 0.3, 0.05)
+    s.set_q(q)
+    cfg = A.main_config(im, jm, km, deltat=dt, mach=1.0)
+    cfg.reynolds = 1e30
+    f = A.Flow(cfg)
+    f.set_metrics(jac.reshape(shape), dev_dxi)
+    f.set_conschm(conschm, recon, chardec, 0.3, 0.05)
+    f.upload(q)
+    f.rk3(20, stats=False)
+    s.rk3(20)
+    qd, prd = f.download(prims=True)
+    qo, pro = s.get()
+    assert max(state_err(qd, qo)) < TOL, state_err(qd, qo)
+    assert relmax(prd, pro) < TOL
+    assert 0.125 * 0.98 < qd[0].min() and qd[0].max() < 1.02
+    f.close()
+
+
 def test_upwind_refusals():
     f = A.Flow(A.main_config(16, 16, 16, deltat=1e-3))
     with pytest.raises(A.AstrError, match="recon_schem"):
