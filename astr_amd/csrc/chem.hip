@@ -586,9 +586,11 @@ __device__ __forceinline__ void add_at(double (&v)[NS], int idx, double x) {
     if (idx == k) v[k] += x;
 }
 
+// Inlined at every call, so that the caller's state arrays stay in registers
+// (a call would take them by address, in scratch memory).
 template <int NS>
-__device__ void reactor_rhs_n(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
-                              double (&f)[NS + 1]) {
+__device__ __forceinline__ void reactor_rhs_n(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
+                                              double (&f)[NS + 1]) {
   const double T = yv[0];
   double c[NS], g[NS], w[NS], cpr[NS], hrt[NS];
   double winv = 0.0;
@@ -669,8 +671,8 @@ __device__ __forceinline__ double conc_pow_d(double c, double nu) {
 // f(y).  Every column is computed alone, so a team evaluates them on separate
 // lanes with the arithmetic of the one-thread kernel's sequential loop.
 template <int NS>
-__device__ void reactor_jac_col(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
-                                const double (&f0)[NS + 1], int j, double (&col)[NS + 1]) {
+__device__ __forceinline__ void reactor_jac_col(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
+                                                const double (&f0)[NS + 1], int j, double (&col)[NS + 1]) {
 #pragma clang fp contract(off)
   const double T = yv[0];
   double c[NS], g[NS], cpr[NS], hrt[NS], dw[NS];
@@ -984,6 +986,68 @@ __device__ __forceinline__ void team_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// ro_stages for a team: the same six stages in the same arithmetic, with the
+// stage vectors U_s in the team's LDS (Us, 6 N doubles) instead of in each
+// lane's registers.  Every lane computes each stage (the right-hand side and
+// the triangular solves, as the one-thread kernel does); lane k < N then
+// stores component k of U_s, and the next stage reads them all back.  The
+// stage loop stays rolled, so that the right-hand side is inlined once.
+template <int NS>
+__device__ __forceinline__ double ro_stages_team(const Mech* __restrict__ M, double p, double h, const double* a,
+                                                 const int (&piv)[NS + 1], const double (&y)[NS + 1],
+                                                 const double (&f0)[NS + 1], double (&ynew)[NS + 1],
+                                                 double rtol, double atol, double* Us, int lane) {
+  constexpr int N = NS + 1;
+  double u[N], yt[N], f1[N];
+  const double gh = kRoGamma * h;
+#pragma unroll
+  for (int k = 0; k < N; k++) u[k] = f0[k];
+  lu_solve_n<N, 1>(a, piv, u);
+#pragma unroll
+  for (int k = 0; k < N; k++) u[k] = gh * u[k];
+  if (lane < N) Us[lane] = pick<N>(u, lane);
+  team_sync();
+#pragma unroll 1
+  for (int s = 1; s < 6; s++) {
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double v = y[k];
+      // Y_s = y + sum_j a_sj U_j (s < 5); Y_6 = Y_5 + U_5
+      const int nj = s < 5 ? s : 4;
+      const int ra = s < 5 ? s - 1 : 3;
+#pragma unroll 1
+      for (int j = 0; j < nj; j++) v += kRoA[ra][j] * Us[j * N + k];
+      if (s == 5) v = v + Us[4 * N + k];
+      yt[k] = v;
+    }
+    reactor_rhs_n<NS>(M, p, yt, f1);
+#pragma unroll
+    for (int k = 0; k < N; k++) {
+      double c = 0.0;
+#pragma unroll 1
+      for (int j = 0; j < s; j++) c += (kRoC[s - 1][j] / h) * Us[j * N + k];
+      u[k] = f1[k] + c;
+    }
+    lu_solve_n<N, 1>(a, piv, u);
+#pragma unroll
+    for (int k = 0; k < N; k++) u[k] = gh * u[k];
+    if (s < 5) {
+      if (lane < N) Us[s * N + lane] = pick<N>(u, lane);
+      team_sync();
+    }
+  }
+  double err = 0.0;
+#pragma unroll
+  for (int k = 0; k < N; k++) {
+    const double yn = yt[k] + u[k];
+    const double sc = (k == 0 ? 1e-6 : atol) + rtol * fmax(fabs(y[k]), fabs(yn));
+    const double e = fabs(u[k]) / sc;
+    if (!(e <= err)) err = e;  // a NaN makes err NaN and rejects the step
+    ynew[k] = yn;
+  }
+  return err;
+}
+
 template <int NS>
 __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M, ReactorPool P,
                                                      int max_tries, double rtol, double atol) {
@@ -991,6 +1055,7 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
   constexpr int TPB = 64 / kTeam;
   static_assert(N < kTeam, "a team needs a lane per column and one for the unperturbed state");
   __shared__ double sA[TPB][N * N];
+  __shared__ double sU[TPB][6 * N];
   const int team = threadIdx.x / kTeam, lane = threadIdx.x % kTeam;
   const int i = blockIdx.x * TPB + team;
   if (i >= P.cap || P.status[i] != RX_RUN) return;  // the whole team leaves
@@ -1048,7 +1113,7 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
       team_sync();
     }
     double err = 2.0;
-    if (ok) err = ro_stages<NS, 1>(M, p, h, a, piv, y, f0, yt, rtol, atol);
+    if (ok) err = ro_stages_team<NS>(M, p, h, a, piv, y, f0, yt, rtol, atol, sU[team], lane);
     // the next step's matrix writes must follow every lane's solve reads
     team_sync();
     if (err <= 1.0) {
