@@ -586,11 +586,29 @@ __device__ __forceinline__ void add_at(double (&v)[NS], int idx, double x) {
     if (idx == k) v[k] += x;
 }
 
+// Lanes of a reactor team (one DPP row); the team kernel is further down.
+constexpr int kTeam = 16;
+
+// The sum of v over the 16 lanes of a team: a fixed tree into the team's
+// lane 0 (the same association on every team), then lane 0's sum to every
+// lane, so that all lanes of the team hold the same bits.
+__device__ __forceinline__ double team_sum(double v) {
+  v += __shfl_down(v, 8, kTeam);
+  v += __shfl_down(v, 4, kTeam);
+  v += __shfl_down(v, 2, kTeam);
+  v += __shfl_down(v, 1, kTeam);
+  return __shfl(v, 0, kTeam);
+}
+
 // Inlined at every call, so that the caller's state arrays stay in registers
-// (a call would take them by address, in scratch memory).
-template <int NS>
+// (a call would take them by address, in scratch memory).  LANES = kTeam:
+// the team evaluates one right-hand side together, lane l taking reactions
+// l, l + 16, ...; each species rate is then summed over the team (team_sum),
+// and every lane ends with the same f.  LANES = 1: one thread, every
+// reaction in order.
+template <int NS, int LANES = 1>
 __device__ __forceinline__ void reactor_rhs_n(const Mech* __restrict__ M, double p, const double (&yv)[NS + 1],
-                                              double (&f)[NS + 1]) {
+                                              double (&f)[NS + 1], int lane = 0) {
   const double T = yv[0];
   double c[NS], g[NS], w[NS], cpr[NS], hrt[NS];
   double winv = 0.0;
@@ -612,7 +630,7 @@ __device__ __forceinline__ void reactor_rhs_n(const Mech* __restrict__ M, double
     w[k] = 0.0;
   }
   const double lc0 = log(M->p0 / (kRgas * T));
-  for (int r = 0; r < M->nr; r++) {
+  for (int r = LANES > 1 ? lane : 0; r < M->nr; r += LANES) {
     const double kf = exp(M->lnA[r] + M->b[r] * logT - M->EaR[r] * invT);
     double fwd = kf;
     double dg = 0.0;
@@ -641,6 +659,10 @@ __device__ __forceinline__ void reactor_rhs_n(const Mech* __restrict__ M, double
     }
     for (int j = 0; j < M->nre[r]; j++) add_at<NS>(w, M->rsp[r][j], -M->rnu[r][j] * q);
     for (int j = 0; j < M->npr[r]; j++) add_at<NS>(w, M->psp[r][j], M->pnu[r][j] * q);
+  }
+  if constexpr (LANES > 1) {
+#pragma unroll
+    for (int k = 0; k < NS; k++) w[k] = team_sum(w[k]);
   }
   double cp = 0.0, dh = 0.0;
 #pragma unroll
@@ -965,21 +987,20 @@ __global__ __launch_bounds__(64) void k_reactor_eval(const Mech* __restrict__ M,
 
 // ---------------------------------------------------------------------------
 // The same reactor integrated by a team of 16 lanes (one DPP row), four
-// reactors per wavefront, for latency: the step's Jacobian columns, which
-// do not depend on each other, are evaluated by separate lanes at once (lane
-// j the analytic column j, after every lane has the right-hand side), and
-// the LU elimination runs one row per lane.  The pivot search, both
-// triangular solves, the later stages and the step control are done by every
-// lane of the team, in the one-thread kernel's order, so each matrix entry,
-// factor and solution component goes through the same operations as there.
+// reactors per wavefront, for latency: each right-hand side is evaluated by
+// the whole team (the reactions split over the lanes, the species rates
+// summed over the team into identical values on every lane), the step's
+// Jacobian columns, which do not depend on each other, by separate lanes at
+// once (lane j the analytic column j), and the LU elimination runs one row
+// per lane.  The pivot search, both triangular solves, the stage sums and
+// the step control are done by every lane of the team, in the one-thread
+// kernel's order; only the species-rate sums associate differently.
 // A team's lanes hold identical scalars (t, h, err), so its control flow is
 // uniform; teams of one wavefront may diverge, which is why the kernel uses
 // no workgroup barrier.  The team's matrix sits in LDS; a team is part of one
 // wavefront, whose LDS accesses complete in order, so a wave barrier (for
 // the compiler) orders one lane's write before another lane's read.
 // ---------------------------------------------------------------------------
-constexpr int kTeam = 16;
-
 __device__ __forceinline__ void team_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -1020,7 +1041,7 @@ __device__ __forceinline__ double ro_stages_team(const Mech* __restrict__ M, dou
       if (s == 5) v = v + Us[4 * N + k];
       yt[k] = v;
     }
-    reactor_rhs_n<NS>(M, p, yt, f1);
+    reactor_rhs_n<NS, kTeam>(M, p, yt, f1, lane);
 #pragma unroll
     for (int k = 0; k < N; k++) {
       double c = 0.0;
@@ -1073,8 +1094,8 @@ __global__ __launch_bounds__(64) void k_reactor_team(const Mech* __restrict__ M,
     if (t >= tend) { st = RX_DONE; break; }
     if (h > tend - t) h = tend - t;
     if (!(h > 1e-18)) { st = RX_FAIL; break; }
-    // f(y) on every lane, then lane j < N: column j of the analytic Jacobian
-    reactor_rhs_n<NS>(M, p, y, f0);
+    // f(y) by the whole team, then lane j < N: column j of the analytic Jacobian
+    reactor_rhs_n<NS, kTeam>(M, p, y, f0, lane);
     if (lane < N) {
       reactor_jac_col<NS>(M, p, y, f0, lane, f1);
 #pragma unroll

@@ -366,10 +366,13 @@ def test_gpu_reactor_converges_to_radau(mech, lanes):
 
 
 @pytest.mark.gpu
-def test_gpu_reactor_team_equals_one_thread(mech):
-    """The team kernel (16 lanes per reactor: Jacobian columns on separate
-    lanes, row-parallel elimination) takes the same steps and reaches
-    bit-identical states as the one-thread kernel; two slots stay free."""
+def test_gpu_reactor_team_agrees_with_one_thread(mech):
+    """The team kernel (16 lanes per reactor: the reactions of each
+    right-hand side split over the lanes and the species rates summed over
+    the team, Jacobian columns on separate lanes, row-parallel elimination)
+    and the one-thread kernel integrate the same ignitions to within the
+    integrator's tolerance (the team's sums round differently, so the step
+    sequences may part at the rounding level); two slots stay free."""
     gm = AC.Mechanism(MECH)
     rng = np.random.default_rng(11)
     cases = [(rng.uniform(1150.0, 1400.0), rng.uniform(1.0, 10.0) * 101325.0, rng.uniform(0.5, 2.0))
@@ -384,8 +387,11 @@ def test_gpu_reactor_team_equals_one_thread(mech):
         out[lanes] = [pool.get(k) for k in range(6)]
         pool.close()
     for a, b in zip(out[1], out[16]):
-        assert a[3] > 5
-        assert a[0] == b[0] and a[1] == b[1] and np.array_equal(a[2], b[2]) and a[3:] == b[3:]
+        assert a[3] > 5 and b[3] > 5
+        assert a[0] == b[0]                                   # both reached t_end
+        assert abs(b[2][0] / a[2][0] - 1.0) < 1e-5, (a[2][0], b[2][0])
+        assert np.abs(b[2][1:] - a[2][1:]).max() < 1e-6
+        assert abs(a[3] - b[3]) <= max(2, a[3] // 20)         # about the same number of steps
 
 
 @pytest.mark.gpu
