@@ -233,3 +233,21 @@ def test_shock_tube_stays_bounded_where_the_central_scheme_does_not():
         assert 0.125 * 0.98 < rho.min() and rho.max() < 1.0 * 1.02, (up, rec, cd, rho.min(), rho.max())
     rho = run(0, 0, False)
     assert rho.min() < 0.125 * 0.95
+
+
+def test_the_reference_fluxtest_on_periodic_lines():
+    """The reference's own check of this scheme (fluxtest, src/test.F90:670-869:
+    conschm 543c, bfacmpld = 1, the L2 error of (fh(i) - fh(i-1)) / dx
+    against the analytic derivative of a sine) on its periodic directions,
+    128 points: at bfacmpld = 1 the scheme is the symmetric 6th-order one
+    (observed order above 6), at 0.3 the 5th-order upwind one (above 4.5)."""
+    def l2(n, bfac):
+        t = np.arange(-4, n + 5)
+        f = np.sin(2 * np.pi * t / n)
+        fh = O.cuw_flux(n, bfac, True, f)
+        d = (fh[1:] - fh[:-1]) / (2 * np.pi / n)
+        ref = np.cos(2 * np.pi * np.arange(n + 1) / n)
+        return np.sqrt(np.mean((d[1:] - ref[1:]) ** 2))
+    assert l2(128, 1.0) < 2e-11
+    assert np.log2(l2(64, 1.0) / l2(128, 1.0)) > 6.0
+    assert np.log2(l2(64, 0.3) / l2(128, 0.3)) > 4.5
