@@ -363,12 +363,14 @@ __global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
   if (idx >= NL * NR) return;
   long long L;
   int rr;
-  if (dir == 0 && !u.compact) {
-    // explicit scheme: x-lines row-fastest, for k_uw_face_x's lanes
+  if (dir == 0 && !(u.compact && u.chardec)) {
+    // x-lines row-fastest: for k_uw_face_x's lanes, and for k_cuw_line,
+    // which loads whole lines
     L = idx / NR;
     rr = (int)(idx - L * NR);
   } else {
-    // y and z, and every direction of the compact scheme: lines fastest
+    // y and z, and x for the compact scheme's lane-per-line sweeps (with the
+    // characteristic projection): lines fastest
     rr = (int)(idx / NL);
     L = idx - (long long)rr * NL;
   }
@@ -662,7 +664,8 @@ __global__ __launch_bounds__(64) void k_lshock(FlowArgs a, UwArgs u) {
 // (k_cuw_face) and its difference into qrhs (k_cuw_diff).
 // ---------------------------------------------------------------------------
 
-// split-buffer index of row r (-4..n+4) of line L: the compact scheme keeps
+// split-buffer index of row r (-4..n+4) of line L for the compact scheme's
+// lane-per-line sweeps (with the characteristic projection), which keep
 // every direction's lines fastest, so that one lane per line coalesces
 template <int DIR>
 __device__ __forceinline__ long long cuw_at(long long L, int r, int NR, long long NL) {
@@ -853,8 +856,222 @@ __global__ __launch_bounds__(256) void k_cuw_diff(FlowArgs a, UwArgs u, int firs
     face_update1(a, m, o, u.fhc[m * u.sws + c], u.fhc[m * u.sws + cp], first, last);
 }
 
+// ---------------------------------------------------------------------------
+// The compact scheme without the characteristic projection, one pass per
+// direction with every line in LDS: per split component, the upwind and the
+// downwind compact systems of TL lines are solved by the line engine's
+// chunked Thomas recurrences (lane p of a line owns rows p*R-2 ..
+// p*R+R-3, the chunks' carries composed by affine_scan), MP5 turns each
+// interface's compact values into Fh(i) against the 5-point windows, and
+// qrhs takes Fh(t) - Fh(t-1).  Nothing between the split buffer and qrhs
+// goes through HBM.  The recurrences are re-associated across lanes, as in
+// the derivative and filter solves; the arithmetic of each row is the
+// reference's.
+// ---------------------------------------------------------------------------
+
+// LU factors of one chunk of a compact flux system (a != c, so ac1 and ac3
+// differ): c1 = ac1 (0 on the last row), c2 = ac2 (1 on the first), c3 = ac3
+// (0 on the first), all 0 past the last row
+template <int R>
+struct CuwCoef {
+  double c1[R], c2[R], c3[R];
+};
+
+template <int R>
+__device__ __forceinline__ void load_cuw_coef(CuwCoef<R>& C, const double* __restrict__ ac, int M, int u0) {
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int v = u0 + k;
+    const bool ok = v < M;
+    const int vv = ok ? v : 0;
+    C.c1[k] = ok ? ac[3 * vv] : 0.0;
+    C.c2[k] = ok ? ac[3 * vv + 1] : 0.0;
+    C.c3[k] = ok ? ac[3 * vv + 2] : 0.0;
+  }
+}
+
+// flux_compact of one line in LDS: T holds f(-8 .. P*R+2) at offset
+// kLineOff (f(-4..n+4) the split values, zeros elsewhere), X receives
+// fh(-1..n) at X[0..n+1]
+template <int P, int R, bool PLUS>
+__device__ __forceinline__ void solve_line_cuw(const double* __restrict__ T, double* __restrict__ X, int n, int p,
+                                               const CuwCoef<R>& C, const double (&wt)[4]) {
+  constexpr int OFF = 2, H = 3, W = R + 2 * H;
+  const int u0 = p * R;
+  const double* __restrict__ Tw = T + kLineOff + (u0 - OFF - H);
+  double wv[W];
+#pragma unroll
+  for (int k = 0; k < W; k++) wv[k] = Tw[k];
+  const double c37 = 37.0 / 60.0, c2 = 2.0 / 15.0, c1 = 1.0 / 60.0;
+  double y[R];
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const double* c = wv + k + H;  // c[s] = f(r + s), r = u0 + k - OFF
+    const int r = u0 + k - OFF;
+    double d;
+    if (r == -2 || r == n + 1) {
+      const double v1 = c[0] + c[1], v2 = c[-1] + c[2], v3 = c[-2] + c[3];
+      d = c37 * v1 - c2 * v2 + c1 * v3;
+    } else if (PLUS) {
+      d = wt[0] * c[-1] + wt[1] * c[0] + wt[2] * c[1] + wt[3] * c[2];
+    } else {
+      d = wt[0] * c[2] + wt[1] * c[1] + wt[2] * c[0] + wt[3] * c[-1];
+    }
+    y[k] = d;
+  }
+  // forward recurrence d'(v) = d(v) ac2(v) - d'(v-1) ac3(v), zero carry-in
+  double B = 0.0, A = 1.0;
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    B = y[k] * C.c2[k] - B * C.c3[k];
+    A *= -C.c3[k];
+    y[k] = B;
+  }
+  const double cin = affine_scan<P, true>(A, B);
+  {
+    double m = 1.0;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+      m *= -C.c3[k];
+      y[k] = fma(m, cin, y[k]);
+    }
+  }
+  // backward recurrence x(v) = d'(v) - ac1(v) x(v+1), zero carry from the right
+  B = 0.0;
+  A = 1.0;
+#pragma unroll
+  for (int k = R - 1; k >= 0; k--) {
+    B = y[k] - C.c1[k] * B;
+    A *= -C.c1[k];
+    y[k] = B;
+  }
+  const double xin = affine_scan<P, false>(A, B);
+  {
+    double g = 1.0;
+#pragma unroll
+    for (int k = R - 1; k >= 0; k--) {
+      g *= -C.c1[k];
+      y[k] = fma(g, xin, y[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < R; k++) {
+    const int r = u0 + k - OFF;
+    if (r >= -1 && r <= n) X[r + 1] = y[k];
+  }
+}
+
+template <class C>
+constexpr size_t cuw_line_lds() {
+  return 4 * (size_t)C::TL * C::LT * sizeof(double);
+}
+
+template <class C, int DIR>
+__global__ __launch_bounds__(C::NT) void k_cuw_line(FlowArgs a, UwArgs u, int first, int last) {
+  extern __shared__ double smem[];
+  constexpr int TL = C::TL, P = C::P, R = C::R, LT = C::LT, NT = C::NT;
+  const int n = a.n[DIR];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, DIR);
+  const long long L0 = (long long)blockIdx.x * TL;
+  const long long S = u.sws;
+  double* Tp = smem;            // F+ lines
+  double* Tm = Tp + TL * LT;    // F- lines
+  double* Xp = Tm + TL * LT;    // fh+ (then Fh) at rows -1..n -> 0..n+1
+  double* Xm = Xp + TL * LT;    // fh-
+  const int ln = threadIdx.x / P, p = threadIdx.x % P;
+  CuwCoef<R> Cp, Cm;
+  load_cuw_coef<R>(Cp, u.cuw_ac + cuw_ac_offset(a.n, DIR, true), n + 4, p * R);
+  load_cuw_coef<R>(Cm, u.cuw_ac + cuw_ac_offset(a.n, DIR, false), n + 4, p * R);
+  const double b = u.bfac;
+  const double k118 = 1.0 / 18.0, k136 = 1.0 / 36.0, k1918 = 19.0 / 18.0, k59 = 5.0 / 9.0, k936 = 9.0 / 36.0;
+  const double wt[4] = {k118 - k136 * b, k1918 - k936 * b, k59 + k936 * b, k136 * b};
+  // the rows the split does not fill read as zeros (only the padding of the
+  // chunks past the system's last row and the windows' outer edge touch them)
+  for (int e = threadIdx.x; e < 2 * TL * LT; e += NT) smem[e] = 0.0;
+  __syncthreads();
+#pragma unroll 1
+  for (int m = 0; m < 5; m++) {
+    // rows -4..n+4 of F+_m and F-_m of the tile's TL lines, read in the
+    // split buffer's order (x: each line's rows contiguous; y, z: the lines
+    // side by side)
+    for (int e = threadIdx.x; e < TL * NR; e += NT) {
+      const int l = DIR == 0 ? e / NR : e % TL, rr = DIR == 0 ? e % NR : e / TL;
+      const long long L = L0 + l;
+      double vp = 0.0, vm = 0.0;
+      if (L < NL) {
+        const long long c = DIR == 0 ? L * NR + rr : (long long)rr * NL + L;
+        vp = u.sw[m * S + c];
+        vm = u.sw[(5 + m) * S + c];
+      }
+      Tp[l * LT + kLineOff + rr - 4] = vp;
+      Tm[l * LT + kLineOff + rr - 4] = vm;
+    }
+    __syncthreads();
+    solve_line_cuw<P, R, true>(Tp + ln * LT, Xp + ln * LT, n, p, Cp, wt);
+    solve_line_cuw<P, R, false>(Tm + ln * LT, Xm + ln * LT, n, p, Cm, wt);
+    __syncthreads();
+    // Fh(i), i = -1..n: MP5 of the windows F+(i-2..i+2), F-(i+3..i-1)
+    // against the compact values; every interface is flagged (no sensor)
+    for (int e = threadIdx.x; e < TL * (n + 2); e += NT) {
+      const int l = e % TL, i = e / TL - 1;
+      const double* tp = Tp + l * LT + kLineOff;
+      const double* tm = Tm + l * LT + kLineOff;
+      double wp[5], wm[5];
+#pragma unroll
+      for (int k = 0; k < 5; k++) {
+        wp[k] = tp[i + k - 2];
+        wm[k] = tm[i + 3 - k];
+      }
+      const double v1 = mp5(wp, Xp[l * LT + i + 1], true);
+      const double v2 = mp5(wm, Xm[l * LT + i + 1], true);
+      Xp[l * LT + i + 1] = v1 + v2;
+    }
+    __syncthreads();
+    // qrhs(t) = qrhs(t) + Fh(t) - Fh(t-1), t = 0..n
+    for (int e = threadIdx.x; e < TL * (n + 1); e += NT) {
+      int l, t;
+      if (DIR == 0) {
+        l = e / (n + 1);
+        t = e - l * (n + 1);
+      } else {
+        l = e % TL;
+        t = e / TL;
+      }
+      const long long L = L0 + l;
+      if (L >= NL) continue;
+      int c3[3];
+      const long long o0 = line_base(a, DIR, L, c3);
+      face_update1(a, m, o0 + (long long)t * dir_stride(a, DIR), Xp[l * LT + t + 1], Xp[l * LT + t], first,
+                   last);
+    }
+    __syncthreads();
+  }
+}
+
+// launch shapes of k_cuw_line: the first whose chunks hold the n+4 rows
+using CuwCfg16 = LineCfg<256, 16, 5, 5>;  // P = 16: n <= 76
+using CuwCfg32 = LineCfg<256, 8, 5, 5>;   // P = 32: n <= 156
+using CuwCfg64 = LineCfg<512, 8, 5, 5>;   // P = 64: n <= 316
+using CuwCfg64b = LineCfg<512, 8, 9, 9>;  // P = 64, 9 rows per chunk: n <= 572
+
+template <class C, int DIR>
+hipError_t cuw_line_launch(const FlowArgs& a, const UwArgs& u, bool first, bool last, hipStream_t st) {
+  const long long NL = uw_lines(a.n, DIR);
+  const dim3 g((unsigned)((NL + C::TL - 1) / C::TL));
+  hipLaunchKernelGGL((k_cuw_line<C, DIR>), g, dim3(C::NT), cuw_line_lds<C>(), st, a, u, (int)first, (int)last);
+  return hipGetLastError();
+}
+
 template <int DIR>
 hipError_t cuw_launch_dir(const FlowArgs& a, const UwArgs& u, bool first, bool last, hipStream_t st) {
+  if (!u.chardec) {
+    const int n = a.n[DIR];
+    if (n + 4 <= CuwCfg16::P * CuwCfg16::R) return cuw_line_launch<CuwCfg16, DIR>(a, u, first, last, st);
+    if (n + 4 <= CuwCfg32::P * CuwCfg32::R) return cuw_line_launch<CuwCfg32, DIR>(a, u, first, last, st);
+    if (n + 4 <= CuwCfg64::P * CuwCfg64::R) return cuw_line_launch<CuwCfg64, DIR>(a, u, first, last, st);
+    if (n + 4 <= CuwCfg64b::P * CuwCfg64b::R) return cuw_line_launch<CuwCfg64b, DIR>(a, u, first, last, st);
+  }
   const long long NL = uw_lines(a.n, DIR);
   hipLaunchKernelGGL(k_cuw_solve<DIR>, dim3((unsigned)((10 * NL + 255) / 256)), dim3(256), 0, st, a, u);
   hipError_t e = hipGetLastError();
