@@ -55,18 +55,23 @@ EXPORTS = [
     "astr_hip_batch_rk3_begin", "astr_hip_batch_query", "astr_hip_batch_rk3_end",
     "astr_hip_gridgeom", "astr_hip_gridgeom_nt", "astr_hip_line_batch_nt", "astr_hip_updatefvar",
     "astr_hip_set_metrics", "astr_hip_qswap",
-    "astr_hip_get_last_split", "astr_hip_rk3_steps_stats_noalloc", "astr_hip_set_fvar",
+    "astr_hip_get_last_split", "astr_hip_set_filter_fusion", "astr_hip_rk3_steps_stats_noalloc",
+    "astr_hip_set_fvar",
     "astr_hip_write_checkpoint", "astr_hip_read_checkpoint", "astr_hip_set_boundary",
     "astr_hip_set_convection",
 ]
 SHAPES = {0: (4, 64, 5), 1: (4, 64, 9), 2: (16, 16, 5), 3: (8, 32, 9), 4: (8, 32, 5),
           5: (8, 64, 5), 6: (4, 32, 9), 7: (2, 64, 5),
           8: (1, 64, 5), 9: (2, 32, 9), 10: (16, 16, 9), 11: (8, 64, 9), 12: (1, 64, 9)}
-PROF_SLOTS = 36
+PROF_SLOTS = 39
+# pass programs 0..9; filter_gc_first is gc_first preceded by the previous
+# stage's x filter sweep (rk3 steps with filter fusion, the default)
 PASS_NAMES = ["gc_first", "gc_mid", "gc_last", "visc_mid", "visc_last_rhs",
-              "visc_last_rk3_filter", "filter", "gc_last_grad", "gc_last_conv"]
-# the profile slots after the passes: the upwind-biased convection's kernels
-PROF_NAMES = PASS_NAMES + ["uw_split", "uw_face", "uw_sensor"]
+              "visc_last_rk3_filter", "filter", "gc_last_grad", "gc_last_conv",
+              "filter_gc_first"]
+# profile slots (3 per name): passes 0..8, the upwind-biased convection's
+# kernels, then pass 9
+PROF_NAMES = PASS_NAMES[:9] + ["uw_split", "uw_face", "uw_sensor"] + PASS_NAMES[9:]
 
 _lib = None
 
@@ -119,6 +124,7 @@ def lib():
         L.astr_hip_init_vortex.argtypes = [C.c_void_p] + [C.c_double] * 5
         L.astr_hip_set_last_split.argtypes = [C.c_void_p, C.c_int]
         L.astr_hip_get_last_split.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.astr_hip_set_filter_fusion.argtypes = [C.c_void_p, C.c_int]
         L.astr_hip_batch_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.POINTER(C.c_void_p)]
         L.astr_hip_batch_destroy.argtypes = [C.c_void_p]
         L.astr_hip_batch_autotune.argtypes = [C.c_void_p, C.c_int]
@@ -363,6 +369,8 @@ class Flow:
         nd = 3 if self.cfg.km > 0 else 2
         used = {0: [0], 1: list(range(1, nd - 1)), 2: [nd - 1], 3: list(range(nd - 1)),
                 4: [nd - 1], 5: [nd - 1], 6: list(range(nd)), 7: [nd - 1], 8: [nd - 1]}
+        if self.cfg.semantics != SEM_MAIN:
+            used[9] = [0]
         out = {}
         for prog, dirs in used.items():
             for d in dirs:
@@ -384,6 +392,12 @@ class Flow:
     @split_last.setter
     def split_last(self, on):
         _chk(lib().astr_hip_set_last_split(self.h, 1 if on else 0))
+
+    def set_filter_fusion(self, on):
+        """rk3 steps: run each stage's x filter sweep inside the next stage's
+        first pass within a call (on) or as its own pass (off, the default;
+        autotune picks the faster)."""
+        _chk(lib().astr_hip_set_filter_fusion(self.h, 1 if on else 0))
 
 
 def vortex_config(im, jm=None, mach=0.5, reynolds=1.0e5, length=10.0, cfl=0.67, deltat=None):

@@ -16,10 +16,11 @@
 //   V x, V y        viscous-flux derivatives (diffusion :700-787) into qrhs
 //   V z (last)      same along z, then the RK3 update (:1053-1059) and the
 //                   z sweep of filterq (:857-873), in place on q
-//   F x, F y        the x and y sweeps of filterq (:805-847)
+//   F y, F x        the y and x sweeps of filterq (:805-847); optionally the
+//                   x sweep runs at the start of the next stage's GC x pass
 //
 // The filter sweeps act on different indices and are linear, so applying
-// them z,x,y instead of x,y,z gives the same field up to rounding.
+// them z,y,x instead of x,y,z gives the same field up to rounding.
 // Primitive variables are recomputed from q inside every pass
 // (q2fvar :92-121) instead of being stored.
 #include <hip/hip_runtime.h>
@@ -455,6 +456,12 @@ struct astr_hip_ctx {
   // the last direction's gradient/convective work in two kernels
   // (GL_GRAD + GL_CONV) instead of GC_LAST; chosen by the autotuner
   int split_last = 0;
+  // rk3_run: each stage's x filter sweep runs in the next stage's first pass
+  // (FGC_FIRST) inside one call (fuse_fx, off unless the autotuner finds the
+  // fused pass faster than the two it replaces); fx_pending says that the
+  // sweep of the stage just ended is still owed
+  int fuse_fx = 0;
+  bool fx_pending = false;
   double* base = nullptr;   // one allocation for all fields
   double* coef = nullptr;   // LU tables
   double* stats_part = nullptr;
@@ -682,7 +689,8 @@ int timed(astr_hip_ctx* c, int slot, F&& launch) {
   }
   return 0;
 }
-constexpr int kSlotUwSplit = 27, kSlotUwFace = 30, kSlotUwSensor = 33;
+constexpr int kSlotUwSplit = 27, kSlotUwFace = 30, kSlotUwSensor = 33, kSlotFgcFirst = 36;
+static_assert(kSlotFgcFirst + 3 == ASTR_PROF_SLOTS, "profile slots");
 
 int pass(astr_hip_ctx* c, int prog, int d) {
   const bool rec = c->prof && c->ev_used < (int)c->ev_kind.size();
@@ -690,7 +698,7 @@ int pass(astr_hip_ctx* c, int prog, int d) {
   HIP_OK(launch_pass(c->shape[prog][d], prog, c->minb[prog][d], c->a, d, c->stream));
   if (rec) {
     HIP_OK(hipEventRecord(c->ev[2 * c->ev_used + 1], c->stream));
-    c->ev_kind[c->ev_used++] = prog * 3 + d;
+    c->ev_kind[c->ev_used++] = prog < FGC_FIRST ? prog * 3 + d : kSlotFgcFirst + d;
   }
   return 0;
 }
@@ -711,8 +719,9 @@ int run_rhs(astr_hip_ctx* c, bool rk, bool stats, bool store_last_grad, double* 
       if (pass(c, GL_GRAD, d) || (!a.uwd && pass(c, GL_CONV, d))) return -1;
       continue;
     }
-    const int prog = d == nd - 1 ? GC_LAST : (d == 0 ? GC_FIRST : GC_MID);
+    const int prog = d == nd - 1 ? GC_LAST : (d == 0 ? (c->fx_pending ? FGC_FIRST : GC_FIRST) : GC_MID);
     if (pass(c, prog, d)) return -1;
+    if (d == 0) c->fx_pending = false;
   }
   if (a.uwd) {
     // rhscal's upwind branches (solver.F90:219-233): ducrossensor when
@@ -1158,11 +1167,22 @@ int rk3_run(astr_hip_ctx* c, int nsteps, const StatsBufs& sb, double* ke, double
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) rc = fail(std::string("k_stats_reduce: ") + hipGetErrorString(e));
       }
+      // filterq's other sweeps, y then x (z went with the RK update): inside
+      // the call, the x sweep of every stage but the last is left to the next
+      // stage's first pass, which loads the same lines
       const int nd = a.ndir;
-      for (int d = 0; d < nd - 1 && !rc && !a.main_sem; d++)
-        if (pass(c, F_ONLY, d)) rc = -1;
+      if (!rc && !a.main_sem) {
+        for (int d = nd - 2; d >= 1 && !rc; d--)
+          if (pass(c, F_ONLY, d)) rc = -1;
+        const bool defer = c->fuse_fx && !a.uwd && !(it == nsteps - 1 && s == 2);
+        if (!rc) {
+          if (defer) c->fx_pending = true;
+          else if (pass(c, F_ONLY, 0)) rc = -1;
+        }
+      }
     }
   }
+  c->fx_pending = false;
   // the RK passes changed q after the last qswap
   c->halo_valid = false;
   a.stats = stats_keep;
@@ -1642,7 +1662,8 @@ int astr_hip_profile_end(astr_hip_ctx* c, double* ms, int* count) {
 // pass programs a case of ndir directions launches, per direction
 static bool pass_used(int prog, int d, int nd) {
   switch (prog) {
-    case GC_FIRST: return d == 0;
+    case GC_FIRST:
+    case FGC_FIRST: return d == 0;
     case GC_MID: return d > 0 && d < nd - 1;
     case GC_LAST:
     case GL_GRAD:
@@ -1681,7 +1702,7 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   int rc = 0;
   for (int pr = 0; pr < kNumProgs && !rc; pr++) {
     for (int d = 0; d < a.ndir && !rc; d++) {
-      if (!pass_used(pr, d, a.ndir)) continue;
+      if (!pass_used(pr, d, a.ndir) || (pr == FGC_FIRST && a.main_sem)) continue;
       float best = 1e30f;
       for (int sh = 0; sh < kNumShapes && !rc; sh++) {
         if (!selection_ok(sh, pr, a.n[d], c->device)) continue;
@@ -1729,6 +1750,9 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
       if (e != hipSuccess) rc = fail(std::string("autotune rhs: ") + hipGetErrorString(e));
     }
     if (!rc) c->split_last = form_ms[1] < form_ms[0];
+    // the filtering first pass against the first pass plus the x filter pass
+    if (!rc && !a.main_sem)
+      c->fuse_fx = c->tuned_ms[FGC_FIRST][0] < c->tuned_ms[GC_FIRST][0] + c->tuned_ms[F_ONLY][0];
   }
   a = keep;
   hipError_t e = hipMemcpyAsync(a.q, snap, qbytes, hipMemcpyDeviceToDevice, c->stream);
@@ -1765,6 +1789,12 @@ int astr_hip_set_pass_config(astr_hip_ctx* c, int prog, int dir, int shape, int 
 int astr_hip_set_last_split(astr_hip_ctx* c, int split) {
   if (check_ctx(c)) return -1;
   c->split_last = split ? 1 : 0;
+  return 0;
+}
+
+int astr_hip_set_filter_fusion(astr_hip_ctx* c, int on) {
+  if (check_ctx(c)) return -1;
+  c->fuse_fx = on ? 1 : 0;
   return 0;
 }
 

@@ -90,8 +90,13 @@ __host__ __device__ inline int halo_src_index(const int n[3], int ndir, int dir,
 
 // Pass programs.  GL_GRAD + GL_CONV are GC_LAST split in two kernels (the
 // gradient/stress half and the convective half), each with fewer registers.
-enum Prog { GC_FIRST = 0, GC_MID, GC_LAST, V_MID, V_LAST_RHS, V_LAST_RK, F_ONLY, GL_GRAD, GL_CONV };
-constexpr int kNumProgs = 9;
+// FGC_FIRST is GC_FIRST preceded by the previous stage's filter sweep along
+// the same lines (x): it filters the q it loads, writes it back and goes on
+// with the filtered values (the sweeps along different indices commute, so
+// the x sweep may come last).
+enum Prog { GC_FIRST = 0, GC_MID, GC_LAST, V_MID, V_LAST_RHS, V_LAST_RK, F_ONLY, GL_GRAD, GL_CONV,
+            FGC_FIRST };
+constexpr int kNumProgs = 10;
 
 // ntiles of direction dir for lines-per-tile TL
 __host__ __device__ inline int tiles_of(const int n[3], int dir, int TL) {
@@ -171,7 +176,7 @@ constexpr int lds_coef_offset() {
 }
 // Regions of 2 P R doubles a pass stages: one (the derivative's) in the last
 // gradient pass and the viscous passes, one (the filter's) in the filter
-// pass, two in the RK pass, none in the first two gradient passes, which
+// pass and in the filtering first gradient pass, two in the RK pass, none in the first two gradient passes, which
 // keep the factors in registers (there the freed registers would not reach
 // another wave per SIMD, and the reloads only cost time).
 constexpr int coef_regions(int prog) {
@@ -535,8 +540,9 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
   const double idn = a.inv_d[dir];
   const long long fs = a.fs;
 
-  if constexpr (PROG == GC_FIRST || PROG == GC_MID || PROG == GC_LAST || PROG == GL_GRAD ||
-                PROG == GL_CONV) {
+  if constexpr (PROG == GC_FIRST || PROG == FGC_FIRST || PROG == GC_MID || PROG == GC_LAST ||
+                PROG == GL_GRAD || PROG == GL_CONV) {
+    constexpr bool FIRST = PROG == GC_FIRST || PROG == FGC_FIRST;
     constexpr bool LAST = PROG == GC_LAST || PROG == GL_GRAD || PROG == GL_CONV;
     constexpr bool DO_GRAD = PROG != GL_CONV;
     constexpr bool DO_CONV = PROG != GL_GRAD;
@@ -551,6 +557,31 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
     for (int t = 0; t < NPT; t++)
 #pragma unroll
       for (int m = 0; m < 5; m++) qv[m][t] = ldg(a.q + m * fs, S.g[t]);
+    if constexpr (PROG == FGC_FIRST) {
+      // the previous stage's filter sweep along these lines (filterq,
+      // tgvsolver.F90:805-825), as the filter pass does it, on the loaded q;
+      // the filtered q is stored and the gradients and fluxes below take it
+      const double* csf = stage_chunk_coefs<C>(a.fac[dir], n + 7, smem, 0);
+      const double c3l = a.fac[dir][2 * (n + 7) + (n + 7) - 1];
+      const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
+#pragma unroll
+      for (int m = 0; m < 5; m++) {
+#pragma unroll
+        for (int t = 0; t < NPT; t++) put<C>(T, S, t, qv[m][t]);
+        __syncthreads();
+        {
+          ChunkCoef<C::R, true> CF;
+          lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
+          solve_line<C::P, C::R, true>(Tl, Xl, n, p, CF, fl);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int t = 0; t < NPT; t++) {
+          qv[m][t] = X[S.s[t]];
+          stg(a.q + m * fs, S.g[t], qv[m][t]);
+        }
+      }
+    }
 
     // main path: the stored primitives the gradient solves take (pv) and
     // the convective part's u_dir and p are loaded with q, at the start, so
@@ -640,7 +671,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
               if (nn >= a.ndir) continue;
               const int f2 = c < 3 ? c + 3 * nn : 9 + nn;
               const double v = x * ldg(a.met + (1 + dir + 3 * nn) * fs, o);
-              if (PROG == GC_FIRST) stg(a.dv + f2 * fs, o, v);
+              if (FIRST) stg(a.dv + f2 * fs, o, v);
               else stg(a.dv + f2 * fs, o, ldg(a.dv + f2 * fs, o) + v);
             }
           }
@@ -811,7 +842,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
       }
       double* dst = a.qrhs + m * fs;
       double qr[NPT];
-      if (PROG != GC_FIRST) {
+      if (!FIRST) {
         // issued before the solve so the load latency overlaps it
 #pragma unroll
         for (int t = 0; t < NPT; t++) qr[t] = ldg(dst, S.g[t]);
@@ -830,7 +861,7 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
         {
           // the main path's fluxes carry the metrics: no 1/dx after the solve
           const double v = MAIN ? X[S.s[t]] : X[S.s[t]] * idn;
-          if (PROG == GC_FIRST) stg(dst, S.g[t], v);
+          if (FIRST) stg(dst, S.g[t], v);
           else if (PROG == GC_MID) stg(dst, S.g[t], qr[t] + v);
           else stg(dst, S.g[t], -(qr[t] + v));
         }
@@ -1105,6 +1136,7 @@ hipError_t launch_batch_c(int prog, const BatchLaunch& bl, int dir, const StageA
     case V_LAST_RK: hipLaunchKernelGGL((k_pass_batch<C, V_LAST_RK, MINB>), g, b, lds, st, bl, dir, sa); break;
     case GL_GRAD: hipLaunchKernelGGL((k_pass_batch<C, GL_GRAD, MINB>), g, b, lds, st, bl, dir, sa); break;
     case GL_CONV: hipLaunchKernelGGL((k_pass_batch<C, GL_CONV, MINB>), g, b, lds, st, bl, dir, sa); break;
+    case FGC_FIRST: return hipErrorInvalidValue;  // single-case passes only
     default: hipLaunchKernelGGL((k_pass_batch<C, F_ONLY, MINB>), g, b, lds, st, bl, dir, sa); break;
   }
   return hipGetLastError();
@@ -1124,6 +1156,11 @@ void launch_pass_m(int prog, const FlowArgs& a, int dir, hipStream_t st) {
     case V_LAST_RK: hipLaunchKernelGGL((k_pass<C, V_LAST_RK, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
     case GL_GRAD: hipLaunchKernelGGL((k_pass<C, GL_GRAD, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
     case GL_CONV: hipLaunchKernelGGL((k_pass<C, GL_CONV, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
+    // the filtering first pass exists for the miniapp's stage only (the main
+    // path filters at the start of its stage); launch_pass_c refuses it there
+    case FGC_FIRST:
+      if constexpr (!MAIN) hipLaunchKernelGGL((k_pass<C, FGC_FIRST, MINB, false, false>), g, b, lds, st, a, dir);
+      break;
     // the filter pass of the main path also takes physical boundaries
     default: hipLaunchKernelGGL((k_pass<C, F_ONLY, MINB, MAIN, GEN>), g, b, lds, st, a, dir); break;
   }
@@ -1131,6 +1168,7 @@ void launch_pass_m(int prog, const FlowArgs& a, int dir, hipStream_t st) {
 
 template <class C, int MINB>
 hipError_t launch_pass_c(int prog, const FlowArgs& a, int dir, hipStream_t st) {
+  if (prog == FGC_FIRST && (a.main_sem || dir != 0)) return hipErrorInvalidValue;
   if (a.main_sem && (a.met || a.bnd[dir])) launch_pass_m<C, MINB, true, true>(prog, a, dir, st);
   else if (a.main_sem) launch_pass_m<C, MINB, true, false>(prog, a, dir, st);
   else launch_pass_m<C, MINB, false, false>(prog, a, dir, st);

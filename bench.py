@@ -38,6 +38,7 @@ FP64_PEAK_TFS = 78.6
 #   visc_last_rk3_filter: viscous z (12) + RK3 update (15+5)
 #               + filter z (5+5)                                       = 42
 #   filter    : one filter sweep (5+5)                                 = 10
+#   filter_gc_first: gc_first with the previous stage's x filter sweep = 30
 # Summed over a 3-D stage: 20+25+50+12+12+42+10+10 = 181, plus the
 # reference's updatefvar (5+6 = 11) that the passes make unnecessary:
 # 192 words = 1536 B per point per stage.
@@ -46,7 +47,7 @@ FP64_PEAK_TFS = 78.6
 # derivatives (12+5 = 17) of the same 50 words.
 PASS_WORDS = {"gc_first": 20, "gc_mid": 25, "gc_last": 50, "visc_mid": 12,
               "visc_last_rhs": 12, "visc_last_rk3_filter": 42, "filter": 10,
-              "gc_last_grad": 33, "gc_last_conv": 17}
+              "gc_last_grad": 33, "gc_last_conv": 17, "filter_gc_first": 30}
 STAGE_BYTES = 1536
 
 
@@ -529,7 +530,7 @@ def main():
         apply_selection(f, json.loads(line)["launch"]["passes"])
     elif not args.no_autotune:
         t0 = time.perf_counter()
-        f.autotune(reps=3)
+        f.autotune(reps=10)
         tune_s = time.perf_counter() - t0
     launch = {f"{k[0]}_{k[1]}": {"shape": A.SHAPES[v[0]], "minb": v[1]}
               for k, v in f.pass_configs().items()}

@@ -226,8 +226,10 @@ int astr_hip_set_convection(astr_hip_ctx *ctx, int upwind, int recon_schem, int 
  * first/middle/last, viscous middle/last(rhs)/last(RK3+filter), filter, and
  * the two halves of the split last convective+gradient pass (gradients and
  * stress, convective); 9..11 = the upwind-biased convection's split,
- * interface-flux and shock-sensor kernels (the sensor in slot 33). */
-#define ASTR_PROF_SLOTS 36
+ * interface-flux and shock-sensor kernels (the sensor in slot 33); 12 = the
+ * first convective+gradient pass with the previous stage's x filter sweep
+ * (program 9, slot 36). */
+#define ASTR_PROF_SLOTS 39
 int astr_hip_profile_begin(astr_hip_ctx *ctx, int max_launches);
 int astr_hip_profile_end(astr_hip_ctx *ctx, double *ms, int *count);
 
@@ -246,6 +248,16 @@ int astr_hip_autotune(astr_hip_ctx *ctx, int reps);
  * two lighter passes (split = 1, programs 7 and 8); astr_hip_autotune sets
  * it from the timings. */
 int astr_hip_set_last_split(astr_hip_ctx *ctx, int split);
+/* Filter-sweep fusion of astr_hip_rk3_steps* (no reference counterpart).
+ * filterq's three sweeps act on different indices and are linear, so they
+ * commute: a stage ends with z (in the RK pass), then y, then x.  With
+ * on != 0 (the compact-scheme solver's stage only), the x sweep of every
+ * stage but a call's last runs inside the next stage's first
+ * convective+gradient pass (program 9), which loads the same x-lines; the
+ * state between calls is the same up to rounding either way.  Off by
+ * default; astr_hip_autotune turns it on when its timing of program 9 is
+ * below those of programs 0 and 6 along x together. */
+int astr_hip_set_filter_fusion(astr_hip_ctx *ctx, int on);
 int astr_hip_get_last_split(astr_hip_ctx *ctx, int *split);
 int astr_hip_get_pass_config(astr_hip_ctx *ctx, int prog, int dir, int *shape, int *minb, float *ms);
 int astr_hip_set_pass_config(astr_hip_ctx *ctx, int prog, int dir, int shape, int minb);

@@ -276,6 +276,36 @@ def test_last_pass_split_matches_oracle(split):
     assert relmax(ke, ke0) < HIST_TOL and relmax(en, en0) < HIST_TOL
 
 
+@pytest.mark.parametrize("dims", [(40, 36, 28), (48, 40, 0)])
+def test_filter_fusion_bit_identical(dims):
+    """The x filter sweep run inside the next stage's first pass (filter
+    fusion on) gives the same bits as its own filter pass when both use one
+    launch shape: same solve, same inputs, and the first pass goes on with
+    the values it stored.  Three steps in one call fuse 8 of the 9 stage
+    boundaries, the first across a step; the call's last sweep runs as its
+    own pass."""
+    f0, s = make_pair(*dims, dt=2e-3)
+    q0 = f0.download()
+    out = []
+    for fuse in (True, False):
+        f = A.Flow(A.tgv_config(*dims, deltat=2e-3))
+        for prog in (0, 6, 9):
+            f.set_pass_config(prog, 0, 8, 1)
+        f.set_filter_fusion(fuse)
+        f.upload(q0)
+        f.profile_begin(200)
+        ke, en = f.rk3(3)
+        prof = f.profile_end()
+        out.append((f.download(), ke, en, prof))
+    (qa, kea, ena, pa), (qb, keb, enb, pb) = out
+    assert pa[("filter_gc_first", "x")][1] == 8 and pa[("filter", "x")][1] == 1
+    assert ("filter_gc_first", "x") not in pb and pb[("filter", "x")][1] == 9
+    assert np.array_equal(qa, qb)
+    assert np.array_equal(kea, keb) and np.array_equal(ena, enb)
+    s.rk3(3)
+    assert max(state_err(qa, s.q)) < 1e-12
+
+
 def test_autotune_restores_state():
     f, s = make_pair(48, 48, 48)
     before = f.download()
