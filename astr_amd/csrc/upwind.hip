@@ -353,27 +353,15 @@ __device__ __forceinline__ void halo_q_at(const FlowArgs& a, int dir, const int*
 // kernels
 // ---------------------------------------------------------------------------
 
+// The Steger-Warming split of row rr (0..n+8, the line's row rr-4) of line
+// L of direction DIR: F+ in out[0..4], F- in out[5..9] and, for the
+// characteristic projection, the state the face kernels read in
+// out[10..18] (rho, u, v, w, p, the energy q_4 and the metric row)
 template <int DIR>
-__global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
+__device__ __forceinline__ void split_point(const FlowArgs& a, const UwArgs& u, long long L, int rr,
+                                            double (&out)[kUwFields + 9]) {
   constexpr int dir = DIR;
   const int n = a.n[dir];
-  const int NR = n + 9;
-  const long long NL = uw_lines(a.n, dir);
-  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
-  if (idx >= NL * NR) return;
-  long long L;
-  int rr;
-  if (dir == 0 && !(u.compact && u.chardec)) {
-    // x-lines row-fastest: for k_uw_face_x's lanes, and for k_cuw_line,
-    // which loads whole lines
-    L = idx / NR;
-    rr = (int)(idx - L * NR);
-  } else {
-    // y and z, and x for the compact scheme's lane-per-line sweeps (with the
-    // characteristic projection): lines fastest
-    rr = (int)(idx / NL);
-    L = idx - (long long)rr * NL;
-  }
   const int r = rr - 4;
   int c3[3];
   const long long o0 = line_base(a, dir, L, c3);
@@ -405,16 +393,90 @@ __global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
   }
   double fp[5], fm[5];
   sw_split(u.gamma, u.mach, jb, dx, q, pv, fp, fm);
-  double* sw = u.sw + idx;
 #pragma unroll
   for (int m = 0; m < 5; m++) {
-    sw[m * u.sws] = fp[m];
-    sw[(5 + m) * u.sws] = fm[m];
+    out[m] = fp[m];
+    out[5 + m] = fm[m];
   }
-  if (u.chardec) {
-    const double st[9] = {pv[0], pv[1], pv[2], pv[3], pv[4], q[4], dx[0], dx[1], dx[2]};
+  out[10] = pv[0]; out[11] = pv[1]; out[12] = pv[2]; out[13] = pv[3]; out[14] = pv[4];
+  out[15] = q[4]; out[16] = dx[0]; out[17] = dx[1]; out[18] = dx[2];
+}
+
+template <int DIR>
+__global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
+  constexpr int dir = DIR;
+  const int n = a.n[dir];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, dir);
+  const long long idx = (long long)blockIdx.x * 256 + threadIdx.x;
+  if (idx >= NL * NR) return;
+  long long L;
+  int rr;
+  if (dir == 0) {
+    // x-lines row-fastest: for k_uw_face_x's lanes, and for k_cuw_line,
+    // which loads whole lines (the compact scheme's sweeps with the
+    // projection take k_uw_split_xt's order instead)
+    L = idx / NR;
+    rr = (int)(idx - L * NR);
+  } else {
+    // y and z: lines fastest
+    rr = (int)(idx / NL);
+    L = idx - (long long)rr * NL;
+  }
+  double out[kUwFields + 9];
+  split_point<DIR>(a, u, L, rr, out);
+  double* sw = u.sw + idx;
+  const int nf = u.chardec ? kUwFields + 9 : kUwFields;
 #pragma unroll
-    for (int m = 0; m < 9; m++) sw[(kUwFields + m) * u.sws] = st[m];
+  for (int m = 0; m < kUwFields + 9; m++)
+    if (m < nf) sw[m * u.sws] = out[m];
+}
+
+// The x split in the lines-fastest order of the compact scheme's sweeps with
+// the projection (row rr of line L at rr*NL + L).  A workgroup takes 16
+// rows of 64 x-lines: it reads q with the rows fastest (each line's 16
+// points are contiguous in memory), stages the 19 outputs in LDS and writes
+// them with the lines fastest (64 lines of a row are contiguous in the split
+// buffer; shorter runs, which start mid cache line since NL is odd, leave
+// partial lines that other workgroups complete).  The LDS row stride of 66
+// doubles puts the 32 lanes of either half-wave of the row-fastest write (16
+// rows x 2 lines) on distinct banks.
+constexpr int kXtRows = 16, kXtLines = 64, kXtPad = kXtLines + 2, kXtThreads = kXtRows * kXtLines;
+// the outputs pass through LDS in two halves (fields 0..9, then 10..18)
+constexpr int kXtHalf = kUwFields;
+__global__ __launch_bounds__(kXtThreads) void k_uw_split_xt(FlowArgs a, UwArgs u) {
+  __shared__ double sb[kXtHalf][kXtRows * kXtPad];
+  const int NR = a.n[0] + 9;
+  const long long NL = uw_lines(a.n, 0);
+  const int nrt = (NR + kXtRows - 1) / kXtRows;
+  const int rt = (int)(blockIdx.x % nrt);
+  const long long lt = blockIdx.x / nrt;
+  // compute in the row-fastest order
+  const int ri = threadIdx.x % kXtRows, li = threadIdx.x / kXtRows;
+  const int rr = rt * kXtRows + ri;
+  const long long L = lt * kXtLines + li;
+  const bool in = rr < NR && L < NL;
+  double out[kUwFields + 9];
+  if (in) split_point<0>(a, u, L, rr, out);
+  // store in the lines-fastest order
+  const int lo = threadIdx.x % kXtLines, ro = threadIdx.x / kXtLines;
+  const int rro = rt * kXtRows + ro;
+  const long long Lo = lt * kXtLines + lo;
+  const bool ino = rro < NR && Lo < NL;
+  double* sw = u.sw + (long long)rro * NL + Lo;
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    const int m0 = h ? kXtHalf : 0, m1 = h ? kUwFields + 9 : kXtHalf;
+    if (h) __syncthreads();  // the first half's reads are done
+    if (in) {
+#pragma unroll
+      for (int m = m0; m < m1; m++) sb[m - m0][ri * kXtPad + li] = out[m];
+    }
+    __syncthreads();
+    if (ino) {
+#pragma unroll
+      for (int m = m0; m < m1; m++) sw[m * u.sws] = sb[m - m0][ro * kXtPad + lo];
+    }
   }
 }
 
@@ -1098,6 +1160,13 @@ hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st) 
 }
 
 hipError_t uw_launch_split(const FlowArgs& a, const UwArgs& u, int dir, hipStream_t st) {
+  if (dir == 0 && u.compact && u.chardec) {
+    const long long nb = (long long)((a.n[0] + 9 + kXtRows - 1) / kXtRows) *
+                         ((uw_lines(a.n, 0) + kXtLines - 1) / kXtLines);
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_uw_split_xt, dim3((unsigned)nb), dim3(kXtThreads), 0, st, a, u);
+    return hipGetLastError();
+  }
   const dim3 g((unsigned)((uw_points(a.n, dir) + 255) / 256));
   if (dir == 0) hipLaunchKernelGGL(k_uw_split<0>, g, dim3(256), 0, st, a, u);
   else if (dir == 1) hipLaunchKernelGGL(k_uw_split<1>, g, dim3(256), 0, st, a, u);
