@@ -2158,15 +2158,16 @@ hipError_t launch_batch(int shape, int prog, int minb, const BatchLaunch& bl, in
   }
 }
 
-// the eight pass slots of a stage: GC x,y,z; V x,y,z; F x,y.  The program a
-// case of nd directions runs in slot (kind, d), or -1.
+// the eight pass slots of a stage: GC x,y,z; V x,y,z; F y,x (the filter
+// sweeps in the single-case stage's order).  The program a case of nd
+// directions runs in slot (kind, d), or -1.
 int slot_prog(int kind, int d, int nd) {
   if (kind == 0) return d < nd ? (d == nd - 1 ? GC_LAST : (d == 0 ? GC_FIRST : GC_MID)) : -1;
   if (kind == 1) return d < nd ? (d == nd - 1 ? V_LAST_RK : V_MID) : -1;
   return d < nd - 1 ? F_ONLY : -1;
 }
 const int kSlotKind[8] = {0, 0, 0, 1, 1, 1, 2, 2};
-const int kSlotDir[8] = {0, 1, 2, 0, 1, 2, 0, 1};
+const int kSlotDir[8] = {0, 1, 2, 0, 1, 2, 1, 0};
 
 struct BatchGroup {
   int slot, prog, dir, shape, minb, nblocks;
