@@ -394,9 +394,11 @@ class Flow:
         _chk(lib().astr_hip_set_last_split(self.h, 1 if on else 0))
 
     def set_filter_fusion(self, on):
-        """rk3 steps: run each stage's x filter sweep inside the next stage's
-        first pass within a call (on) or as its own pass (off, the default;
-        autotune picks the faster)."""
+        """rk3 steps: fuse filter sweeps into neighbouring passes within a
+        call.  Compact-scheme stage: each stage's x sweep inside the next
+        stage's first pass (off by default; autotune picks the faster).  Main
+        path: the next stage's sweep along the last direction inside each RK
+        pass (on by default)."""
         _chk(lib().astr_hip_set_filter_fusion(self.h, 1 if on else 0))
 
 

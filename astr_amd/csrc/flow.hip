@@ -462,6 +462,9 @@ struct astr_hip_ctx {
   // sweep of the stage just ended is still owed
   int fuse_fx = 0;
   bool fx_pending = false;
+  // main path: each stage's RK pass also applies the next stage's filterq
+  // sweep along its direction (the last one) inside one rk3_steps call
+  int fuse_main_filter = 1;
   double* base = nullptr;   // one allocation for all fields
   double* coef = nullptr;   // LU tables
   double* stats_part = nullptr;
@@ -692,10 +695,11 @@ int timed(astr_hip_ctx* c, int slot, F&& launch) {
 constexpr int kSlotUwSplit = 27, kSlotUwFace = 30, kSlotUwSensor = 33, kSlotFgcFirst = 36;
 static_assert(kSlotFgcFirst + 3 == ASTR_PROF_SLOTS, "profile slots");
 
-int pass(astr_hip_ctx* c, int prog, int d) {
+// shape >= 0 overrides the pass's launch shape
+int pass(astr_hip_ctx* c, int prog, int d, int shape = -1) {
   const bool rec = c->prof && c->ev_used < (int)c->ev_kind.size();
   if (rec) HIP_OK(hipEventRecord(c->ev[2 * c->ev_used], c->stream));
-  HIP_OK(launch_pass(c->shape[prog][d], prog, c->minb[prog][d], c->a, d, c->stream));
+  HIP_OK(launch_pass(shape >= 0 ? shape : c->shape[prog][d], prog, c->minb[prog][d], c->a, d, c->stream));
   if (rec) {
     HIP_OK(hipEventRecord(c->ev[2 * c->ev_used + 1], c->stream));
     c->ev_kind[c->ev_used++] = prog < FGC_FIRST ? prog * 3 + d : kSlotFgcFirst + d;
@@ -1146,18 +1150,31 @@ int rk3_run(astr_hip_ctx* c, int nsteps, const StatsBufs& sb, double* ke, double
   const int stats_prog = c->split_last ? GL_GRAD : GC_LAST;
   const int ntl = tiles_of(a.n, a.ndir - 1, kShapes[c->shape[stats_prog][a.ndir - 1]].TL);
   int rc = 0;
+  // main path: the last direction's filter sweep of this stage was applied by
+  // the previous stage's RK pass (a.rk_filter)
+  bool last_filtered = false;
   for (int it = 0; it < nsteps && !rc; it++) {
     for (int s = 0; s < 3 && !rc; s++) {
       a.stage = s;
       a.rk_a = rk[s][0]; a.rk_b = rk[s][1]; a.rk_c = rk[s][2];
       if (a.main_sem) {
-        for (int d = 0; d < a.ndir && !rc; d++)
+        // filterq's sweeps act on different indices and are linear, so they
+        // run last direction first: from the previous RK pass, or here on
+        // that pass's launch shape, which makes both round alike (a call
+        // split in two gives the bits of the whole)
+        const int dl = a.ndir - 1;
+        if (!last_filtered && pass(c, F_ONLY, dl, c->shape[V_LAST_RK][dl])) rc = -1;
+        for (int d = 0; d < dl && !rc; d++)
           if (pass(c, F_ONLY, d)) rc = -1;
         if (!rc && qswap_main(c, true)) rc = -1;
         if (rc) break;
       }
-      if (run_rhs(c, true, want && s == 0, false,
-                  diss && s == 0 ? sb.out + kStepStats * it + 2 : nullptr)) {
+      a.rk_filter = a.main_sem && c->fuse_main_filter && !(it == nsteps - 1 && s == 2) ? 1 : 0;
+      const int rhs_rc = run_rhs(c, true, want && s == 0, false,
+                                 diss && s == 0 ? sb.out + kStepStats * it + 2 : nullptr);
+      last_filtered = a.rk_filter != 0;
+      a.rk_filter = 0;
+      if (rhs_rc) {
         rc = -1;
         break;
       }
@@ -1795,6 +1812,7 @@ int astr_hip_set_last_split(astr_hip_ctx* c, int split) {
 int astr_hip_set_filter_fusion(astr_hip_ctx* c, int on) {
   if (check_ctx(c)) return -1;
   c->fuse_fx = on ? 1 : 0;
+  c->fuse_main_filter = on ? 1 : 0;
   return 0;
 }
 

@@ -59,6 +59,9 @@ struct FlowArgs {
   // main path: the convective term is the upwind-biased one (upwind.hip),
   // which the gradient passes then leave out
   int uwd;
+  // main path: the RK pass also applies the next stage's filterq sweep along
+  // its direction to the new q, after updatefvar took the primitives
+  int rk_filter;
 };
 
 // Halo sources kept by qswap for the lines whose halo the reference fills
@@ -977,6 +980,30 @@ __device__ __forceinline__ void pass_body(const FlowArgs& a, const StageArgs& sa
           stg(a.pr + 3 * fs, o, P.w);
           stg(a.pr + 4 * fs, o, P.p);
           stg(a.pr + 5 * fs, o, P.t);
+        }
+        if (a.rk_filter) {
+          // the next stage's filterq sweep along dir (comsolver.F90:514-632),
+          // as the filter pass does it, on the new q held in registers; the
+          // primitives above stay those of the unfiltered q, as after the
+          // reference's filterq.  The barrier before the solve orders the
+          // staging below after every read of the viscous part's LDS.
+          const double* csf = stage_chunk_coefs<C>(a.fac[dir], n + 7, smem, 1);
+          const double c3l = a.fac[dir][2 * (n + 7) + (n + 7) - 1];
+          const FilterCoef fl = stage_halo_coefs<C>(a.fc, smem);
+#pragma unroll
+          for (int m = 0; m < 5; m++) {
+#pragma unroll
+            for (int t = 0; t < NPT; t++) put<C>(T, S, t, qn[m][t]);
+            __syncthreads();
+            {
+              ChunkCoef<C::R, true> CF;
+              lds_chunk_coef_f<C>(CF, csf, p * C::R, n + 7, c3l);
+              solve_pass_line<C, true, GEN>(Tl, Xl, n, p, CF, fl, a.bnd[dir]);
+            }
+            __syncthreads();
+#pragma unroll
+            for (int t = 0; t < NPT; t++) stg(a.q + m * fs, S.g[t], X[S.s[t]]);
+          }
         }
       }
     } else {

@@ -249,14 +249,18 @@ int astr_hip_autotune(astr_hip_ctx *ctx, int reps);
  * it from the timings. */
 int astr_hip_set_last_split(astr_hip_ctx *ctx, int split);
 /* Filter-sweep fusion of astr_hip_rk3_steps* (no reference counterpart).
- * filterq's three sweeps act on different indices and are linear, so they
- * commute: a stage ends with z (in the RK pass), then y, then x.  With
- * on != 0 (the compact-scheme solver's stage only), the x sweep of every
- * stage but a call's last runs inside the next stage's first
- * convective+gradient pass (program 9), which loads the same x-lines; the
- * state between calls is the same up to rounding either way.  Off by
- * default; astr_hip_autotune turns it on when its timing of program 9 is
- * below those of programs 0 and 6 along x together. */
+ * filterq's sweeps act on different indices and are linear, so they commute.
+ * - Compact-scheme solver's stage: it ends with z (in the RK pass), then y,
+ *   then x.  With on != 0 the x sweep of every stage but a call's last runs
+ *   inside the next stage's first convective+gradient pass (program 9),
+ *   which loads the same x-lines.  Off by default; astr_hip_autotune turns
+ *   it on when its timing of program 9 is below those of programs 0 and 6
+ *   along x together.
+ * - Main path (ASTR_SEM_MAIN), whose stage starts with filterq: with
+ *   on != 0 (the default) every RK pass but a call's last also applies the
+ *   next stage's sweep along the last direction to the new q, after
+ *   updatefvar took its primitives, and that stage's filterq skips it.
+ * The state between calls is the same up to rounding either way. */
 int astr_hip_set_filter_fusion(astr_hip_ctx *ctx, int on);
 int astr_hip_get_last_split(astr_hip_ctx *ctx, int *split);
 int astr_hip_get_pass_config(astr_hip_ctx *ctx, int prog, int dir, int *shape, int *minb, float *ms);

@@ -344,3 +344,32 @@ def test_channel_every_launch_selection():
             assert max(state_err(f.download(), q_ref)) < RHS_TOL, (shape, minb)
             tried += 1
     assert tried >= 4
+
+
+@pytest.mark.parametrize("channel", [False, True])
+def test_main_filter_fusion(channel):
+    """The next stage's filter sweep along the last direction applied inside
+    each RK pass (the default) and as its own pass on the RK pass's launch
+    shape give the same bits, periodic and with a wall-bounded direction,
+    and both agree with the oracle; a 3-step call then launches the last
+    direction's filter pass once (its first stage) instead of 9 times."""
+    out = []
+    for fuse in (True, False):
+        if channel:
+            f, s, _, _ = channel_pair()
+        else:
+            f, s, _ = make_pair(24, 20, 16)
+        f.set_filter_fusion(fuse)
+        f.profile_begin(400)
+        f.rk3(3, stats=False)
+        prof = f.profile_end()
+        s.rk3(3)
+        qd, prd = f.download(prims=True)
+        qo, pro = s.get()
+        assert max(state_err(qd, qo)) < RHS_TOL, fuse
+        assert relmax(prd, pro) < RHS_TOL, fuse
+        out.append((prof, qd, prd))
+    (fused, qa, pa), (plain, qb, pb) = out
+    assert np.array_equal(qa, qb) and np.array_equal(pa, pb)
+    assert fused[("filter", "z")][1] == 1 and plain[("filter", "z")][1] == 9
+    assert fused[("filter", "x")][1] == 9 and plain[("filter", "x")][1] == 9
