@@ -918,6 +918,42 @@ __global__ __launch_bounds__(256) void k_cuw_diff(FlowArgs a, UwArgs u, int firs
     face_update1(a, m, o, u.fhc[m * u.sws + c], u.fhc[m * u.sws + cp], first, last);
 }
 
+// k_cuw_diff along x through LDS: the interface fluxes are held with the
+// lines fastest and qrhs with the points of a line contiguous.  A workgroup
+// takes 16 points of 64 x-lines: it reads the flux rows t0-1 .. t0+15 with
+// the lines fastest into LDS (row stride 66 doubles, so that the 32 lanes of
+// a half-wave reading by points, 16 points x 2 lines, fall on distinct banks)
+// and updates qrhs with the points fastest.
+constexpr int kDtRows = 16, kDtLines = 64, kDtPad = kDtLines + 2, kDtThreads = kDtRows * kDtLines;
+__global__ __launch_bounds__(kDtThreads) void k_cuw_diff_xt(FlowArgs a, UwArgs u, int first, int last) {
+  __shared__ double sb[5][(kDtRows + 1) * kDtPad];
+  const int n = a.n[0];
+  const int NR = n + 9;
+  const long long NL = uw_lines(a.n, 0);
+  const int ntt = (n + 1 + kDtRows - 1) / kDtRows;
+  const int t0 = (int)(blockIdx.x % ntt) * kDtRows;
+  const long long L0 = (long long)(blockIdx.x / ntt) * kDtLines;
+  for (int e = threadIdx.x; e < (kDtRows + 1) * kDtLines; e += kDtThreads) {
+    const int rr = e / kDtLines, l = e % kDtLines;
+    const int t = t0 - 1 + rr;  // -1 .. n: the rows fh(-1:n) holds
+    if (t <= n && L0 + l < NL) {
+      const long long c = cuw_at<0>(L0 + l, t, NR, NL);
+#pragma unroll
+      for (int m = 0; m < 5; m++) sb[m][rr * kDtPad + l] = u.fhc[m * u.sws + c];
+    }
+  }
+  __syncthreads();
+  const int ti = threadIdx.x % kDtRows, l = threadIdx.x / kDtRows;
+  const int t = t0 + ti;
+  if (t <= n && L0 + l < NL) {
+    int c3[3];
+    const long long o = line_base(a, 0, L0 + l, c3) + t;
+#pragma unroll
+    for (int m = 0; m < 5; m++)
+      face_update1(a, m, o, sb[m][(ti + 1) * kDtPad + l], sb[m][ti * kDtPad + l], first, last);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // The compact scheme without the characteristic projection, one pass per
 // direction with every line in LDS: per split component, the upwind and the
@@ -1143,6 +1179,12 @@ hipError_t cuw_launch_dir(const FlowArgs& a, const UwArgs& u, bool first, bool l
   else hipLaunchKernelGGL((k_cuw_face<DIR, false>), gf, dim3(256), 0, st, a, u);
   e = hipGetLastError();
   if (e != hipSuccess) return e;
+  if (DIR == 0) {
+    const long long nb = (long long)((a.n[0] + 1 + kDtRows - 1) / kDtRows) * ((NL + kDtLines - 1) / kDtLines);
+    if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cuw_diff_xt, dim3((unsigned)nb), dim3(kDtThreads), 0, st, a, u, (int)first, (int)last);
+    return hipGetLastError();
+  }
   const dim3 gd((unsigned)(((a.n[DIR] + 1) * NL + 255) / 256));
   hipLaunchKernelGGL(k_cuw_diff<DIR>, gd, dim3(256), 0, st, a, u, (int)first, (int)last);
   return hipGetLastError();
