@@ -414,8 +414,8 @@ __global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
   int rr;
   if (dir == 0) {
     // x-lines row-fastest: for k_uw_face_x's lanes, and for k_cuw_line,
-    // which loads whole lines (the compact scheme's sweeps with the
-    // projection take k_uw_split_xt's order instead)
+    // which loads whole lines (the compact scheme's sweeps take
+    // k_uw_split_xt's order instead)
     L = idx / NR;
     rr = (int)(idx - L * NR);
   } else {
@@ -432,8 +432,8 @@ __global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
     if (m < nf) sw[m * u.sws] = out[m];
 }
 
-// The x split in the lines-fastest order of the compact scheme's sweeps with
-// the projection (row rr of line L at rr*NL + L).  A workgroup takes 16
+// The x split in the lines-fastest order of the compact scheme's sweeps
+// (row rr of line L at rr*NL + L).  A workgroup takes 16
 // rows of 64 x-lines: it reads q with the rows fastest (each line's 16
 // points are contiguous in memory), stages the 19 outputs in LDS and writes
 // them with the lines fastest (64 lines of a row are contiguous in the split
@@ -442,7 +442,8 @@ __global__ __launch_bounds__(256) void k_uw_split(FlowArgs a, UwArgs u) {
 // doubles puts the 32 lanes of either half-wave of the row-fastest write (16
 // rows x 2 lines) on distinct banks.
 constexpr int kXtRows = 16, kXtLines = 64, kXtPad = kXtLines + 2, kXtThreads = kXtRows * kXtLines;
-// the outputs pass through LDS in two halves (fields 0..9, then 10..18)
+// the outputs pass through LDS in two halves (fields 0..9, then, with the
+// projection, 10..18)
 constexpr int kXtHalf = kUwFields;
 __global__ __launch_bounds__(kXtThreads) void k_uw_split_xt(FlowArgs a, UwArgs u) {
   __shared__ double sb[kXtHalf][kXtRows * kXtPad];
@@ -466,6 +467,7 @@ __global__ __launch_bounds__(kXtThreads) void k_uw_split_xt(FlowArgs a, UwArgs u
   double* sw = u.sw + (long long)rro * NL + Lo;
 #pragma unroll
   for (int h = 0; h < 2; h++) {
+    if (h && !u.chardec) break;  // uniform: the buffer holds 10 fields
     const int m0 = h ? kXtHalf : 0, m1 = h ? kUwFields + 9 : kXtHalf;
     if (h) __syncthreads();  // the first half's reads are done
     if (in) {
@@ -1153,6 +1155,15 @@ using CuwCfg32 = LineCfg<256, 8, 5, 5>;   // P = 32: n <= 156
 using CuwCfg64 = LineCfg<512, 8, 5, 5>;   // P = 64: n <= 316
 using CuwCfg64b = LineCfg<512, 8, 9, 9>;  // P = 64, 9 rows per chunk: n <= 572
 
+// The compact scheme along dir runs the three-kernel sweeps (one lane per
+// line and component, the split buffer holding the lines fastest) rather
+// than the LDS-resident pass: with the characteristic projection, or for
+// lines longer than the LDS pass holds (the flow passes stop at 569 points,
+// below its 572).  The x split and the face launch both follow it.
+bool cuw_sweeps(const FlowArgs& a, const UwArgs& u, int dir) {
+  return u.compact && (u.chardec || a.n[dir] + 4 > CuwCfg64b::P * CuwCfg64b::R);
+}
+
 template <class C, int DIR>
 hipError_t cuw_line_launch(const FlowArgs& a, const UwArgs& u, bool first, bool last, hipStream_t st) {
   const long long NL = uw_lines(a.n, DIR);
@@ -1163,7 +1174,7 @@ hipError_t cuw_line_launch(const FlowArgs& a, const UwArgs& u, bool first, bool 
 
 template <int DIR>
 hipError_t cuw_launch_dir(const FlowArgs& a, const UwArgs& u, bool first, bool last, hipStream_t st) {
-  if (!u.chardec) {
+  if (!cuw_sweeps(a, u, DIR)) {
     const int n = a.n[DIR];
     if (n + 4 <= CuwCfg16::P * CuwCfg16::R) return cuw_line_launch<CuwCfg16, DIR>(a, u, first, last, st);
     if (n + 4 <= CuwCfg32::P * CuwCfg32::R) return cuw_line_launch<CuwCfg32, DIR>(a, u, first, last, st);
@@ -1202,7 +1213,7 @@ hipError_t uw_launch_sensor(const FlowArgs& a, const UwArgs& u, hipStream_t st) 
 }
 
 hipError_t uw_launch_split(const FlowArgs& a, const UwArgs& u, int dir, hipStream_t st) {
-  if (dir == 0 && u.compact && u.chardec) {
+  if (dir == 0 && cuw_sweeps(a, u, 0)) {
     const long long nb = (long long)((a.n[0] + 9 + kXtRows - 1) / kXtRows) *
                          ((uw_lines(a.n, 0) + kXtLines - 1) / kXtLines);
     if (nb > 0x7fffffffLL) return hipErrorInvalidValue;
