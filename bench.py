@@ -393,7 +393,7 @@ def main_path_step(n, dt, steps, warmup):
 
     f = A.Flow(A.main_config(n, deltat=dt))
     f.init_tgv(1e-3, 2024)
-    f.autotune(reps=3)
+    f.autotune(reps=10)
     if warmup:
         f.rk3(warmup, stats=True)
     wall, wall_ev, ke, en, prof = timed_steps(f, steps, 3 * 9)
