@@ -1716,6 +1716,9 @@ int astr_hip_autotune(astr_hip_ctx* c, int reps) {
   a.rk_a = 1.0; a.rk_b = 0.0; a.rk_c = 1.0;
   a.dostats = 0;
   a.store_last_grad = 0;
+  // the main path's RK pass is timed with the filter sweep it carries in
+  // rk3 steps
+  a.rk_filter = a.main_sem && c->fuse_main_filter ? 1 : 0;
   int rc = 0;
   for (int pr = 0; pr < kNumProgs && !rc; pr++) {
     for (int d = 0; d < a.ndir && !rc; d++) {
